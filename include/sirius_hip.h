@@ -1,0 +1,246 @@
+/*
+ * sirius_hip.h — C-ABI of the MI355X geodesic-tracing backend (libsirius_hip.so).
+ *
+ * Plain C: opaque handle, plain structs of doubles/ints, plain pointers. No HIP, CUDA or
+ * torch types cross this boundary, and no C++ exception escapes it: every entry point that
+ * can fail returns an int status (SIRIUS_HIP_OK = 0) and records a message readable with
+ * sirius_hip_last_error().
+ *
+ * The lifecycle/framebuffer half mirrors the OptiX backend's C API one for one
+ * (src/Sirius.Render/Acceleration/OptiX/RDOP006A.h:10-39, implemented at
+ * src/Sirius.Render/Acceleration/OptiX/RDOP001A.cu:1367-1458); the physics half
+ * (sirius_hip_trace_batch and friends) is new: the reference has no batched, per-ray physics
+ * entry point.
+ *
+ * Conventions (G = c = M_unit = 1):
+ *  - Metric families follow LaunchConfig::metricFamily
+ *    (src/Sirius.Render/Acceleration/Backend/ACIB001A.h:89): 0 Kerr-Schild (M,a,Q,Lambda),
+ *    1 Morris-Thorne, 2 Alcubierre.
+ *  - Cameras are pinhole cameras as PinholeCamera::generateRay
+ *    (src/Sirius.Core/Camera/CMBS001A.h:120-185); the observer sits at Boyer-Lindquist
+ *    (r_obs, theta_obs, phi_obs), mapped to Cartesian with the flat map the CPU tracer uses
+ *    (src/Sirius.Core/Coordinate/PHCT002A.h:129-140).
+ *  - Rays of a batch are numbered camera by camera, row-major inside a camera:
+ *    ray = offset(camera) + y * width + x.
+ */
+#ifndef SIRIUS_HIP_H
+#define SIRIUS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SIRIUS_HIP_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------------------ */
+enum {
+    SIRIUS_HIP_OK = 0,
+    SIRIUS_HIP_E_INVALID = -1,     /* bad argument (null pointer, size, enum)            */
+    SIRIUS_HIP_E_DEVICE = -2,      /* HIP runtime error or no usable gfx950 device         */
+    SIRIUS_HIP_E_NOMEM = -3,       /* device allocation failed                             */
+    SIRIUS_HIP_E_STATE = -4,       /* call out of order (not initialised, no network, ...) */
+    SIRIUS_HIP_E_UNSUPPORTED = -5  /* option combination not implemented                   */
+};
+
+/* ---- enums ---------------------------------------------------------------------------- */
+enum { SIRIUS_METRIC_KERR_SCHILD = 0, SIRIUS_METRIC_MORRIS_THORNE = 1, SIRIUS_METRIC_ALCUBIERRE = 2 };
+enum { SIRIUS_INTEGRATOR_DP45 = 0, SIRIUS_INTEGRATOR_SYMPLECTIC = 1 };
+enum { SIRIUS_PRECISION_FP64 = 0, SIRIUS_PRECISION_FP32_ESCALATE = 1 };
+enum { SIRIUS_EMISSION_NONE = 0, SIRIUS_EMISSION_THIN_DISK = 1, SIRIUS_EMISSION_VOLUMETRIC = 2 };
+enum { SIRIUS_SCHEDULE_GRID = 0, SIRIUS_SCHEDULE_PERSISTENT = 1 };
+
+/* Per-ray termination class. Values 0-4 keep TraceResult::Outcome's order
+ * (src/Sirius.Render/Integration/GTRC001A.h:46-52); 5 is new. */
+enum {
+    SIRIUS_RAY_ESCAPED = 0,   /* reached r_escape moving outward                           */
+    SIRIUS_RAY_CAPTURED = 1,  /* reached r < horizon_factor * r_+                          */
+    SIRIUS_RAY_DISK = 2,      /* hit the emitting disk (thin disk)                         */
+    SIRIUS_RAY_MAX_STEPS = 3, /* step budget exhausted                                     */
+    SIRIUS_RAY_SPIRAL = 4,    /* reserved (reference's SPIRALING); never produced          */
+    SIRIUS_RAY_FAILED = 5     /* NaN/Inf state or step size below h_min                    */
+};
+/* For two-sided wormholes, ESCAPED rays carry the exit side in flags bit 4. */
+
+/* RayResult flags (bit mask) */
+enum {
+    SIRIUS_FLAG_ESCALATED = 1,   /* the FP32 path handed this ray to FP64                  */
+    SIRIUS_FLAG_SURROGATE = 2,   /* class/direction came from the MLP, not the integrator  */
+    SIRIUS_FLAG_RENORM = 4,      /* the null constraint was re-imposed at least once       */
+    SIRIUS_FLAG_AUDITED = 8,     /* ray belongs to the random audit set                    */
+    SIRIUS_FLAG_FAR_SIDE = 16    /* wormhole: escaped on the far side of the throat        */
+};
+
+/* ---- descriptors ---------------------------------------------------------------------- */
+typedef struct SiriusHipMetric {
+    int32_t family;      /* SIRIUS_METRIC_*                                               */
+    int32_t reserved0;
+    double M, a, Q, Lambda;             /* Kerr-Schild family, PHMT100A.h:54-67            */
+    double throat_b0;                   /* Morris-Thorne throat, b(r) = b0^2 / r           */
+    double warp_v, warp_R, warp_sigma;  /* Alcubierre bubble (static, centred at origin)   */
+} SiriusHipMetric;
+
+typedef struct SiriusHipCamera {
+    double r_obs, theta_obs, phi_obs;   /* observer, Boyer-Lindquist (CMBS001A.h:37-41)    */
+    double fov_deg;                     /* vertical field of view (CMBS001A.h:49)          */
+    double yaw, pitch, roll;            /* radians (CMBS001A.h:44-46)                      */
+    double disk_r_in, disk_r_out;       /* thin disk bounds; disk_r_in <= 0 means ISCO     */
+    int32_t width, height;
+    int32_t metric_index;               /* index into the metric array                     */
+    int32_t reserved0;
+} SiriusHipCamera;
+
+typedef struct SiriusHipTraceOpts {
+    int32_t integrator;      /* SIRIUS_INTEGRATOR_*                                         */
+    int32_t precision;       /* SIRIUS_PRECISION_*                                          */
+    int32_t emission;        /* SIRIUS_EMISSION_*                                           */
+    int32_t schedule;        /* SIRIUS_SCHEDULE_*                                           */
+    double rtol, atol;       /* DP45 tolerances (IntegratorConfig, PHGD001A.h:48-58)        */
+    double h_init, h_min, h_max;
+    double far_slope;        /* FP32 path: h_max(r) = max(h_max, far_slope * r); 0 = off    */
+    double symplectic_h;     /* fixed step of the symplectic integrator                     */
+    int32_t symplectic_order;/* 2 or 4 (Yoshida composition)                                */
+    int32_t max_steps;       /* step attempts per ray                                       */
+    double r_escape;         /* escape radius                                               */
+    double horizon_factor;   /* capture at r < horizon_factor * r_+                         */
+    int32_t n_bins;          /* spectral bins, log-spaced in h nu / k T_max                 */
+    int32_t use_surrogate;   /* 1: MLP far-field surrogate with integrator fallback         */
+    double x_min, x_max;     /* bin range in h nu / k T_max                                 */
+    double escalate_margin;  /* FP32 path: escalate inside r < escalate_margin * r_ph(retro)*/
+    double surrogate_band;   /* fallback band around the critical curve (M)                 */
+    double surrogate_pmin;   /* fallback when max class probability is below this           */
+    double audit_fraction;   /* fraction of surrogate rays re-traced in FP64                */
+    uint64_t audit_seed;
+    int32_t block_size;      /* 0: autotuned                                                */
+    int32_t blocks_per_cu;   /* 0: autotuned                                                */
+    int32_t refill_min;      /* persistent queue: refill a wave when >= this many lanes idle*/
+    int32_t reserved0;
+    /* volumetric emission (emitter frame): j_nu = j0 r^-2 exp(-z^2 / (2 (h_over_r r)^2))
+     * * x^3 / (e^{x / tau_v(r)} - 1), tau_v = (6 / r)^{3/4}; alpha = kappa j0 r^-2 exp(...) */
+    double vol_j0, vol_h_over_r, vol_kappa, vol_r_min, vol_r_max, vol_tau_max;
+} SiriusHipTraceOpts;
+
+/* Structure-of-arrays result buffers. Every array holds n_rays entries, except intensity
+ * (n_bins * n_rays, bin-major: intensity[b * n_rays + ray]) and state (8 * n_rays, optional).
+ * on_device != 0: non-null pointers are device memory written in place, and null ones leave
+ * that result in library-owned device memory (fetch it with sirius_hip_fetch_results);
+ * on_device == 0: non-null pointers are host memory (the library stages through device
+ * scratch and copies back before returning), null ones are skipped. */
+typedef struct SiriusHipRayOutput {
+    uint8_t* cls;        /* SIRIUS_RAY_*                                                   */
+    uint8_t* flags;      /* SIRIUS_FLAG_*                                                  */
+    float* theta;        /* escape direction polar angle of k^i (rad)                       */
+    float* phi;          /* escape direction azimuth of k^i (rad)                           */
+    float* lambda;       /* affine length travelled                                        */
+    float* disk_r;       /* Boyer-Lindquist radius of the disk hit                          */
+    float* redshift;     /* g = nu_obs / nu_emit at the disk hit                            */
+    float* null_drift;   /* |g_{mu nu} k^mu k^nu| at termination (evaluated in FP64)        */
+    uint32_t* steps;     /* step attempts (accepted + rejected)                             */
+    float* intensity;    /* n_bins * n_rays                                                 */
+    double* state;       /* optional final (x^mu, p_mu), 8 * n_rays, ray-major              */
+    int32_t on_device;
+    int32_t reserved0;
+} SiriusHipRayOutput;
+
+typedef struct SiriusHipStats {
+    uint64_t rays;
+    uint64_t steps_accepted;
+    uint64_t steps_rejected;
+    uint64_t steps_landing;      /* event-location steps (disk, horizon, escape sphere)     */
+    uint64_t rays_escalated;     /* FP32 rays finished in FP64                              */
+    uint64_t rays_surrogate;     /* rays answered by the MLP                                */
+    uint64_t rays_fallback;      /* rays the gate sent back to the integrator               */
+    uint64_t rays_audited;
+    uint64_t steps_bulk;         /* step attempts made by the bulk trace kernel             */
+    uint64_t steps_escalated;    /* step attempts made by the FP64 escalation kernel        */
+    double kernel_ms;            /* HIP-event time of the bulk trace kernel                 */
+    double escalate_ms;          /* HIP-event time of the FP64 escalation kernel            */
+    double surrogate_ms;         /* HIP-event time of the MLP inference + gate              */
+    double total_ms;             /* HIP-event time of the whole batch                       */
+    double flops;                /* algorithmic flops of the bulk kernel (fixed per step)   */
+    double flops_escalated;      /* algorithmic flops of the escalation kernel              */
+    double null_drift_max;
+    double null_drift_median;
+} SiriusHipStats;
+
+typedef void* SiriusHipHandle;
+
+/* ---- lifecycle: mirrors sirius_optix_* (RDOP006A.h:14-19) ----------------------------- */
+SiriusHipHandle sirius_hip_create(void);
+void sirius_hip_destroy(SiriusHipHandle h);
+/* Allocates a width*height float4 framebuffer (IAccelerator::initialise, ACIB001A.h:160). */
+int sirius_hip_initialize(SiriusHipHandle h, int width, int height);
+void sirius_hip_cleanup(SiriusHipHandle h);
+int sirius_hip_is_initialized(SiriusHipHandle h);
+const char* sirius_hip_last_error(SiriusHipHandle h);
+int sirius_hip_abi_version(void);
+/* Device name, memory, CU count, FP64 support (DeviceCapabilities, ACIB001A.h:50-59). */
+int sirius_hip_device_info(SiriusHipHandle h, char* name, int name_len, size_t* total_mem,
+                           size_t* free_mem, int* cu_count, int* max_threads_per_block);
+
+/* ---- rendering: mirrors sirius_optix_launch / get_frame_buffer (RDOP006A.h:22-29) ------ */
+/* Traces one camera and shades the RGBA framebuffer (synchronous, like RDOP001A.cu:1044). */
+int sirius_hip_launch(SiriusHipHandle h, const SiriusHipCamera* cam, const SiriusHipMetric* metric,
+                      const SiriusHipTraceOpts* opts);
+int sirius_hip_resize(SiriusHipHandle h, int width, int height);
+/* Host RGBA float pointer (width*height*4); the caller does not own it (ACIB001A.h:169-170). */
+float* sirius_hip_get_frame_buffer(SiriusHipHandle h);
+size_t sirius_hip_get_frame_buffer_size(SiriusHipHandle h);
+/* Equirectangular RGBA8 background (IAccelerator::uploadBackground, ACIB001A.h:176). */
+int sirius_hip_upload_background(SiriusHipHandle h, const unsigned char* rgba, int width, int height);
+int sirius_hip_synchronize(SiriusHipHandle h);
+/* Copies the most recent trace_batch results held in library device memory into host
+ * buffers (host pointers; null ones skipped). */
+int sirius_hip_fetch_results(SiriusHipHandle h, SiriusHipRayOutput* host_out);
+
+/* ---- physics: batched trace (new; no reference counterpart) --------------------------- */
+/* Fills opts with the defaults of the given precision (FP64 reference contract or FP32). */
+void sirius_hip_default_opts(SiriusHipTraceOpts* opts, int precision);
+/* Traces every pixel of n_cam cameras. metrics[cam.metric_index] gives each camera's
+ * spacetime. Returns SIRIUS_HIP_OK or an error code; stats may be null. */
+int sirius_hip_trace_batch(SiriusHipHandle h, const SiriusHipCamera* cams, int n_cam,
+                           const SiriusHipMetric* metrics, int n_metric,
+                           const SiriusHipTraceOpts* opts, SiriusHipRayOutput* out,
+                           SiriusHipStats* stats);
+/* Distance of each ray's (alpha, beta) from the critical curve (M; negative inside the
+ * shadow); rays are all pixels of the cameras, output n_rays floats (device or host). */
+int sirius_hip_critical_distance(SiriusHipHandle h, const SiriusHipCamera* cams, int n_cam,
+                                 const SiriusHipMetric* metrics, int n_metric, float* out,
+                                 int on_device);
+
+/* ---- autotuner (new) ------------------------------------------------------------------- */
+/* Times the persistent kernel over a sweep of (block_size, blocks_per_cu, refill_min) on a
+ * sample of the batch and writes the fastest choice into opts. */
+int sirius_hip_autotune(SiriusHipHandle h, const SiriusHipCamera* cams, int n_cam,
+                        const SiriusHipMetric* metrics, int n_metric, SiriusHipTraceOpts* opts,
+                        double* best_ms);
+
+/* ---- MLP far-field surrogate (new) ------------------------------------------------------ */
+/* Network: 5 inputs -> 4 x 256 GELU -> 6 outputs (3 class logits, 3 direction residual).
+ * Weights are FP32 masters on the device; forward/backward run in BF16 MFMA with FP32
+ * accumulation. */
+#define SIRIUS_MLP_IN 5
+#define SIRIUS_MLP_HIDDEN 256
+#define SIRIUS_MLP_LAYERS 4
+#define SIRIUS_MLP_OUT 6
+size_t sirius_hip_mlp_param_count(void);
+int sirius_hip_mlp_init(SiriusHipHandle h, uint64_t seed);
+int sirius_hip_mlp_get_params(SiriusHipHandle h, float* host_params, size_t n);
+int sirius_hip_mlp_set_params(SiriusHipHandle h, const float* host_params, size_t n);
+/* Forward pass on n samples x[n*5] (device), logits+direction y[n*6] (device). */
+int sirius_hip_mlp_forward(SiriusHipHandle h, const float* x, float* y, int n);
+/* Loss and parameter gradient for a batch; targets cls[n] (int32) and dir[n*3];
+ * grad is FP32, length param_count (device). Returns the mean loss in *loss. */
+int sirius_hip_mlp_loss_grad(SiriusHipHandle h, const float* x, const int32_t* cls,
+                             const float* dir, int n, float* grad, float* loss);
+/* One Adam step over a batch (forward + backward + update), on device buffers. */
+int sirius_hip_mlp_train_step(SiriusHipHandle h, const float* x, const int32_t* cls,
+                              const float* dir, int n, float lr, float* loss);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIRIUS_HIP_H */

@@ -1,0 +1,117 @@
+"""ctypes binding of oracle/_ref/libsirius_ref.so (the reference's own CPU code, compiled in
+place by `make -C oracle ref`). TEST INFRASTRUCTURE ONLY: used to make golden vectors and as
+the CPU baseline; the library exists only where the reference tree was present at build time.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_ref", "libsirius_ref.so")
+_lib = None
+
+
+def available():
+    return os.path.exists(LIB_PATH)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = C.CDLL(LIB_PATH)
+        d, dp, i, ip = C.c_double, C.POINTER(C.c_double), C.c_int, C.c_void_p
+        L.ref_ks_evaluate.argtypes = [d, d, d, d, dp, dp, dp]
+        L.ref_mt_evaluate.argtypes = [d, dp, dp, dp]
+        L.ref_warp_evaluate.argtypes = [d, d, d, dp, dp, dp]
+        L.ref_ks_radius.restype = d
+        L.ref_ks_radius.argtypes = [d, d, d, d]
+        L.ref_camera_ray.argtypes = [d, d, d, d, i, i, i, i, dp, dp]
+        L.ref_init_state.argtypes = [d, d, d, d, d, d, d, d, i, i, i, i, dp, dp]
+        L.ref_normalize_null.restype = d
+        L.ref_normalize_null.argtypes = [dp, dp]
+        L.ref_rk45_step.restype = i
+        L.ref_rk45_step.argtypes = [d, d, d, d, dp, dp, dp, d, d, d, d]
+        L.ref_trace.argtypes = [d, d, d, d, d, d, d, d, i, i, ip, C.c_int64, d, d, d, d, d, i,
+                                d, d, d, d, i, i, ip, ip, ip, ip, ip, ip]
+        L.ref_isco.restype = d
+        L.ref_isco.argtypes = [d]
+        L.ref_pt_q.restype = d
+        L.ref_pt_q.argtypes = [d, d]
+        L.ref_circular.argtypes = [d, d, dp, dp, dp]
+        L.ref_symplectic_run.argtypes = [d, d, d, d, d, d, d, i, d, dp, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def ks_evaluate(M, a, Q, Lam, x):
+    g, dg = np.zeros(16), np.zeros(64)
+    x = np.ascontiguousarray(x, np.float64)
+    lib().ref_ks_evaluate(M, a, Q, Lam, _p(x), _p(g), _p(dg))
+    return g.reshape(4, 4), dg.reshape(4, 4, 4)
+
+
+def mt_evaluate(b0, x):
+    g, dg = np.zeros(16), np.zeros(64)
+    x = np.ascontiguousarray(x, np.float64)
+    lib().ref_mt_evaluate(b0, _p(x), _p(g), _p(dg))
+    return g.reshape(4, 4), dg.reshape(4, 4, 4)
+
+
+def warp_evaluate(v, R, sigma, x):
+    g, dg = np.zeros(16), np.zeros(64)
+    x = np.ascontiguousarray(x, np.float64)
+    lib().ref_warp_evaluate(v, R, sigma, _p(x), _p(g), _p(dg))
+    return g.reshape(4, 4), dg.reshape(4, 4, 4)
+
+
+def camera_ray(r, theta, phi, fov, w, h, px, py):
+    o, d = np.zeros(4), np.zeros(4)
+    lib().ref_camera_ray(r, theta, phi, fov, w, h, px, py, _p(o), _p(d))
+    return o, d
+
+
+def init_state(M, a, Q, Lam, r, theta, phi, fov, w, h, px, py):
+    x, k = np.zeros(4), np.zeros(4)
+    lib().ref_init_state(M, a, Q, Lam, r, theta, phi, fov, w, h, px, py, _p(x), _p(k))
+    return x, k
+
+
+def trace(M, a, Q, Lam, r_obs, theta, phi, fov, w, h, pix, rtol=1e-8, atol=1e-10, h0=0.01,
+          hmin=1e-8, hmax=2.0, max_steps=20000, r_cap=2.02, r_esc=1000.0, disk=(6.0, 20.0),
+          use_disk=True, threads=0):
+    pix = np.ascontiguousarray(pix, np.int64)
+    n = len(pix)
+    cls = np.zeros(n, np.int32)
+    steps = np.zeros(n, np.int32)
+    xf, kf = np.zeros((n, 4)), np.zeros((n, 4))
+    dr, lam = np.zeros(n), np.zeros(n)
+    lib().ref_trace(M, a, Q, Lam, r_obs, theta, phi, fov, w, h, pix.ctypes.data, n, rtol, atol,
+                    h0, hmin, hmax, max_steps, r_cap, r_esc, disk[0], disk[1], int(use_disk),
+                    threads, cls.ctypes.data, steps.ctypes.data, xf.ctypes.data, kf.ctypes.data,
+                    dr.ctypes.data, lam.ctypes.data)
+    return dict(cls=cls, steps=steps, x=xf, k=kf, disk_r=dr, lam=lam)
+
+
+def isco(a):
+    return lib().ref_isco(a)
+
+
+def pt_q(a, r):
+    return lib().ref_pt_q(a, r)
+
+
+def circular(a, r):
+    E, L, Om = C.c_double(), C.c_double(), C.c_double()
+    lib().ref_circular(a, r, C.byref(E), C.byref(L), C.byref(Om))
+    return E.value, L.value, Om.value
+
+
+def symplectic_run(a, r, th, pr, pth, E, Lz, n, h):
+    mh, de, dl = C.c_double(), C.c_double(), C.c_double()
+    lib().ref_symplectic_run(a, r, th, pr, pth, E, Lz, n, h, C.byref(mh), C.byref(de), C.byref(dl))
+    return mh.value, de.value, dl.value

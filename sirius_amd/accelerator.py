@@ -1,0 +1,210 @@
+"""Python mirror of the HIP backend: the IAccelerator method set
+(src/Sirius.Render/Acceleration/Backend/ACIB001A.h:148-189) plus the batched physics entry
+point, all calls going through the C-ABI of libsirius_hip.so (include/sirius_hip.h).
+
+There is no CPU fallback: if the library is missing or the device is unusable, every call
+raises SiriusHipError.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libsirius_hip.so")
+_lib = None
+
+
+class SiriusHipError(RuntimeError):
+    def __init__(self, rc, msg):
+        super().__init__(f"sirius_hip error {rc}: {msg}")
+        self.rc = rc
+
+
+def lib():
+    """Load libsirius_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SiriusHipError(abi.E_STATE, f"{LIB_PATH} not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    H, i, d = C.c_void_p, C.c_int, C.c_double
+    L.sirius_hip_create.restype = H
+    L.sirius_hip_destroy.argtypes = [H]
+    L.sirius_hip_initialize.argtypes = [H, i, i]
+    L.sirius_hip_cleanup.argtypes = [H]
+    L.sirius_hip_is_initialized.argtypes = [H]
+    L.sirius_hip_last_error.restype = C.c_char_p
+    L.sirius_hip_last_error.argtypes = [H]
+    L.sirius_hip_device_info.argtypes = [H, C.c_char_p, i, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
+                                         C.POINTER(i), C.POINTER(i)]
+    L.sirius_hip_launch.argtypes = [H, C.POINTER(abi.Camera), C.POINTER(abi.Metric), C.POINTER(abi.TraceOpts)]
+    L.sirius_hip_resize.argtypes = [H, i, i]
+    L.sirius_hip_get_frame_buffer.restype = C.POINTER(C.c_float)
+    L.sirius_hip_get_frame_buffer.argtypes = [H]
+    L.sirius_hip_get_frame_buffer_size.restype = C.c_size_t
+    L.sirius_hip_get_frame_buffer_size.argtypes = [H]
+    L.sirius_hip_upload_background.argtypes = [H, C.c_void_p, i, i]
+    L.sirius_hip_synchronize.argtypes = [H]
+    L.sirius_hip_fetch_results.argtypes = [H, C.POINTER(abi.RayOutput)]
+    L.sirius_hip_default_opts.argtypes = [C.POINTER(abi.TraceOpts), i]
+    L.sirius_hip_default_opts.restype = None
+    L.sirius_hip_trace_batch.argtypes = [H, C.POINTER(abi.Camera), i, C.POINTER(abi.Metric), i,
+                                         C.POINTER(abi.TraceOpts), C.POINTER(abi.RayOutput),
+                                         C.POINTER(abi.Stats)]
+    L.sirius_hip_critical_distance.argtypes = [H, C.POINTER(abi.Camera), i, C.POINTER(abi.Metric), i,
+                                               C.c_void_p, i]
+    L.sirius_hip_autotune.argtypes = [H, C.POINTER(abi.Camera), i, C.POINTER(abi.Metric), i,
+                                      C.POINTER(abi.TraceOpts), C.POINTER(d)]
+    L.sirius_hip_mlp_param_count.restype = C.c_size_t
+    L.sirius_hip_mlp_init.argtypes = [H, C.c_uint64]
+    L.sirius_hip_mlp_get_params.argtypes = [H, C.c_void_p, C.c_size_t]
+    L.sirius_hip_mlp_set_params.argtypes = [H, C.c_void_p, C.c_size_t]
+    L.sirius_hip_mlp_forward.argtypes = [H, C.c_void_p, C.c_void_p, i]
+    L.sirius_hip_mlp_loss_grad.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, i, C.c_void_p, C.c_void_p]
+    L.sirius_hip_mlp_train_step.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, i, C.c_float, C.c_void_p]
+    _lib = L
+    return L
+
+
+def default_opts(precision=abi.PRECISION_FP64):
+    o = abi.TraceOpts()
+    lib().sirius_hip_default_opts(C.byref(o), precision)
+    return o
+
+
+class HipAccelerator:
+    """IAccelerator method set over the C-ABI (getType/getCapabilities/initialise/...)."""
+
+    BACKEND_HIP = 3  # BackendType::HIP (ACIB001A.h:28)
+
+    def __init__(self):
+        self._L = lib()
+        self._h = self._L.sirius_hip_create()
+        if not self._h:
+            raise SiriusHipError(abi.E_DEVICE, "sirius_hip_create failed")
+
+    def close(self):
+        if self._h:
+            self._L.sirius_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != abi.OK:
+            raise SiriusHipError(rc, self._L.sirius_hip_last_error(self._h).decode())
+
+    # ---- IAccelerator ----
+    def getType(self):
+        return self.BACKEND_HIP
+
+    def getCapabilities(self):
+        name = C.create_string_buffer(256)
+        tot, fr = C.c_size_t(), C.c_size_t()
+        cu, mt = C.c_int(), C.c_int()
+        self._check(self._L.sirius_hip_device_info(self._h, name, 256, C.byref(tot), C.byref(fr),
+                                                   C.byref(cu), C.byref(mt)))
+        return dict(deviceName=name.value.decode(), totalMemory=tot.value, freeMemory=fr.value,
+                    supportsFP64=True, multiprocessorCount=cu.value, maxThreadsPerBlock=mt.value)
+
+    def initialise(self, width, height):
+        self._check(self._L.sirius_hip_initialize(self._h, width, height))
+        return True
+
+    def isInitialised(self):
+        return bool(self._L.sirius_hip_is_initialized(self._h))
+
+    def launch(self, cam, metric, opts):
+        self._check(self._L.sirius_hip_launch(self._h, C.byref(cam), C.byref(metric), C.byref(opts)))
+
+    def getFrameBuffer(self):
+        p = self._L.sirius_hip_get_frame_buffer(self._h)
+        n = self._L.sirius_hip_get_frame_buffer_size(self._h) // 4
+        if not p:
+            raise SiriusHipError(abi.E_STATE, "no framebuffer (initialise first)")
+        return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+    def getFrameBufferSize(self):
+        return self._L.sirius_hip_get_frame_buffer_size(self._h)
+
+    def uploadBackground(self, rgba):
+        rgba = np.ascontiguousarray(rgba, np.uint8)
+        h, w = rgba.shape[:2]
+        self._check(self._L.sirius_hip_upload_background(self._h, rgba.ctypes.data, w, h))
+        return True
+
+    def synchronise(self):
+        self._check(self._L.sirius_hip_synchronize(self._h))
+
+    def cleanup(self):
+        self._L.sirius_hip_cleanup(self._h)
+
+    def getLastError(self):
+        return self._L.sirius_hip_last_error(self._h).decode()
+
+    # ---- physics ----
+    def trace_batch(self, cams, metrics, opts, with_state=False, device_out=None):
+        """Trace every pixel of the cameras. Returns (abi.HostRays, stats dict), or with
+        device_out (an abi.RayOutput of device pointers, on_device=1) (None, stats)."""
+        cams, metrics = list(cams), list(metrics)
+        ca = (abi.Camera * len(cams))(*cams)
+        ma = (abi.Metric * len(metrics))(*metrics)
+        st = abi.Stats()
+        if device_out is not None:
+            self._check(self._L.sirius_hip_trace_batch(self._h, ca, len(cams), ma, len(metrics),
+                                                       C.byref(opts), C.byref(device_out), C.byref(st)))
+            return None, st.as_dict()
+        n = sum(c.width * c.height for c in cams)
+        out = abi.HostRays(n, opts.n_bins, with_state)
+        ro = out.output()
+        self._check(self._L.sirius_hip_trace_batch(self._h, ca, len(cams), ma, len(metrics),
+                                                   C.byref(opts), C.byref(ro), C.byref(st)))
+        return out, st.as_dict()
+
+    def trace_resident(self, cams, metrics, opts):
+        """Trace with every result left in library device memory (nothing copied back).
+        Returns the stats dict; fetch_results() copies the results afterwards."""
+        cams, metrics = list(cams), list(metrics)
+        ca = (abi.Camera * len(cams))(*cams)
+        ma = (abi.Metric * len(metrics))(*metrics)
+        st = abi.Stats()
+        ro = abi.RayOutput(on_device=1)
+        self._check(self._L.sirius_hip_trace_batch(self._h, ca, len(cams), ma, len(metrics),
+                                                   C.byref(opts), C.byref(ro), C.byref(st)))
+        self._resident_n = sum(c.width * c.height for c in cams)
+        self._resident_bins = opts.n_bins
+        return st.as_dict()
+
+    def fetch_results(self, with_state=False):
+        """Host copy of the results of the last trace_resident call."""
+        out = abi.HostRays(self._resident_n, self._resident_bins, with_state)
+        ro = out.output()
+        self._check(self._L.sirius_hip_fetch_results(self._h, C.byref(ro)))
+        return out
+
+    def critical_distance(self, cams, metrics):
+        cams, metrics = list(cams), list(metrics)
+        ca = (abi.Camera * len(cams))(*cams)
+        ma = (abi.Metric * len(metrics))(*metrics)
+        n = sum(c.width * c.height for c in cams)
+        out = np.zeros(n, np.float32)
+        self._check(self._L.sirius_hip_critical_distance(self._h, ca, len(cams), ma, len(metrics),
+                                                         out.ctypes.data, 0))
+        return out
+
+    def autotune(self, cams, metrics, opts):
+        cams, metrics = list(cams), list(metrics)
+        ca = (abi.Camera * len(cams))(*cams)
+        ma = (abi.Metric * len(metrics))(*metrics)
+        best = C.c_double()
+        self._check(self._L.sirius_hip_autotune(self._h, ca, len(cams), ma, len(metrics),
+                                                C.byref(opts), C.byref(best)))
+        return best.value

@@ -1,0 +1,523 @@
+// geodesic.hpp — device-side physics of the MI355X geodesic backend, templated on the scalar
+// type (double for the FP64 path, float for the FP32 bulk path).
+//
+// One lane owns one ray; every quantity below lives in VGPRs. Reference lines each part
+// reproduces (paths relative to the reference's src/):
+//   Kerr-Schild metric and gradient  Sirius.Core/Metric/PHMT100A.h:248-396
+//   Ellis wormhole                   Sirius.Core/Metric/PHMT101A.h:285-345 (two-sided l form)
+//   Alcubierre                       Sirius.Core/Metric/PHMT102A.h:485-578 (static bubble)
+//   Hamiltonian RHS                  Sirius.Core/Geodesic/PHGD001A.cpp:159-202
+//   DP45 tableau / step control      Sirius.Core/Geodesic/PHGD001A.cpp:16-79,471-583
+//   null re-normalisation            Sirius.Core/Tensor/MTTN001A.cpp:171-237
+//   camera                           Sirius.Core/Camera/CMBS001A.h:120-185
+//   initial state                    Sirius.Render/Integration/GTRC001A.cpp:15-134
+// The metric gradient comes from forward-mode dual numbers with three tangents (the role of
+// Dual<T>, Sirius.Core/Autodiff/MTDL001A.h:19-368), and g^{mu nu} from the Kerr-Schild
+// closed form eta - H l l (plus a Sherman-Morrison rank-1 term for Lambda), so no 4x4
+// inverse is ever formed on the device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Every device function is also a host function, so the identical per-ray code can be
+// compiled for the CPU by diagnostic harnesses.
+#define SIRIUS_HD __host__ __device__
+
+namespace sirius {
+
+// ------------------------------------------------------------------------------------------
+// Per-camera constants, computed once on the host in FP64 (sirius_hip_api.cpp).
+// ------------------------------------------------------------------------------------------
+struct CamConst {
+    // camera model
+    double r_obs, theta_obs, phi_obs;
+    double tan_half, aspect;
+    double cyaw, syaw, cpitch, spitch, croll, sroll;
+    double er[3], et[3], ep[3];      // spherical basis at the observer
+    int32_t width, height, family, pad0;
+    int64_t offset;                  // first global ray number of this camera
+    // spacetime
+    double M, a, Q2, Lambda;
+    double b0;                       // wormhole throat
+    double wv, wR, wsig, winv2tanh; // warp: v, R, sigma, 1/(2 tanh(sigma R))
+    // termination and emission
+    double r_cap, r_in, r_out;
+    double fmax;                     // max of f(r)/r over the disk (Page-Thorne)
+    double obs_norm;                 // 1/sqrt(-g_tt) at the observer
+    double r_ph_retro;               // retrograde circular photon orbit
+    double pt_x0, pt_x[3], pt_c[3];  // Page-Thorne closed-form constants
+    double l_obs;                    // wormhole: observer's proper-length coordinate
+};
+
+struct TraceParams {
+    double rtol, atol, h_init, h_min, h_max, far_slope;
+    double r_escape;
+    double x_min, x_max;
+    double escalate_margin;
+    double sym_h;
+    double vol_j0, vol_h_over_r, vol_kappa, vol_r_min, vol_r_max, vol_tau_max;
+    int32_t max_steps, n_bins, emission, sym_order;
+    int32_t n_cam, escalate, pad0, pad1;
+    int64_t n_rays;
+};
+
+// Finite test usable on host and device: x - x is NaN exactly when x is NaN or infinite.
+template <class T>
+SIRIUS_HD __forceinline__ bool sfinite(T x) {
+    T y = x - x;
+    return y == y;
+}
+
+// Device result arrays (structure of arrays, one slot per ray; intensity is bin-major).
+struct DevOut {
+    uint8_t* cls;
+    uint8_t* flags;
+    float* theta;
+    float* phi;
+    float* lam;
+    float* disk_r;
+    float* redshift;
+    float* null_drift;
+    uint32_t* steps;
+    uint32_t* rejected;
+    uint32_t* landings;
+    float* intensity;
+    double* state;  // optional, 8 per ray
+    int64_t n_rays;
+};
+
+// ------------------------------------------------------------------------------------------
+// Dual numbers: value + gradient w.r.t. (x, y, z)
+// ------------------------------------------------------------------------------------------
+template <class T>
+struct D3 {
+    T v, g0, g1, g2;
+};
+template <class T> SIRIUS_HD __forceinline__ D3<T> dcst(T v) { return {v, T(0), T(0), T(0)}; }
+template <class T> SIRIUS_HD __forceinline__ D3<T> operator+(D3<T> a, D3<T> b) { return {a.v + b.v, a.g0 + b.g0, a.g1 + b.g1, a.g2 + b.g2}; }
+template <class T> SIRIUS_HD __forceinline__ D3<T> operator-(D3<T> a, D3<T> b) { return {a.v - b.v, a.g0 - b.g0, a.g1 - b.g1, a.g2 - b.g2}; }
+template <class T> SIRIUS_HD __forceinline__ D3<T> operator*(D3<T> a, D3<T> b) {
+    return {a.v * b.v, a.g0 * b.v + a.v * b.g0, a.g1 * b.v + a.v * b.g1, a.g2 * b.v + a.v * b.g2};
+}
+template <class T> SIRIUS_HD __forceinline__ D3<T> operator*(D3<T> a, T s) { return {a.v * s, a.g0 * s, a.g1 * s, a.g2 * s}; }
+template <class T> SIRIUS_HD __forceinline__ D3<T> operator+(D3<T> a, T s) { return {a.v + s, a.g0, a.g1, a.g2}; }
+template <class T> SIRIUS_HD __forceinline__ D3<T> operator-(D3<T> a, T s) { return {a.v - s, a.g0, a.g1, a.g2}; }
+template <class T> SIRIUS_HD __forceinline__ D3<T> dinv(D3<T> a) {
+    T r = T(1) / a.v, r2 = -r * r;
+    return {r, a.g0 * r2, a.g1 * r2, a.g2 * r2};
+}
+template <class T> SIRIUS_HD __forceinline__ D3<T> dsqrt(D3<T> a) {
+    T s = sqrt(a.v), h = T(0.5) / s;
+    return {s, a.g0 * h, a.g1 * h, a.g2 * h};
+}
+
+// Scalar parameters of a spacetime in the working precision.
+template <class T>
+struct Spacetime {
+    int family;
+    T M, a, a2, Q2, Lambda;
+    T b0, b02;
+    T wv, wR, wsig, winv2tanh;
+};
+
+template <class T>
+SIRIUS_HD __forceinline__ Spacetime<T> make_spacetime(const CamConst& c) {
+    Spacetime<T> s;
+    s.family = c.family;
+    s.M = T(c.M); s.a = T(c.a); s.a2 = T(c.a * c.a); s.Q2 = T(c.Q2); s.Lambda = T(c.Lambda);
+    s.b0 = T(c.b0); s.b02 = T(c.b0 * c.b0);
+    s.wv = T(c.wv); s.wR = T(c.wR); s.wsig = T(c.wsig); s.winv2tanh = T(c.winv2tanh);
+    return s;
+}
+
+// Kerr radius (value only), PHMT100A.h:198-216.
+template <class T>
+SIRIUS_HD __forceinline__ T kerr_r(const Spacetime<T>& s, T x, T y, T z) {
+    T R2 = x * x + y * y + z * z;
+    if (s.family != 0) return sqrt(R2);
+    T w = R2 - s.a2;
+    T disc = w * w + T(4) * s.a2 * z * z;
+    disc = disc < T(1e-20) ? T(1e-20) : disc;
+    T r2 = T(0.5) * (w + sqrt(disc));
+    r2 = r2 < T(1e-10) ? T(1e-10) : r2;
+    return sqrt(r2);
+}
+
+// Event radius used for termination: Kerr r, Euclidean R (warp) or |l| (wormhole).
+template <class T>
+SIRIUS_HD __forceinline__ T event_radius(const Spacetime<T>& s, const T x[4]) {
+    if (s.family == 1) return fabs(x[1]);
+    return kerr_r(s, x[1], x[2], x[3]);
+}
+
+// Covariant metric values (no gradient), for initial conditions and emitter velocities.
+template <class T>
+SIRIUS_HD inline void metric_cov(const Spacetime<T>& s, const T x[4], T g[4][4]) {
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) g[i][j] = (i == j) ? (i == 0 ? T(-1) : T(1)) : T(0);
+    if (s.family == 1) {
+        g[2][2] = x[1] * x[1] + s.b02;
+        return;
+    }
+    if (s.family == 2) {
+        T rs = sqrt(x[1] * x[1] + x[2] * x[2] + x[3] * x[3]);
+        T f = (tanh(s.wsig * (rs + s.wR)) - tanh(s.wsig * (rs - s.wR))) * s.winv2tanh;
+        g[0][0] = -(T(1) - s.wv * s.wv * f * f);
+        g[0][1] = g[1][0] = -s.wv * f;
+        return;
+    }
+    T r = kerr_r(s, x[1], x[2], x[3]);
+    T r2 = r * r, den = T(1) / (r2 + s.a2);
+    T l[4] = {T(1), (r * x[1] + s.a * x[2]) * den, (r * x[2] - s.a * x[1]) * den, x[3] / r};
+    T H = (T(2) * s.M * r - s.Q2) * r2 / (r2 * r2 + s.a2 * x[3] * x[3]);
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) g[i][j] += H * l[i] * l[j];
+    if (s.Lambda != T(0)) g[0][0] -= s.Lambda * (x[1] * x[1] + x[2] * x[2] + x[3] * x[3]) / T(3);
+}
+
+// ------------------------------------------------------------------------------------------
+// Hamiltonian right-hand side: k^mu = g^{mu nu} p_nu (written to dx), and
+// dp_i = 1/2 d_i g_{rs} k^r k^s for i = 1..3 (dp_t = 0: all metrics are stationary).
+// Returns p.k = g_{mu nu} k^mu k^nu (the null-constraint value).
+// ------------------------------------------------------------------------------------------
+template <class T>
+SIRIUS_HD __forceinline__ T rhs_ks(const Spacetime<T>& s, const T x[4], const T p[4], T k[4], T dp[3]) {
+    D3<T> X = {x[1], T(1), T(0), T(0)};
+    D3<T> Y = {x[2], T(0), T(1), T(0)};
+    D3<T> Z = {x[3], T(0), T(0), T(1)};
+    D3<T> R2 = X * X + Y * Y + Z * Z;
+    D3<T> w = R2 - s.a2;
+    D3<T> disc = w * w + Z * Z * (T(4) * s.a2);
+    if (disc.v < T(1e-20)) disc = dcst(T(1e-20));
+    D3<T> r2 = (w + dsqrt(disc)) * T(0.5);
+    if (r2.v < T(1e-10)) r2 = dcst(T(1e-10));
+    D3<T> r = dsqrt(r2);
+    D3<T> iden = dinv(r2 + s.a2);
+    D3<T> l1 = (r * X + Y * s.a) * iden;
+    D3<T> l2 = (r * Y - X * s.a) * iden;
+    D3<T> l3 = Z * dinv(r);
+    D3<T> num = (r * (T(2) * s.M) - s.Q2) * r2;
+    D3<T> H = num * dinv(r2 * r2 + Z * Z * s.a2);
+    // k = (eta - H l l) p, l^mu = (-1, l1, l2, l3)
+    T lp = -p[0] + l1.v * p[1] + l2.v * p[2] + l3.v * p[3];
+    T Hl = H.v * lp;
+    k[0] = -p[0] + Hl;
+    k[1] = p[1] - Hl * l1.v;
+    k[2] = p[2] - Hl * l2.v;
+    k[3] = p[3] - Hl * l3.v;
+    T lam3 = T(0);
+    if (s.Lambda != T(0)) {
+        // g = g0 + c e0 e0^T with c = -Lambda R^2 / 3; g^-1 by Sherman-Morrison.
+        T c = -s.Lambda * R2.v / T(3);
+        T u0 = -T(1) - H.v;
+        T fac = c * k[0] / (T(1) + c * u0);
+        k[0] -= fac * u0;
+        k[1] -= fac * H.v * l1.v;
+        k[2] -= fac * H.v * l2.v;
+        k[3] -= fac * H.v * l3.v;
+        lam3 = s.Lambda / T(3);
+    }
+    T lk = k[0] + l1.v * k[1] + l2.v * k[2] + l3.v * k[3];
+    T hl = T(0.5) * lk * lk, Hlk = H.v * lk, k00 = k[0] * k[0];
+    dp[0] = H.g0 * hl + Hlk * (l1.g0 * k[1] + l2.g0 * k[2] + l3.g0 * k[3]) - lam3 * x[1] * k00;
+    dp[1] = H.g1 * hl + Hlk * (l1.g1 * k[1] + l2.g1 * k[2] + l3.g1 * k[3]) - lam3 * x[2] * k00;
+    dp[2] = H.g2 * hl + Hlk * (l1.g2 * k[1] + l2.g2 * k[2] + l3.g2 * k[3]) - lam3 * x[3] * k00;
+    return p[0] * k[0] + p[1] * k[1] + p[2] * k[2] + p[3] * k[3];
+}
+
+// Ellis wormhole, orbital-plane reduction (t, l, psi, w): g = diag(-1, 1, l^2 + b0^2, 1).
+template <class T>
+SIRIUS_HD __forceinline__ T rhs_mt(const Spacetime<T>& s, const T x[4], const T p[4], T k[4], T dp[3]) {
+    T r2 = x[1] * x[1] + s.b02;
+    k[0] = -p[0];
+    k[1] = p[1];
+    k[2] = p[2] / r2;
+    k[3] = p[3];
+    dp[0] = x[1] * k[2] * k[2];
+    dp[1] = T(0);
+    dp[2] = T(0);
+    return p[0] * k[0] + p[1] * k[1] + p[2] * k[2] + p[3] * k[3];
+}
+
+// Alcubierre, static bubble: g^{00} = -1, g^{01} = -v f, g^{11} = 1 - v^2 f^2.
+template <class T>
+SIRIUS_HD __forceinline__ T rhs_warp(const Spacetime<T>& s, const T x[4], const T p[4], T k[4], T dp[3]) {
+    T R2 = x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
+    R2 = R2 < T(1e-20) ? T(1e-20) : R2;
+    T rs = sqrt(R2);
+    T tp = tanh(s.wsig * (rs + s.wR)), tm = tanh(s.wsig * (rs - s.wR));
+    T f = (tp - tm) * s.winv2tanh;
+    T dfdr = s.wsig * ((T(1) - tp * tp) - (T(1) - tm * tm)) * s.winv2tanh;
+    T vf = s.wv * f;
+    k[0] = -p[0] - vf * p[1];
+    k[1] = -vf * p[0] + (T(1) - vf * vf) * p[1];
+    k[2] = p[2];
+    k[3] = p[3];
+    T c = s.wv * dfdr / rs * (vf * k[0] * k[0] - k[0] * k[1]);
+    dp[0] = c * x[1];
+    dp[1] = c * x[2];
+    dp[2] = c * x[3];
+    return p[0] * k[0] + p[1] * k[1] + p[2] * k[2] + p[3] * k[3];
+}
+
+template <class T, int FAM>
+SIRIUS_HD __forceinline__ T rhs(const Spacetime<T>& s, const T x[4], const T p[4], T k[4], T dp[3]) {
+    if constexpr (FAM == 1) return rhs_mt<T>(s, x, p, k, dp);
+    else if constexpr (FAM == 2) return rhs_warp<T>(s, x, p, k, dp);
+    else return rhs_ks<T>(s, x, p, k, dp);
+}
+
+// Derivative of the state: dx (= k), dp (spatial), and p.k.
+template <class T>
+struct Der {
+    T k[4];
+    T f[3];
+    T pk;
+};
+
+template <class T, int FAM>
+SIRIUS_HD __forceinline__ void eval(const Spacetime<T>& s, const T x[4], const T p[4], Der<T>& d) {
+    d.pk = rhs<T, FAM>(s, x, p, d.k, d.f);
+}
+
+// ------------------------------------------------------------------------------------------
+// Dormand-Prince 5(4), PHGD001A.cpp:16-79. p_t is a constant of motion and is not stepped.
+// ------------------------------------------------------------------------------------------
+template <class T, int FAM>
+SIRIUS_HD inline void dp45_step(const Spacetime<T>& s, const T x0[4], const T p0[4], const Der<T>& d1,
+                                 T h, T x1[4], T p1[4], Der<T>& d7, T err[4], bool want_err) {
+    constexpr double a21 = 1.0 / 5.0;
+    constexpr double a31 = 3.0 / 40.0, a32 = 9.0 / 40.0;
+    constexpr double a41 = 44.0 / 45.0, a42 = -56.0 / 15.0, a43 = 32.0 / 9.0;
+    constexpr double a51 = 19372.0 / 6561.0, a52 = -25360.0 / 2187.0, a53 = 64448.0 / 6561.0, a54 = -212.0 / 729.0;
+    constexpr double a61 = 9017.0 / 3168.0, a62 = -355.0 / 33.0, a63 = 46732.0 / 5247.0, a64 = 49.0 / 176.0,
+                     a65 = -5103.0 / 18656.0;
+    constexpr double b1 = 35.0 / 384.0, b3 = 500.0 / 1113.0, b4 = 125.0 / 192.0, b5 = -2187.0 / 6784.0,
+                     b6 = 11.0 / 84.0;
+    constexpr double e1 = 71.0 / 57600.0, e3 = -71.0 / 16695.0, e4 = 71.0 / 1920.0, e5 = -17253.0 / 339200.0,
+                     e6 = 22.0 / 525.0, e7 = -1.0 / 40.0;
+    Der<T> d2, d3, d4, d5, d6;
+    T xt[4], pt[4];
+    pt[0] = p0[0];
+    xt[0] = x0[0] + h * T(a21) * d1.k[0];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) xt[i] = x0[i] + h * (T(a21) * d1.k[i]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) pt[i + 1] = p0[i + 1] + h * (T(a21) * d1.f[i]);
+    eval<T, FAM>(s, xt, pt, d2);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) xt[i] = x0[i] + h * (T(a31) * d1.k[i] + T(a32) * d2.k[i]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) pt[i + 1] = p0[i + 1] + h * (T(a31) * d1.f[i] + T(a32) * d2.f[i]);
+    eval<T, FAM>(s, xt, pt, d3);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) xt[i] = x0[i] + h * (T(a41) * d1.k[i] + T(a42) * d2.k[i] + T(a43) * d3.k[i]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) pt[i + 1] = p0[i + 1] + h * (T(a41) * d1.f[i] + T(a42) * d2.f[i] + T(a43) * d3.f[i]);
+    eval<T, FAM>(s, xt, pt, d4);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        xt[i] = x0[i] + h * (T(a51) * d1.k[i] + T(a52) * d2.k[i] + T(a53) * d3.k[i] + T(a54) * d4.k[i]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        pt[i + 1] = p0[i + 1] + h * (T(a51) * d1.f[i] + T(a52) * d2.f[i] + T(a53) * d3.f[i] + T(a54) * d4.f[i]);
+    eval<T, FAM>(s, xt, pt, d5);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        xt[i] = x0[i] + h * (T(a61) * d1.k[i] + T(a62) * d2.k[i] + T(a63) * d3.k[i] + T(a64) * d4.k[i] +
+                             T(a65) * d5.k[i]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        pt[i + 1] = p0[i + 1] + h * (T(a61) * d1.f[i] + T(a62) * d2.f[i] + T(a63) * d3.f[i] + T(a64) * d4.f[i] +
+                                     T(a65) * d5.f[i]);
+    eval<T, FAM>(s, xt, pt, d6);
+    p1[0] = p0[0];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        x1[i] = x0[i] + h * (T(b1) * d1.k[i] + T(b3) * d3.k[i] + T(b4) * d4.k[i] + T(b5) * d5.k[i] + T(b6) * d6.k[i]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        p1[i + 1] = p0[i + 1] + h * (T(b1) * d1.f[i] + T(b3) * d3.f[i] + T(b4) * d4.f[i] + T(b5) * d5.f[i] +
+                                     T(b6) * d6.f[i]);
+    eval<T, FAM>(s, x1, p1, d7);
+    if (want_err) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            err[i] = h * (T(e1) * d1.k[i] + T(e3) * d3.k[i] + T(e4) * d4.k[i] + T(e5) * d5.k[i] + T(e6) * d6.k[i] +
+                          T(e7) * d7.k[i]);
+    }
+}
+
+// computeOptimalStep (PHGD001A.cpp:471-483): h * clamp(0.9 err^-1/5, 0.1, 2), clamped.
+template <class T>
+SIRIUS_HD __forceinline__ T next_step(T h, T err, T hmin, T hmax) {
+    T hn;
+    if (err < T(1e-15)) hn = h * T(2);
+    else {
+        T fac = T(0.9) * pow(err, T(-0.2));
+        fac = fac < T(0.1) ? T(0.1) : (fac > T(2) ? T(2) : fac);
+        hn = h * fac;
+    }
+    hn = hn < hmin ? hmin : hn;
+    hn = hn > hmax ? hmax : hn;
+    return hn;
+}
+
+// Cubic Hermite position over a step, th in [0, 1].
+template <class T>
+SIRIUS_HD __forceinline__ void hermite(const T x0[4], const T k0[4], const T x1[4], const T k1[4], T h, T th,
+                                        T out[4]) {
+    T t2 = th * th, t3 = t2 * th;
+    T h00 = T(2) * t3 - T(3) * t2 + T(1), h10 = t3 - T(2) * t2 + th, h01 = T(3) * t2 - T(2) * t3, h11 = t3 - t2;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[i] = h00 * x0[i] + h10 * h * k0[i] + h01 * x1[i] + h11 * h * k1[i];
+}
+
+enum { EV_Z = 0, EV_R = 1 };
+
+template <class T>
+SIRIUS_HD __forceinline__ T event_value(const Spacetime<T>& s, int kind, const T x[4], T target) {
+    if (kind == EV_Z) return x[3];
+    return event_radius(s, x) - target;
+}
+
+template <class T>
+SIRIUS_HD inline T locate_event(const Spacetime<T>& s, int kind, T target, const T x0[4], const T k0[4],
+                                 const T x1[4], const T k1[4], T h) {
+    constexpr int iters = sizeof(T) == 8 ? 52 : 24;
+    T lo = T(0), hi = T(1), xm[4];
+    bool neg_lo = event_value(s, kind, x0, target) < T(0);
+    for (int it = 0; it < iters; ++it) {
+        T mid = T(0.5) * (lo + hi);
+        hermite(x0, k0, x1, k1, h, mid, xm);
+        bool neg = event_value(s, kind, xm, target) < T(0);
+        if (neg == neg_lo) lo = mid; else hi = mid;
+    }
+    return T(0.5) * (lo + hi);
+}
+
+// ------------------------------------------------------------------------------------------
+// Camera and initial state (CMBS001A.h:120-185, GTRC001A.cpp:15-106)
+// ------------------------------------------------------------------------------------------
+template <class T>
+SIRIUS_HD inline void camera_dir(const CamConst& c, int px, int py, T nsph[3]) {
+    T sx = (T(2) * (T(px) + T(0.5)) / T(c.width) - T(1)) * T(c.aspect);
+    T sy = T(1) - T(2) * (T(py) + T(0.5)) / T(c.height);
+    T dx = sx * T(c.tan_half), dy = sy * T(c.tan_half), dz = T(-1);
+    T cr = T(c.croll), sr = T(c.sroll), cp = T(c.cpitch), sp = T(c.spitch), cy = T(c.cyaw), sy_ = T(c.syaw);
+    T rx = dx * cr - dy * sr, ry = dx * sr + dy * cr;
+    T ry2 = ry * cp - dz * sp, rz = ry * sp + dz * cp;
+    T rx2 = rx * cy + rz * sy_, rz2 = -rx * sy_ + rz * cy;
+    T il = T(1) / sqrt(rx2 * rx2 + ry2 * ry2 + rz2 * rz2);
+    nsph[0] = rz2 * il;
+    nsph[1] = -ry2 * il;
+    nsph[2] = rx2 * il;
+}
+
+// Future-directed k^0 of MTTN001A.cpp:171-237 (same root choice and fallbacks). With
+// near > 0 (re-normalisation of a ray in flight) the root nearest the current k^0 is taken
+// instead: inside the ergoregion g_00 > 0 and both roots can be positive, where the
+// reference's first-positive-root rule can jump to the wrong branch.
+template <class T>
+SIRIUS_HD inline T null_k0(const T g[4][4], const T k[4], T near = T(0)) {
+    T A = g[0][0], B = T(0), C = T(0);
+    for (int i = 1; i < 4; ++i) B += T(2) * g[0][i] * k[i];
+    for (int i = 1; i < 4; ++i)
+        for (int j = 1; j < 4; ++j) C += g[i][j] * k[i] * k[j];
+    if (fabs(A) < T(1e-15)) return fabs(B) > T(1e-15) ? -C / B : T(1);
+    T disc = B * B - T(4) * A * C;
+    if (disc < T(0)) return T(1);
+    T sq = sqrt(disc);
+    T kp = (-B + sq) / (T(2) * A), km = (-B - sq) / (T(2) * A);
+    if (near > T(0)) return fabs(kp - near) < fabs(km - near) ? kp : km;
+    if (kp > T(0)) return kp;
+    if (km > T(0)) return km;
+    return kp > km ? kp : km;
+}
+
+// Wormhole orbital-plane frame of a ray: n_r (radial) and n_t (tangential magnitude).
+template <class T>
+SIRIUS_HD inline void mt_radial_tangential(const CamConst& c, const T ns[3], T& nr, T& nt) {
+    nr = ns[0];
+    nt = sqrt(ns[1] * ns[1] + ns[2] * ns[2]);
+}
+
+template <class T>
+SIRIUS_HD inline void init_state(const CamConst& c, const Spacetime<T>& s, int px, int py, T x[4], T p[4]) {
+    T ns[3];
+    camera_dir<T>(c, px, py, ns);
+    T k[4], g[4][4];
+    if (c.family == 1) {
+        T nr, nt;
+        mt_radial_tangential(c, ns, nr, nt);
+        T r = T(c.r_obs);
+        x[0] = T(0); x[1] = T(c.l_obs); x[2] = T(0); x[3] = T(0);
+        k[0] = T(1); k[1] = nr; k[2] = nt / r; k[3] = T(0);
+    } else {
+        x[0] = T(0);
+        k[0] = T(0);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            x[i + 1] = T(c.r_obs * c.er[i]);
+            k[i + 1] = ns[0] * T(c.er[i]) + ns[1] * T(c.et[i]) + ns[2] * T(c.ep[i]);
+        }
+    }
+    metric_cov(s, x, g);
+    if (c.family != 1) k[0] = null_k0(g, k);
+    for (int mu = 0; mu < 4; ++mu) {
+        T acc = T(0);
+        for (int nu = 0; nu < 4; ++nu) acc += g[mu][nu] * k[nu];
+        p[mu] = acc;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Emission (evaluated in FP64 on both paths; once per ray or per volume node)
+// ------------------------------------------------------------------------------------------
+// Page & Thorne (1974) closed-form f(r) with the camera's precomputed roots (M = 1).
+SIRIUS_HD inline double page_thorne_f(const CamConst& c, double r) {
+    double x = sqrt(r), x0 = c.pt_x0;
+    if (x <= x0) return 0.0;
+    double br = x - x0 - 1.5 * c.a * log(x / x0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) br -= c.pt_c[i] * log((x - c.pt_x[i]) / (x0 - c.pt_x[i]));
+    return 1.5 / (x * x * (x * x * x - 3.0 * x + 2.0 * c.a)) * br;
+}
+
+// Kepler angular velocity of a prograde circular Kerr-Newman orbit at BL radius r.
+SIRIUS_HD inline double kepler_omega(const CamConst& c, double r) {
+    double sq = sqrt(fmax(c.M * r - c.Q2, 0.0));
+    return sq / (r * r + c.a * sq);
+}
+
+// u^t of the circular emitter at x with angular velocity Om (u = u^t (d_t + Om d_phi)).
+SIRIUS_HD inline double emitter_ut(const CamConst& c, const Spacetime<double>& s, const double x[4], double Om) {
+    double g[4][4];
+    metric_cov(s, x, g);
+    double v[4] = {1.0, -Om * x[2], Om * x[1], 0.0};
+    double vv = 0;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) vv += g[i][j] * v[i] * v[j];
+    return vv < 0 ? 1.0 / sqrt(-vv) : -1.0;
+}
+
+// Thin disk: g = (p.u_obs)/(p.u_emit), I_k = x_k^3 / (e^{x_k/(g tau)} - 1).
+SIRIUS_HD inline void disk_emission(const CamConst& c, const TraceParams& tp, const double x[4], const double p[4],
+                                     double r, double& gf, float* I, int stride) {
+    Spacetime<double> s = make_spacetime<double>(c);
+    double Om = kepler_omega(c, r);
+    double ut = emitter_ut(c, s, x, Om);
+    double pphi = x[1] * p[2] - x[2] * p[1];
+    gf = (p[0] * c.obs_norm) / (ut * (p[0] + Om * pphi));
+    double f = page_thorne_f(c, r) / r;
+    double tau = (c.fmax > 0 && f > 0) ? pow(f / c.fmax, 0.25) : 0.0;
+    double lr = log(tp.x_max / tp.x_min);
+    for (int k = 0; k < tp.n_bins; ++k) {
+        double xk = tp.x_min * exp(lr * (k + 0.5) / tp.n_bins);
+        double v = (tau > 0 && gf > 0 && ut > 0) ? xk * xk * xk / expm1(xk / (gf * tau)) : 0.0;
+        I[(int64_t)k * stride] = (float)v;
+    }
+}
+
+}  // namespace sirius

@@ -1,0 +1,129 @@
+// hip_accelerator.cpp — HIPAccelerator: the IAccelerator implementation over the C-ABI.
+// Mirrors the OptiX wrapper's shape (src/Sirius.Render/Acceleration/OptiX/RDOX001A.cpp:91-341:
+// launch -> convertConfig -> C API launch; getFrameBuffer returns the C layer's host copy).
+#include "../../include/sirius_hip_accelerator.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace Sirius::Acceleration::HIP {
+
+HIPAccelerator::HIPAccelerator() : m_handle(sirius_hip_create()) {
+    if (!m_handle) m_error = "sirius_hip_create failed";
+}
+
+HIPAccelerator::~HIPAccelerator() {
+    if (m_handle) sirius_hip_destroy(m_handle);
+}
+
+DeviceCapabilities HIPAccelerator::getCapabilities() const {
+    DeviceCapabilities caps;
+    if (!m_handle) return caps;
+    char name[256] = {0};
+    size_t total = 0, free_b = 0;
+    int cus = 0, mtpb = 0;
+    if (sirius_hip_device_info(m_handle, name, sizeof(name), &total, &free_b, &cus, &mtpb) == SIRIUS_HIP_OK) {
+        caps.deviceName = name;
+        caps.totalMemory = total;
+        caps.freeMemory = free_b;
+        caps.computeCapability = 0;  // CUDA-only notion
+        caps.supportsRTCores = false;
+        caps.supportsFP64 = true;
+        caps.multiprocessorCount = cus;
+        caps.maxThreadsPerBlock = mtpb;
+    }
+    return caps;
+}
+
+bool HIPAccelerator::initialise(int width, int height) {
+    if (!m_handle) return false;
+    int rc = sirius_hip_initialize(m_handle, width, height);
+    if (rc != SIRIUS_HIP_OK) {
+        m_error = sirius_hip_last_error(m_handle);
+        return false;
+    }
+    return true;
+}
+
+bool HIPAccelerator::isInitialised() const { return m_handle && sirius_hip_is_initialized(m_handle); }
+
+void HIPAccelerator::convertConfig(const LaunchConfig& cfg, SiriusHipCamera& cam, SiriusHipMetric& metric,
+                                   SiriusHipTraceOpts& opts, int precision) {
+    // Lengths in units of M (G = c = M = 1); blackHoleSpin is a/M.
+    double M = cfg.blackHoleMass > 0 ? cfg.blackHoleMass : 1.0;
+    std::memset(&metric, 0, sizeof(metric));
+    metric.family = cfg.metricFamily;
+    if (cfg.metricFamily == SIRIUS_METRIC_MORRIS_THORNE) {
+        metric.throat_b0 = cfg.throatRadius / M;
+    } else if (cfg.metricFamily == SIRIUS_METRIC_ALCUBIERRE) {
+        metric.warp_v = cfg.warpVelocity;
+        metric.warp_R = cfg.bubbleRadius / M;
+        metric.warp_sigma = cfg.bubbleSigma * M;
+    } else {
+        metric.family = SIRIUS_METRIC_KERR_SCHILD;
+        metric.M = 1.0;
+        metric.a = std::clamp((double)cfg.blackHoleSpin, -0.9999, 0.9999);
+    }
+    std::memset(&cam, 0, sizeof(cam));
+    cam.r_obs = cfg.observerDistance / M;
+    cam.theta_obs = cfg.observerInclination;
+    cam.phi_obs = cfg.observerAzimuth;
+    cam.fov_deg = cfg.cameraFOV;
+    cam.disk_r_in = cfg.diskInnerRadius / M;
+    cam.disk_r_out = cfg.diskOuterRadius / M;
+    cam.width = cfg.width;
+    cam.height = cfg.height;
+    cam.metric_index = 0;
+    sirius_hip_default_opts(&opts, precision);
+    opts.n_bins = 16;
+    if (cfg.maxSteps > 0) opts.max_steps = cfg.maxSteps;
+}
+
+void HIPAccelerator::launch(const LaunchConfig& config) {
+    if (!m_handle) return;
+    SiriusHipCamera cam;
+    SiriusHipMetric metric;
+    SiriusHipTraceOpts opts;
+    convertConfig(config, cam, metric, opts, m_precision);
+    int rc = sirius_hip_launch(m_handle, &cam, &metric, &opts);
+    if (rc != SIRIUS_HIP_OK) m_error = sirius_hip_last_error(m_handle);
+}
+
+float* HIPAccelerator::getFrameBuffer() { return m_handle ? sirius_hip_get_frame_buffer(m_handle) : nullptr; }
+
+size_t HIPAccelerator::getFrameBufferSize() const {
+    return m_handle ? sirius_hip_get_frame_buffer_size(m_handle) : 0;
+}
+
+bool HIPAccelerator::uploadBackground(const uint8_t* data, int width, int height) {
+    if (!m_handle) return false;
+    int rc = sirius_hip_upload_background(m_handle, data, width, height);
+    if (rc != SIRIUS_HIP_OK) {
+        m_error = sirius_hip_last_error(m_handle);
+        return false;
+    }
+    return true;
+}
+
+void HIPAccelerator::synchronise() {
+    if (m_handle && sirius_hip_synchronize(m_handle) != SIRIUS_HIP_OK) m_error = sirius_hip_last_error(m_handle);
+}
+
+void HIPAccelerator::cleanup() {
+    if (m_handle) sirius_hip_cleanup(m_handle);
+}
+
+std::string HIPAccelerator::getLastError() const { return m_error; }
+
+int HIPAccelerator::traceBatch(const SiriusHipCamera* cams, int n_cam, const SiriusHipMetric* metrics, int n_metric,
+                               const SiriusHipTraceOpts& opts, SiriusHipRayOutput& out, SiriusHipStats* stats) {
+    if (!m_handle) return SIRIUS_HIP_E_STATE;
+    int rc = sirius_hip_trace_batch(m_handle, cams, n_cam, metrics, n_metric, &opts, &out, stats);
+    if (rc != SIRIUS_HIP_OK) m_error = sirius_hip_last_error(m_handle);
+    return rc;
+}
+
+std::unique_ptr<IAccelerator> createHIPAccelerator() { return std::make_unique<HIPAccelerator>(); }
+
+}  // namespace Sirius::Acceleration::HIP

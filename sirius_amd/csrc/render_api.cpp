@@ -1,0 +1,241 @@
+// render_api.cpp — the IAccelerator-facing half of the C-ABI: launch + shade into the RGBA
+// framebuffer, background upload, framebuffer access (RDOP006A.h:22-33 roles), the
+// critical-curve distance query and the launch-shape autotuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/sirius_hip.h"
+#include "geodesic.hpp"
+#include "sirius_internal.hpp"
+
+namespace sirius {
+hipError_t launch_shade(const DevOut& o, int64_t n, int width, int height, int n_bins, const uint8_t* bg, int bgw,
+                        int bgh, float* fb, hipStream_t st);
+hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float* curves, const int* curve_n,
+                                    int curve_stride, int64_t n, float* out, hipStream_t st);
+int persistent_blocks_per_cu(bool fp64, int family, int block);
+
+// Critical curve of a Kerr-Newman camera in (alpha, |beta|): spherical photon orbits with
+// xi(r) = ((r^2 + a^2) D' - 4 r D) / (a D'), eta = (r^2 + a^2 - a xi)^2 / D - (xi - a)^2,
+// D = r^2 - 2 M r + a^2 + Q^2 (Bardeen 1973 generalised). Returns the number of points, or -1
+// with pts[0] = radius for a non-rotating hole.
+int critical_curve(const CamConst& c, float* pts, int cap) {
+    double M = c.M, a = c.a, Q2 = c.Q2;
+    if (std::fabs(a) < 1e-6) {
+        double rph = 0.5 * (3 * M + std::sqrt(std::max(9 * M * M - 8 * Q2, 0.0)));
+        double D = rph * rph - 2 * M * rph + Q2;
+        pts[0] = (float)(rph * rph / std::sqrt(D));
+        return -1;
+    }
+    double rp = M + std::sqrt(std::max(M * M - a * a - Q2, 0.0));
+    // observer polar angle as the ray start sees it (Kerr r of the flat-mapped position)
+    double x = c.r_obs * c.er[0], y = c.r_obs * c.er[1], z = c.r_obs * c.er[2];
+    double R2 = x * x + y * y + z * z, w = R2 - a * a;
+    double r0 = std::sqrt(0.5 * (w + std::sqrt(w * w + 4 * a * a * z * z)));
+    double ct = z / r0, st = std::sqrt(1 - ct * ct);
+    int n = 0;
+    for (int i = 0; i < cap; ++i) {
+        double r = rp + (5.0 * M - rp) * (i + 0.5) / cap;
+        double D = r * r - 2 * M * r + a * a + Q2, Dp = 2 * r - 2 * M;
+        if (D <= 0 || std::fabs(Dp) < 1e-12) continue;
+        double xi = ((r * r + a * a) * Dp - 4 * r * D) / (a * Dp);
+        double t1 = r * r + a * a - a * xi;
+        double eta = t1 * t1 / D - (xi - a) * (xi - a);
+        if (eta < 0) continue;
+        double b2 = eta + a * a * ct * ct - xi * xi * ct * ct / (st * st);
+        if (b2 < 0) continue;
+        pts[2 * n] = (float)(-xi / st);
+        pts[2 * n + 1] = (float)std::sqrt(b2);
+        ++n;
+    }
+    return n;
+}
+
+int critical_distance_dev(Handle& H, Batch& B, float* out_dev) {
+    const int stride = 2048;
+    int n_cam = (int)B.cc.size();
+    std::vector<float> curves((size_t)n_cam * stride * 2, 0.f);
+    std::vector<int> cn(n_cam, 0);
+    for (int i = 0; i < n_cam; ++i)
+        cn[i] = B.cc[i].family == 0 ? critical_curve(B.cc[i], curves.data() + (size_t)i * stride * 2, stride) : 0;
+    int rc = 0;
+    float* cd = (float*)H.scratch("crit_curves", curves.size() * 4, &rc);
+    int* nd = (int*)H.scratch("crit_n", cn.size() * 4, &rc);
+    if (rc) return rc;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(cd, curves.data(), curves.size() * 4, hipMemcpyHostToDevice, H.stream)) != hipSuccess)
+        return H.hip_fail(e, "upload curves");
+    if ((e = hipMemcpyAsync(nd, cn.data(), cn.size() * 4, hipMemcpyHostToDevice, H.stream)) != hipSuccess)
+        return H.hip_fail(e, "upload curves");
+    if ((e = launch_critical_distance(B.cams_dev, n_cam, cd, nd, stride, B.n, out_dev, H.stream)) != hipSuccess)
+        return H.hip_fail(e, "critical distance");
+    // the host vectors die here: wait for the async copies
+    if ((e = hipStreamSynchronize(H.stream)) != hipSuccess) return H.hip_fail(e, "synchronize");
+    return SIRIUS_HIP_OK;
+}
+
+}  // namespace sirius
+
+using namespace sirius;
+
+#define GUARD_BEGIN try {
+#define GUARD_END(h)                                                                     \
+    }                                                                                    \
+    catch (const std::bad_alloc&) { return (h)->fail(SIRIUS_HIP_E_NOMEM, "host allocation"); } \
+    catch (...) { return (h)->fail(SIRIUS_HIP_E_DEVICE, "unexpected exception"); }
+
+extern "C" {
+
+int sirius_hip_launch(SiriusHipHandle hh, const SiriusHipCamera* cam, const SiriusHipMetric* metric,
+                      const SiriusHipTraceOpts* opts) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    GUARD_BEGIN
+    if (!H->initialised) return H->fail(SIRIUS_HIP_E_STATE, "sirius_hip_initialize first");
+    if (!cam || !metric || !opts) return H->fail(SIRIUS_HIP_E_INVALID, "camera, metric and opts are required");
+    if (cam->width != H->fb_w || cam->height != H->fb_h)
+        return H->fail(SIRIUS_HIP_E_INVALID, "camera size differs from the framebuffer (resize first)");
+    SiriusHipCamera c = *cam;
+    c.metric_index = 0;
+    Batch B;
+    int rc = prepare_batch(*H, &c, 1, metric, 1, *opts, B);
+    if (rc) return rc;
+    DevOut dout;
+    if ((rc = bind_outputs(*H, nullptr, B.n, opts->n_bins, dout))) return rc;
+    if (opts->use_surrogate) rc = surrogate_trace(*H, B, *opts, dout, nullptr);
+    else rc = run_trace(*H, B, *opts, dout, nullptr, nullptr, 0, nullptr, false);
+    if (rc) return rc;
+    hipError_t e = launch_shade(dout, B.n, c.width, c.height, opts->n_bins, H->bg_dev, H->bg_w, H->bg_h, H->fb_dev,
+                                H->stream);
+    if (e != hipSuccess) return H->hip_fail(e, "shade");
+    e = hipMemcpyAsync(H->fb_host.data(), H->fb_dev, H->fb_host.size() * sizeof(float), hipMemcpyDeviceToHost,
+                       H->stream);
+    if (e != hipSuccess) return H->hip_fail(e, "framebuffer copy");
+    e = hipStreamSynchronize(H->stream);  // launch is synchronous (RDOP001A.cu:1044)
+    if (e != hipSuccess) return H->hip_fail(e, "synchronize");
+    return SIRIUS_HIP_OK;
+    GUARD_END(H)
+}
+
+float* sirius_hip_get_frame_buffer(SiriusHipHandle hh) {
+    if (!hh) return nullptr;
+    Handle* H = static_cast<Handle*>(hh);
+    return H->initialised && !H->fb_host.empty() ? H->fb_host.data() : nullptr;
+}
+
+size_t sirius_hip_get_frame_buffer_size(SiriusHipHandle hh) {
+    if (!hh) return 0;
+    Handle* H = static_cast<Handle*>(hh);
+    return (size_t)H->fb_w * H->fb_h * 4 * sizeof(float);
+}
+
+int sirius_hip_upload_background(SiriusHipHandle hh, const unsigned char* rgba, int width, int height) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    GUARD_BEGIN
+    if (!rgba || width <= 0 || height <= 0) return H->fail(SIRIUS_HIP_E_INVALID, "background image");
+    int rc = H->ensure_device();
+    if (rc) return rc;
+    if (H->bg_dev) (void)hipFree(H->bg_dev);
+    H->bg_dev = nullptr;
+    size_t bytes = (size_t)width * height * 4;
+    hipError_t e = hipMalloc(&H->bg_dev, bytes);
+    if (e != hipSuccess) { H->bg_dev = nullptr; return H->hip_fail(e, "background"); }
+    e = hipMemcpy(H->bg_dev, rgba, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return H->hip_fail(e, "background upload");
+    H->bg_w = width;
+    H->bg_h = height;
+    return SIRIUS_HIP_OK;
+    GUARD_END(H)
+}
+
+int sirius_hip_critical_distance(SiriusHipHandle hh, const SiriusHipCamera* cams, int n_cam,
+                                 const SiriusHipMetric* metrics, int n_metric, float* out, int on_device) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    GUARD_BEGIN
+    if (!out) return H->fail(SIRIUS_HIP_E_INVALID, "out");
+    int rc = H->ensure_device();
+    if (rc) return rc;
+    SiriusHipTraceOpts o;
+    sirius_hip_default_opts(&o, SIRIUS_PRECISION_FP64);
+    Batch B;
+    if ((rc = prepare_batch(*H, cams, n_cam, metrics, n_metric, o, B))) return rc;
+    float* d = on_device ? out : (float*)H->scratch("crit_out", 4 * (size_t)B.n, &rc);
+    if (rc) return rc;
+    if ((rc = critical_distance_dev(*H, B, d))) return rc;
+    if (!on_device) {
+        hipError_t e = hipMemcpy(out, d, 4 * (size_t)B.n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return H->hip_fail(e, "copy");
+    }
+    return SIRIUS_HIP_OK;
+    GUARD_END(H)
+}
+
+// Autotuner: times the FP32 persistent trace of a strided sample of the batch (every k-th ray,
+// about 1/16 of it, at least 64k rays) over block sizes {64, 128, 256} x blocks/CU
+// {occupancy, 1/2, 1/4 of it} x refill threshold {1, 8, 16, 32}, and keeps the fastest.
+int sirius_hip_autotune(SiriusHipHandle hh, const SiriusHipCamera* cams, int n_cam, const SiriusHipMetric* metrics,
+                        int n_metric, SiriusHipTraceOpts* opts, double* best_ms) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    GUARD_BEGIN
+    if (!opts) return H->fail(SIRIUS_HIP_E_INVALID, "opts");
+    int rc = H->ensure_device();
+    if (rc) return rc;
+    if (opts->precision != SIRIUS_PRECISION_FP32_ESCALATE || opts->integrator != SIRIUS_INTEGRATOR_DP45)
+        return H->fail(SIRIUS_HIP_E_UNSUPPORTED, "autotune applies to the FP32 persistent DP45 path");
+    Batch B;
+    if ((rc = prepare_batch(*H, cams, n_cam, metrics, n_metric, *opts, B))) return rc;
+    int64_t stride = std::max<int64_t>(1, B.n / std::max<int64_t>(65536, B.n / 16));
+    std::vector<int32_t> ids;
+    for (int64_t i = 0; i < B.n; i += stride) ids.push_back((int32_t)i);
+    int32_t* list = (int32_t*)H->scratch("tune_list", 4 * ids.size(), &rc);
+    uint32_t* cnt = (uint32_t*)H->scratch("tune_cnt", 4, &rc);
+    if (rc) return rc;
+    uint32_t nn = (uint32_t)ids.size();
+    hipError_t e;
+    if ((e = hipMemcpy(list, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice)) != hipSuccess)
+        return H->hip_fail(e, "upload");
+    if ((e = hipMemcpy(cnt, &nn, 4, hipMemcpyHostToDevice)) != hipSuccess) return H->hip_fail(e, "upload");
+    DevOut dout;
+    if ((rc = bind_outputs(*H, nullptr, B.n, opts->n_bins, dout))) return rc;
+    double best = 1e300;
+    SiriusHipTraceOpts bo = *opts;
+    const int blocks[3] = {64, 128, 256};
+    const int refills[4] = {1, 8, 16, 32};
+    for (int bs : blocks) {
+        int occ = persistent_blocks_per_cu(false, B.family, bs);
+        int bpcs[3] = {occ, std::max(1, occ / 2), std::max(1, occ / 4)};
+        for (int j = 0; j < 3; ++j) {
+            if (j > 0 && bpcs[j] == bpcs[j - 1]) continue;
+            for (int rf : refills) {
+                SiriusHipTraceOpts t = *opts;
+                t.block_size = bs;
+                t.blocks_per_cu = bpcs[j];
+                t.refill_min = rf;
+                t.use_surrogate = 0;
+                SiriusHipStats st;
+                std::memset(&st, 0, sizeof(st));
+                rc = run_trace(*H, B, t, dout, list, cnt, (int64_t)ids.size(), &st, true);
+                if (rc) return rc;
+                if (st.kernel_ms < best) {
+                    best = st.kernel_ms;
+                    bo = t;
+                }
+            }
+        }
+    }
+    opts->block_size = bo.block_size;
+    opts->blocks_per_cu = bo.blocks_per_cu;
+    opts->refill_min = bo.refill_min;
+    if (best_ms) *best_ms = best;
+    return SIRIUS_HIP_OK;
+    GUARD_END(H)
+}
+
+}  // extern "C"

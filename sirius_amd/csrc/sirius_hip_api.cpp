@@ -1,0 +1,709 @@
+// sirius_hip_api.cpp — the extern "C" boundary of libsirius_hip.so (include/sirius_hip.h).
+//
+// Lifecycle and framebuffer calls mirror the OptiX backend's C API
+// (src/Sirius.Render/Acceleration/OptiX/RDOP006A.h:10-39; implementation
+// src/Sirius.Render/Acceleration/OptiX/RDOP001A.cu:1367-1458): opaque handle, bool-like
+// status, no exception crosses the boundary. The physics entry point sirius_hip_trace_batch
+// is new. One device, one stream per handle (RDOP001A.cu:621,631).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sirius_hip.h"
+#include "geodesic.hpp"
+#include "sirius_internal.hpp"
+
+namespace sirius {
+
+// ---- kernels defined in trace_kernels.hip / util_kernels.hip ----
+hipError_t launch_fill(uint32_t* p, int n, uint32_t v, hipStream_t st);
+hipError_t launch_trace_grid(bool fp64, int family, const CamConst* cams, const TraceParams& tp, const DevOut& o,
+                             const int32_t* list, int64_t n, const uint32_t* count_dev, uint32_t flag_or, int block,
+                             hipStream_t st);
+hipError_t launch_trace_persistent(bool fp64, int family, const CamConst* cams, const TraceParams& tp,
+                                   const DevOut& o, const int32_t* list, const uint32_t* list_count, int64_t n,
+                                   uint32_t* head, uint32_t* esc_count, int32_t* esc_list,
+                                   unsigned long long* attempts_out, int grid, int block, int refill_min,
+                                   uint32_t flag_or, hipStream_t st);
+hipError_t launch_symplectic(bool fp64, int family, const CamConst* cams, const TraceParams& tp, const DevOut& o,
+                             int block, hipStream_t st);
+int persistent_blocks_per_cu(bool fp64, int family, int block);
+hipError_t launch_reduce_stats(const DevOut& o, int64_t n, unsigned long long* acc, hipStream_t st);
+hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float* curves, const int* curve_n,
+                                    int curve_stride, int64_t n, float* out, hipStream_t st);
+hipError_t launch_shade(const DevOut& o, int64_t n, int width, int height, int n_bins, const uint8_t* bg, int bgw,
+                        int bgh, float* fb, hipStream_t st);
+
+// ---------------------------------------------------------------------------------------
+// Host-side closed forms (FP64), shared with the camera constant setup.
+// ---------------------------------------------------------------------------------------
+double isco(double a_star) {
+    double a = std::fabs(a_star);
+    if (a > 1) a = 1;
+    double z1 = 1 + std::cbrt(1 - a * a) * (std::cbrt(1 + a) + std::cbrt(1 - a));
+    double z2 = std::sqrt(3 * a * a + z1 * z1);
+    double s = std::sqrt((3 - z1) * (3 + z1 + 2 * z2));
+    return a_star >= 0 ? 3 + z2 - s : 3 + z2 + s;
+}
+
+double photon_orbit(double a, bool prograde) {
+    double arg = std::clamp(prograde ? -a : a, -1.0, 1.0);
+    return 2.0 * (1.0 + std::cos((2.0 / 3.0) * std::acos(arg)));
+}
+
+static double pt_f_host(const CamConst& c, double r) {
+    double x = std::sqrt(r), x0 = c.pt_x0;
+    if (x <= x0) return 0.0;
+    double br = x - x0 - 1.5 * c.a * std::log(x / x0);
+    for (int i = 0; i < 3; ++i) br -= c.pt_c[i] * std::log((x - c.pt_x[i]) / (x0 - c.pt_x[i]));
+    return 1.5 / (x * x * (x * x * x - 3.0 * x + 2.0 * c.a)) * br;
+}
+
+static double tmax_f_over_r(const CamConst& c) {
+    double lo = std::max(c.r_in, c.pt_x0 * c.pt_x0), hi = c.r_out;
+    if (hi <= lo) return 0.0;
+    const int n = 4096;
+    double best = -1, rb = lo;
+    for (int i = 0; i <= n; ++i) {
+        double r = lo + (hi - lo) * i / n;
+        double v = pt_f_host(c, r) / r;
+        if (v > best) { best = v; rb = r; }
+    }
+    double h = (hi - lo) / n;
+    double A = std::max(rb - h, lo), B = std::min(rb + h, hi);
+    const double gr = 0.6180339887498949;
+    for (int it = 0; it < 200; ++it) {
+        double cc = B - gr * (B - A), d = A + gr * (B - A);
+        if (pt_f_host(c, cc) / cc > pt_f_host(c, d) / d) B = d; else A = cc;
+    }
+    double rr = 0.5 * (A + B);
+    return std::max(best, pt_f_host(c, rr) / rr);
+}
+
+// Covariant metric at a point, FP64 host copy of metric_cov (for the observer normalisation).
+static double gtt_at(const SiriusHipMetric& m, const double x[4]) {
+    if (m.family == SIRIUS_METRIC_MORRIS_THORNE) return -1.0;
+    if (m.family == SIRIUS_METRIC_ALCUBIERRE) {
+        double rs = std::sqrt(x[1] * x[1] + x[2] * x[2] + x[3] * x[3]);
+        double f = (std::tanh(m.warp_sigma * (rs + m.warp_R)) - std::tanh(m.warp_sigma * (rs - m.warp_R))) /
+                   (2.0 * std::tanh(m.warp_sigma * m.warp_R));
+        return -(1.0 - m.warp_v * m.warp_v * f * f);
+    }
+    double a2 = m.a * m.a;
+    double R2 = x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
+    double w = R2 - a2;
+    double r2 = 0.5 * (w + std::sqrt(std::max(w * w + 4 * a2 * x[3] * x[3], 1e-20)));
+    r2 = std::max(r2, 1e-10);
+    double r = std::sqrt(r2);
+    double H = (2 * m.M * r - m.Q * m.Q) * r2 / (r2 * r2 + a2 * x[3] * x[3]);
+    return -1.0 + H - m.Lambda * R2 / 3.0;
+}
+
+bool make_cam_const(const SiriusHipCamera& cam, const SiriusHipMetric& m, int64_t offset, CamConst& c,
+                    std::string& err) {
+    std::memset(&c, 0, sizeof(c));
+    if (cam.width <= 0 || cam.height <= 0) { err = "camera width/height must be positive"; return false; }
+    if (!(cam.fov_deg > 0 && cam.fov_deg < 180)) { err = "camera fov_deg must be in (0, 180)"; return false; }
+    if (!(cam.r_obs > 0)) { err = "camera r_obs must be positive"; return false; }
+    c.r_obs = cam.r_obs;
+    c.theta_obs = std::clamp(cam.theta_obs, 0.01, M_PI - 0.01);  // GTRC001A.cpp:25
+    c.phi_obs = cam.phi_obs;
+    c.tan_half = std::tan(cam.fov_deg * M_PI / 360.0);
+    c.aspect = (double)cam.width / cam.height;
+    c.cyaw = std::cos(cam.yaw); c.syaw = std::sin(cam.yaw);
+    c.cpitch = std::cos(cam.pitch); c.spitch = std::sin(cam.pitch);
+    c.croll = std::cos(cam.roll); c.sroll = std::sin(cam.roll);
+    double st = std::sin(c.theta_obs), ct = std::cos(c.theta_obs), sp = std::sin(c.phi_obs), cp = std::cos(c.phi_obs);
+    c.er[0] = st * cp; c.er[1] = st * sp; c.er[2] = ct;
+    c.et[0] = ct * cp; c.et[1] = ct * sp; c.et[2] = -st;
+    c.ep[0] = -sp; c.ep[1] = cp; c.ep[2] = 0;
+    c.width = cam.width;
+    c.height = cam.height;
+    c.family = m.family;
+    c.offset = offset;
+    c.M = m.M; c.a = m.a; c.Q2 = m.Q * m.Q; c.Lambda = m.Lambda;
+    c.b0 = m.throat_b0;
+    c.wv = m.warp_v; c.wR = m.warp_R; c.wsig = m.warp_sigma;
+    if (m.family == SIRIUS_METRIC_ALCUBIERRE) {
+        if (!(m.warp_sigma > 0 && m.warp_R > 0)) { err = "warp sigma and R must be positive"; return false; }
+        c.winv2tanh = 1.0 / (2.0 * std::tanh(m.warp_sigma * m.warp_R));
+    }
+    if (m.family == SIRIUS_METRIC_MORRIS_THORNE) {
+        if (!(m.throat_b0 > 0) || cam.r_obs <= m.throat_b0) { err = "wormhole needs 0 < b0 < r_obs"; return false; }
+        c.l_obs = std::sqrt(cam.r_obs * cam.r_obs - m.throat_b0 * m.throat_b0);
+    }
+    if (m.family == SIRIUS_METRIC_KERR_SCHILD) {
+        double d = m.M * m.M - m.a * m.a - m.Q * m.Q;
+        if (!(m.M > 0) || d < 0) { err = "Kerr-Schild family needs M > 0 and M^2 >= a^2 + Q^2"; return false; }
+        if (std::fabs(m.M - 1.0) > 1e-12) { err = "units: set M = 1 (G = c = M = 1)"; return false; }
+        double rplus = m.M + std::sqrt(d);
+        c.r_cap = 0.0;  // set from opts by the caller
+        c.r_in = cam.disk_r_in > 0 ? cam.disk_r_in : isco(m.a);
+        c.r_out = cam.disk_r_out;
+        c.r_ph_retro = photon_orbit(std::fabs(m.a), false);
+        // Page-Thorne constants (M = 1)
+        double a = m.a;
+        c.pt_x0 = std::sqrt(isco(a));
+        double ac = std::acos(std::clamp(a, -1.0, 1.0));
+        c.pt_x[0] = 2.0 * std::cos((ac - M_PI) / 3.0);
+        c.pt_x[1] = 2.0 * std::cos((ac + M_PI) / 3.0);
+        c.pt_x[2] = -2.0 * std::cos(ac / 3.0);
+        for (int i = 0; i < 3; ++i) {
+            double xi = c.pt_x[i], xj = c.pt_x[(i + 1) % 3], xk = c.pt_x[(i + 2) % 3];
+            c.pt_c[i] = 3.0 * (xi - a) * (xi - a) / (xi * (xi - xj) * (xi - xk));
+        }
+        c.fmax = tmax_f_over_r(c);
+        c.r_cap = rplus;  // multiplied by horizon_factor in trace_batch
+    }
+    double x[4] = {0, c.r_obs * c.er[0], c.r_obs * c.er[1], c.r_obs * c.er[2]};
+    double gtt = gtt_at(m, x);
+    if (!(gtt < 0)) { err = "observer is not static (g_tt >= 0 at the camera)"; return false; }
+    c.obs_norm = 1.0 / std::sqrt(-gtt);
+    return true;
+}
+
+}  // namespace sirius
+
+using namespace sirius;
+
+// ---------------------------------------------------------------------------------------
+// Handle
+// ---------------------------------------------------------------------------------------
+namespace {
+
+const char* status_name(int rc) {
+    switch (rc) {
+        case SIRIUS_HIP_OK: return "ok";
+        case SIRIUS_HIP_E_INVALID: return "invalid argument";
+        case SIRIUS_HIP_E_DEVICE: return "device error";
+        case SIRIUS_HIP_E_NOMEM: return "out of device memory";
+        case SIRIUS_HIP_E_STATE: return "bad state";
+        default: return "unsupported";
+    }
+}
+
+}  // namespace
+
+Handle::~Handle() { release(); }
+
+void Handle::release() {
+    for (auto& kv : bufs) if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    bufs.clear();
+    if (fb_dev) (void)hipFree(fb_dev);
+    fb_dev = nullptr;
+    if (bg_dev) (void)hipFree(bg_dev);
+    bg_dev = nullptr;
+    mlp_release(*this);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (ev2) (void)hipEventDestroy(ev2);
+    if (ev3) (void)hipEventDestroy(ev3);
+    if (ev4) (void)hipEventDestroy(ev4);
+    ev0 = ev1 = ev2 = ev3 = ev4 = nullptr;
+    delete last;
+    last = nullptr;
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+    ready = false;
+}
+
+int Handle::fail(int rc, const std::string& msg) {
+    last_error = std::string(status_name(rc)) + ": " + msg;
+    return rc;
+}
+
+int Handle::hip_fail(hipError_t e, const char* what) {
+    last_error = std::string("device error: ") + what + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? SIRIUS_HIP_E_NOMEM : SIRIUS_HIP_E_DEVICE;
+}
+
+int Handle::ensure_device() {
+    if (ready) return SIRIUS_HIP_OK;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) return fail(SIRIUS_HIP_E_DEVICE, "no HIP device visible");
+    e = hipSetDevice(0);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, 0);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(SIRIUS_HIP_E_DEVICE, std::string("built for gfx950, device is ") + prop.gcnArchName);
+    cu_count = prop.multiProcessorCount;
+    max_threads = prop.maxThreadsPerBlock;
+    device_name = prop.name;
+    total_mem = prop.totalGlobalMem;
+    if ((e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "stream");
+    if ((e = hipEventCreate(&ev0)) != hipSuccess) return hip_fail(e, "event");
+    if ((e = hipEventCreate(&ev1)) != hipSuccess) return hip_fail(e, "event");
+    if ((e = hipEventCreate(&ev2)) != hipSuccess) return hip_fail(e, "event");
+    if ((e = hipEventCreate(&ev3)) != hipSuccess) return hip_fail(e, "event");
+    if ((e = hipEventCreate(&ev4)) != hipSuccess) return hip_fail(e, "event");
+    ready = true;
+    return SIRIUS_HIP_OK;
+}
+
+void* Handle::scratch(const char* name, size_t bytes, int* rc) {
+    Buf& b = bufs[name];
+    if (b.bytes >= bytes && b.ptr) return b.ptr;
+    if (b.ptr) (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+    hipError_t e = hipMalloc(&b.ptr, std::max<size_t>(bytes, 256));
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        *rc = hip_fail(e, "hipMalloc");
+        return nullptr;
+    }
+    b.bytes = bytes;
+    return b.ptr;
+}
+
+// ---------------------------------------------------------------------------------------
+// Trace pipeline
+// ---------------------------------------------------------------------------------------
+namespace sirius {
+
+static int validate_opts(Handle& H, const SiriusHipTraceOpts& o) {
+    if (o.n_bins < 0 || o.n_bins > 64) return H.fail(SIRIUS_HIP_E_INVALID, "n_bins must be in [0, 64]");
+    if (!(o.rtol > 0) || !(o.atol > 0)) return H.fail(SIRIUS_HIP_E_INVALID, "rtol and atol must be positive");
+    if (!(o.h_init > 0) || !(o.h_min > 0) || !(o.h_max >= o.h_min))
+        return H.fail(SIRIUS_HIP_E_INVALID, "need 0 < h_min <= h_max and h_init > 0");
+    if (o.max_steps <= 0) return H.fail(SIRIUS_HIP_E_INVALID, "max_steps must be positive");
+    if (!(o.r_escape > 0) || !(o.horizon_factor >= 1.0))
+        return H.fail(SIRIUS_HIP_E_INVALID, "need r_escape > 0 and horizon_factor >= 1");
+    if (!(o.x_min > 0 && o.x_max > o.x_min)) return H.fail(SIRIUS_HIP_E_INVALID, "need 0 < x_min < x_max");
+    if (o.integrator != SIRIUS_INTEGRATOR_DP45 && o.integrator != SIRIUS_INTEGRATOR_SYMPLECTIC)
+        return H.fail(SIRIUS_HIP_E_INVALID, "integrator");
+    if (o.precision != SIRIUS_PRECISION_FP64 && o.precision != SIRIUS_PRECISION_FP32_ESCALATE)
+        return H.fail(SIRIUS_HIP_E_INVALID, "precision");
+    if (o.emission < 0 || o.emission > 2) return H.fail(SIRIUS_HIP_E_INVALID, "emission");
+    if (o.schedule != SIRIUS_SCHEDULE_GRID && o.schedule != SIRIUS_SCHEDULE_PERSISTENT)
+        return H.fail(SIRIUS_HIP_E_INVALID, "schedule");
+    if (o.block_size != 0 && (o.block_size < 64 || o.block_size > 1024 || o.block_size % 64))
+        return H.fail(SIRIUS_HIP_E_INVALID, "block_size must be 0 or a multiple of 64 in [64, 1024]");
+    if (o.integrator == SIRIUS_INTEGRATOR_SYMPLECTIC && !(o.symplectic_h > 0))
+        return H.fail(SIRIUS_HIP_E_INVALID, "symplectic_h must be positive");
+    return SIRIUS_HIP_OK;
+}
+
+int prepare_batch(Handle& H, const SiriusHipCamera* cams, int n_cam, const SiriusHipMetric* metrics, int n_metric,
+                  const SiriusHipTraceOpts& o, Batch& B) {
+    if (!cams || n_cam <= 0 || !metrics || n_metric <= 0) return H.fail(SIRIUS_HIP_E_INVALID, "cameras/metrics");
+    int rc = validate_opts(H, o);
+    if (rc) return rc;
+    B.cc.resize(n_cam);
+    int64_t off = 0;
+    B.family = -1;
+    for (int i = 0; i < n_cam; ++i) {
+        if (cams[i].metric_index < 0 || cams[i].metric_index >= n_metric)
+            return H.fail(SIRIUS_HIP_E_INVALID, "camera metric_index out of range");
+        const SiriusHipMetric& m = metrics[cams[i].metric_index];
+        if (m.family < 0 || m.family > 2) return H.fail(SIRIUS_HIP_E_INVALID, "metric family");
+        if (B.family < 0) B.family = m.family;
+        if (B.family != m.family) return H.fail(SIRIUS_HIP_E_UNSUPPORTED, "one metric family per batch");
+        std::string err;
+        if (!make_cam_const(cams[i], m, off, B.cc[i], err)) return H.fail(SIRIUS_HIP_E_INVALID, err);
+        B.cc[i].r_cap *= o.horizon_factor;
+        off += (int64_t)cams[i].width * cams[i].height;
+    }
+    if (off <= 0 || off > (int64_t)INT32_MAX - (1 << 20)) return H.fail(SIRIUS_HIP_E_INVALID, "ray count out of range");
+    B.n = off;
+    TraceParams& tp = B.tp;
+    std::memset(&tp, 0, sizeof(tp));
+    tp.rtol = o.rtol; tp.atol = o.atol; tp.h_init = o.h_init; tp.h_min = o.h_min; tp.h_max = o.h_max;
+    tp.far_slope = o.precision == SIRIUS_PRECISION_FP32_ESCALATE ? o.far_slope : 0.0;
+    tp.r_escape = o.r_escape;
+    tp.x_min = o.x_min; tp.x_max = o.x_max;
+    tp.escalate_margin = o.escalate_margin;
+    tp.sym_h = o.symplectic_h;
+    tp.vol_j0 = o.vol_j0; tp.vol_h_over_r = o.vol_h_over_r; tp.vol_kappa = o.vol_kappa;
+    tp.vol_r_min = o.vol_r_min; tp.vol_r_max = o.vol_r_max; tp.vol_tau_max = o.vol_tau_max;
+    tp.max_steps = o.max_steps;
+    tp.n_bins = o.n_bins;
+    tp.emission = B.family == SIRIUS_METRIC_KERR_SCHILD ? o.emission : SIRIUS_EMISSION_NONE;
+    tp.sym_order = o.symplectic_order;
+    tp.n_cam = n_cam;
+    tp.escalate = (o.precision == SIRIUS_PRECISION_FP32_ESCALATE && o.escalate_margin > 0) ? 1 : 0;
+    tp.n_rays = off;
+    if (tp.emission == SIRIUS_EMISSION_VOLUMETRIC && !(o.vol_h_over_r > 0 && o.vol_r_max > o.vol_r_min))
+        return H.fail(SIRIUS_HIP_E_INVALID, "volumetric parameters");
+    void* d = H.scratch("cams", sizeof(CamConst) * n_cam, &rc);
+    if (!d) return rc;
+    B.cams_dev = (CamConst*)d;
+    hipError_t e = hipMemcpyAsync(d, B.cc.data(), sizeof(CamConst) * n_cam, hipMemcpyHostToDevice, H.stream);
+    if (e != hipSuccess) return H.hip_fail(e, "upload cameras");
+    return SIRIUS_HIP_OK;
+}
+
+// Device output arrays: the caller's (on_device) or library scratch.
+int bind_outputs(Handle& H, const SiriusHipRayOutput* out, int64_t n, int n_bins, DevOut& o) {
+    int rc = SIRIUS_HIP_OK;
+    auto get = [&](const char* name, void* user, size_t bytes) -> void* {
+        if (out && out->on_device && user) return user;
+        return H.scratch(name, bytes, &rc);
+    };
+    o.n_rays = n;
+    o.cls = (uint8_t*)get("o_cls", out ? out->cls : nullptr, n);
+    o.flags = (uint8_t*)get("o_flags", out ? out->flags : nullptr, n);
+    o.theta = (float*)get("o_theta", out ? out->theta : nullptr, 4 * n);
+    o.phi = (float*)get("o_phi", out ? out->phi : nullptr, 4 * n);
+    o.lam = (float*)get("o_lam", out ? out->lambda : nullptr, 4 * n);
+    o.disk_r = (float*)get("o_disk_r", out ? out->disk_r : nullptr, 4 * n);
+    o.redshift = (float*)get("o_redshift", out ? out->redshift : nullptr, 4 * n);
+    o.null_drift = (float*)get("o_null", out ? out->null_drift : nullptr, 4 * n);
+    o.steps = (uint32_t*)get("o_steps", out ? out->steps : nullptr, 4 * n);
+    o.rejected = (uint32_t*)H.scratch("o_rej", 4 * n, &rc);
+    o.landings = (uint32_t*)H.scratch("o_land", 4 * n, &rc);
+    o.intensity = (float*)get("o_int", out ? out->intensity : nullptr, 4 * n * std::max(n_bins, 1));
+    o.state = nullptr;
+    if (out && out->state) o.state = (double*)get("o_state", out->state, 64 * n);
+    return rc;
+}
+
+int copy_outputs_to_host(Handle& H, const SiriusHipRayOutput* out, const DevOut& o, int n_bins) {
+    if (!out || out->on_device) return SIRIUS_HIP_OK;
+    int64_t n = o.n_rays;
+    struct { void* dst; const void* src; size_t bytes; } c[] = {
+        {out->cls, o.cls, (size_t)n},           {out->flags, o.flags, (size_t)n},
+        {out->theta, o.theta, 4 * (size_t)n},   {out->phi, o.phi, 4 * (size_t)n},
+        {out->lambda, o.lam, 4 * (size_t)n},    {out->disk_r, o.disk_r, 4 * (size_t)n},
+        {out->redshift, o.redshift, 4 * (size_t)n}, {out->null_drift, o.null_drift, 4 * (size_t)n},
+        {out->steps, o.steps, 4 * (size_t)n},   {out->intensity, o.intensity, 4 * (size_t)n * n_bins},
+        {out->state, o.state, 64 * (size_t)n},
+    };
+    for (auto& x : c) {
+        if (!x.dst || !x.src || !x.bytes) continue;
+        hipError_t e = hipMemcpyAsync(x.dst, x.src, x.bytes, hipMemcpyDeviceToHost, H.stream);
+        if (e != hipSuccess) return H.hip_fail(e, "copy results to host");
+    }
+    hipError_t e = hipStreamSynchronize(H.stream);
+    if (e != hipSuccess) return H.hip_fail(e, "synchronize");
+    return SIRIUS_HIP_OK;
+}
+
+// Per-family algorithmic flops of one DP45 attempt (6 RHS + 426, fixed constants).
+double flops_per_step(int family) {
+    switch (family) {
+        case 1: return 576.0;
+        case 2: return 678.0;
+        default: return 1326.0;
+    }
+}
+
+int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, const int32_t* list_dev,
+              const uint32_t* list_count_dev, int64_t list_cap, SiriusHipStats* stats, bool time_it) {
+    hipError_t e;
+    int rc = SIRIUS_HIP_OK;
+    bool fp64 = o.precision == SIRIUS_PRECISION_FP64;
+    int block = o.block_size > 0 ? o.block_size : 256;
+    int64_t n_work = list_dev ? list_cap : B.n;
+    bool persistent = o.schedule == SIRIUS_SCHEDULE_PERSISTENT && o.integrator == SIRIUS_INTEGRATOR_DP45;
+    // work[0] bulk queue head, work[1] escalation count, work[2..3] bulk attempts (u64),
+    // work[4] escalation queue head, work[5] unused, work[6..7] escalation attempts (u64)
+    uint32_t* work = (uint32_t*)H.scratch("work", 64, &rc);
+    if (rc) return rc;
+    if ((e = launch_fill(work, 16, 0u, H.stream)) != hipSuccess) return H.hip_fail(e, "fill");
+    if (time_it) (void)hipEventRecord(H.ev0, H.stream);
+    if (o.integrator == SIRIUS_INTEGRATOR_SYMPLECTIC) {
+        if (list_dev) return H.fail(SIRIUS_HIP_E_UNSUPPORTED, "symplectic integrator with a ray list");
+        if ((e = launch_symplectic(fp64, B.family, B.cams_dev, B.tp, dout, block, H.stream)) != hipSuccess)
+            return H.hip_fail(e, "symplectic launch");
+    } else if (!persistent) {
+        // the grid schedule has no escalation queue: FP32 lanes run to the end in FP32
+        TraceParams tp = B.tp;
+        tp.escalate = 0;
+        if ((e = launch_trace_grid(fp64, B.family, B.cams_dev, tp, dout, list_dev, n_work, list_count_dev, 0,
+                                   block, H.stream)) != hipSuccess)
+            return H.hip_fail(e, "grid launch");
+    } else {
+        int32_t* esc = (int32_t*)H.scratch("esc_list", 4 * (size_t)std::max<int64_t>(n_work, 1), &rc);
+        if (rc) return rc;
+        int bpc = o.blocks_per_cu > 0 ? o.blocks_per_cu : persistent_blocks_per_cu(fp64, B.family, block);
+        int grid = std::max(1, bpc * H.cu_count);
+        int refill = o.refill_min > 0 ? std::min(o.refill_min, 64) : 16;
+        TraceParams tp = B.tp;
+        if (fp64) tp.escalate = 0;
+        if ((e = launch_trace_persistent(fp64, B.family, B.cams_dev, tp, dout, list_dev, list_count_dev, n_work, work,
+                                         work + 1, esc, (unsigned long long*)(work + 2), grid, block, refill, 0,
+                                         H.stream)) != hipSuccess)
+            return H.hip_fail(e, "persistent launch");
+        if (time_it) (void)hipEventRecord(H.ev1, H.stream);
+        if (tp.escalate) {
+            // Escalated rays are re-traced from the camera by the FP64 persistent kernel at the
+            // FP64 contract's tolerances, so they carry exactly the plain FP64 answer.
+            SiriusHipTraceOpts d64;
+            sirius_hip_default_opts(&d64, SIRIUS_PRECISION_FP64);
+            TraceParams t64 = B.tp;
+            t64.rtol = d64.rtol;
+            t64.atol = d64.atol;
+            t64.h_init = d64.h_init;
+            t64.h_min = d64.h_min;
+            t64.h_max = d64.h_max;
+            t64.far_slope = 0.0;
+            t64.escalate = 0;
+            int bpc64 = persistent_blocks_per_cu(true, B.family, 256);
+            if ((e = launch_trace_persistent(true, B.family, B.cams_dev, t64, dout, esc, work + 1, n_work, work + 4,
+                                             work + 5, esc, (unsigned long long*)(work + 6),
+                                             std::max(1, bpc64 * H.cu_count), 256, 16, SIRIUS_FLAG_ESCALATED,
+                                             H.stream)) != hipSuccess)
+                return H.hip_fail(e, "escalation launch");
+        }
+    }
+    if (time_it) {
+        if (!persistent) (void)hipEventRecord(H.ev1, H.stream);
+        (void)hipEventRecord(H.ev4, H.stream);
+    }
+    if (stats) {
+        unsigned long long* acc = (unsigned long long*)H.scratch("stat_acc", 8 * 8, &rc);
+        if (rc) return rc;
+        if ((e = launch_fill((uint32_t*)acc, 16, 0u, H.stream)) != hipSuccess) return H.hip_fail(e, "fill");
+        if ((e = launch_reduce_stats(dout, B.n, acc, H.stream)) != hipSuccess) return H.hip_fail(e, "reduce");
+        unsigned long long h[8] = {0};
+        if ((e = hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, H.stream)) != hipSuccess)
+            return H.hip_fail(e, "stats copy");
+        uint32_t wk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (persistent) {
+            if ((e = hipMemcpyAsync(wk, H.bufs["work"].ptr, sizeof(wk), hipMemcpyDeviceToHost, H.stream)) != hipSuccess)
+                return H.hip_fail(e, "stats copy");
+        }
+        if ((e = hipStreamSynchronize(H.stream)) != hipSuccess) return H.hip_fail(e, "synchronize");
+        // h: attempts, rejected, landings, drift_max bits, escalated, surrogate, audited, esc attempts
+        stats->rays += (uint64_t)B.n;
+        stats->steps_accepted += h[0] - h[1];
+        stats->steps_rejected += h[1];
+        stats->steps_landing += h[2];
+        double dm;
+        std::memcpy(&dm, &h[3], sizeof(dm));
+        stats->null_drift_max = std::max(stats->null_drift_max, dm);
+        stats->rays_escalated += h[4];
+        stats->rays_audited += h[6];
+        unsigned long long bulk = h[0];
+        if (persistent) {
+            unsigned long long esc_att = 0;
+            std::memcpy(&bulk, &wk[2], sizeof(bulk));
+            std::memcpy(&esc_att, &wk[6], sizeof(esc_att));
+            stats->steps_escalated += esc_att;
+            stats->flops_escalated += (double)esc_att * flops_per_step(B.family);
+        }
+        stats->steps_bulk += bulk;
+        stats->flops += (double)bulk * flops_per_step(B.family);
+        if (time_it) {
+            float ms = 0, ms2 = 0;
+            (void)hipEventElapsedTime(&ms, H.ev0, H.ev1);
+            (void)hipEventElapsedTime(&ms2, H.ev1, H.ev4);
+            stats->kernel_ms += ms;
+            stats->escalate_ms += ms2;
+        }
+    }
+    return SIRIUS_HIP_OK;
+}
+
+}  // namespace sirius
+
+// ---------------------------------------------------------------------------------------
+// extern "C" entry points
+// ---------------------------------------------------------------------------------------
+#define GUARD_BEGIN try {
+#define GUARD_END(h)                                                                     \
+    }                                                                                    \
+    catch (const std::bad_alloc&) { return (h)->fail(SIRIUS_HIP_E_NOMEM, "host allocation"); } \
+    catch (...) { return (h)->fail(SIRIUS_HIP_E_DEVICE, "unexpected exception"); }
+
+extern "C" {
+
+int sirius_hip_abi_version(void) { return SIRIUS_HIP_ABI_VERSION; }
+
+SiriusHipHandle sirius_hip_create(void) {
+    try {
+        return new Handle();
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+void sirius_hip_destroy(SiriusHipHandle h) {
+    if (!h) return;
+    try {
+        delete static_cast<Handle*>(h);
+    } catch (...) {
+    }
+}
+
+const char* sirius_hip_last_error(SiriusHipHandle h) {
+    if (!h) return "null handle";
+    return static_cast<Handle*>(h)->last_error.c_str();
+}
+
+int sirius_hip_initialize(SiriusHipHandle hh, int width, int height) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    GUARD_BEGIN
+    if (width <= 0 || height <= 0) return H->fail(SIRIUS_HIP_E_INVALID, "framebuffer size");
+    int rc = H->ensure_device();
+    if (rc) return rc;
+    return sirius_hip_resize(hh, width, height);
+    GUARD_END(H)
+}
+
+int sirius_hip_resize(SiriusHipHandle hh, int width, int height) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    GUARD_BEGIN
+    if (width <= 0 || height <= 0) return H->fail(SIRIUS_HIP_E_INVALID, "framebuffer size");
+    int rc = H->ensure_device();
+    if (rc) return rc;
+    if (H->fb_dev) (void)hipFree(H->fb_dev);
+    H->fb_dev = nullptr;
+    size_t bytes = (size_t)width * height * 4 * sizeof(float);
+    hipError_t e = hipMalloc(&H->fb_dev, bytes);
+    if (e != hipSuccess) { H->fb_dev = nullptr; return H->hip_fail(e, "framebuffer"); }
+    H->fb_host.assign((size_t)width * height * 4, 0.f);
+    H->fb_w = width;
+    H->fb_h = height;
+    H->initialised = true;
+    return SIRIUS_HIP_OK;
+    GUARD_END(H)
+}
+
+void sirius_hip_cleanup(SiriusHipHandle hh) {
+    if (!hh) return;
+    Handle* H = static_cast<Handle*>(hh);
+    try {
+        H->release();
+        H->initialised = false;
+        H->fb_host.clear();
+    } catch (...) {
+    }
+}
+
+int sirius_hip_is_initialized(SiriusHipHandle hh) {
+    return hh && static_cast<Handle*>(hh)->initialised ? 1 : 0;
+}
+
+int sirius_hip_device_info(SiriusHipHandle hh, char* name, int name_len, size_t* total_mem, size_t* free_mem,
+                           int* cu_count, int* max_threads_per_block) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    GUARD_BEGIN
+    int rc = H->ensure_device();
+    if (rc) return rc;
+    if (name && name_len > 0) {
+        std::snprintf(name, (size_t)name_len, "%s", H->device_name.c_str());
+    }
+    size_t fr = 0, tot = 0;
+    (void)hipMemGetInfo(&fr, &tot);
+    if (total_mem) *total_mem = tot;
+    if (free_mem) *free_mem = fr;
+    if (cu_count) *cu_count = H->cu_count;
+    if (max_threads_per_block) *max_threads_per_block = H->max_threads;
+    return SIRIUS_HIP_OK;
+    GUARD_END(H)
+}
+
+void sirius_hip_default_opts(SiriusHipTraceOpts* o, int precision) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->integrator = SIRIUS_INTEGRATOR_DP45;
+    o->precision = precision;
+    o->emission = SIRIUS_EMISSION_THIN_DISK;
+    o->schedule = precision == SIRIUS_PRECISION_FP64 ? SIRIUS_SCHEDULE_GRID : SIRIUS_SCHEDULE_PERSISTENT;
+    o->rtol = 1e-8;
+    o->atol = 1e-10;
+    o->h_init = 0.01;
+    o->h_min = 1e-8;
+    o->h_max = 2.0;
+    o->far_slope = 0.0;
+    o->symplectic_h = 0.01;
+    o->symplectic_order = 4;
+    o->max_steps = 20000;
+    o->r_escape = 1000.0;
+    o->horizon_factor = 1.01;
+    o->n_bins = 16;
+    o->x_min = 0.1;
+    o->x_max = 10.0;
+    o->escalate_margin = 1.0;
+    o->surrogate_band = 0.5;
+    o->surrogate_pmin = 0.99;
+    o->audit_fraction = 0.01;
+    o->audit_seed = 0x51121007ULL;
+    o->vol_j0 = 1.0;
+    o->vol_h_over_r = 0.1;
+    o->vol_kappa = 0.3;
+    o->vol_r_min = 2.0;
+    o->vol_r_max = 50.0;
+    o->vol_tau_max = 10.0;
+    if (precision == SIRIUS_PRECISION_FP32_ESCALATE) {
+        o->rtol = 1e-6;
+        o->atol = 1e-8;
+        o->far_slope = 0.25;
+    }
+}
+
+int sirius_hip_trace_batch(SiriusHipHandle hh, const SiriusHipCamera* cams, int n_cam, const SiriusHipMetric* metrics,
+                           int n_metric, const SiriusHipTraceOpts* opts, SiriusHipRayOutput* out,
+                           SiriusHipStats* stats) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    GUARD_BEGIN
+    if (!opts || !out) return H->fail(SIRIUS_HIP_E_INVALID, "opts and out are required");
+    int rc = H->ensure_device();
+    if (rc) return rc;
+    Batch B;
+    if ((rc = prepare_batch(*H, cams, n_cam, metrics, n_metric, *opts, B))) return rc;
+    DevOut dout;
+    if ((rc = bind_outputs(*H, out, B.n, opts->n_bins, dout))) return rc;
+    if (!H->last) H->last = new DevOut;
+    *H->last = dout;
+    H->last_bins = opts->n_bins;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    (void)hipEventRecord(H->ev2, H->stream);
+    if (opts->use_surrogate) {
+        rc = surrogate_trace(*H, B, *opts, dout, stats);
+    } else {
+        rc = run_trace(*H, B, *opts, dout, nullptr, nullptr, 0, stats, true);
+    }
+    if (rc) return rc;
+    (void)hipEventRecord(H->ev3, H->stream);
+    hipError_t e = hipEventSynchronize(H->ev3);
+    if (e != hipSuccess) return H->hip_fail(e, "trace");
+    if (stats) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, H->ev2, H->ev3);
+        stats->total_ms = ms;
+    }
+    return copy_outputs_to_host(*H, out, dout, opts->n_bins);
+    GUARD_END(H)
+}
+
+int sirius_hip_fetch_results(SiriusHipHandle hh, SiriusHipRayOutput* host_out) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    GUARD_BEGIN
+    if (!host_out) return H->fail(SIRIUS_HIP_E_INVALID, "host_out");
+    if (!H->last) return H->fail(SIRIUS_HIP_E_STATE, "no trace_batch results to fetch");
+    SiriusHipRayOutput o = *host_out;
+    o.on_device = 0;
+    return copy_outputs_to_host(*H, &o, *H->last, H->last_bins);
+    GUARD_END(H)
+}
+
+int sirius_hip_synchronize(SiriusHipHandle hh) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    GUARD_BEGIN
+    if (!H->ready) return SIRIUS_HIP_OK;
+    hipError_t e = hipStreamSynchronize(H->stream);
+    if (e != hipSuccess) return H->hip_fail(e, "synchronize");
+    return SIRIUS_HIP_OK;
+    GUARD_END(H)
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// sirius_internal.hpp — host-side state shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/sirius_hip.h"
+#include "geodesic.hpp"
+
+namespace sirius {
+
+struct DevOut;
+
+struct Buf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct MlpState;
+
+// One handle = one device (device 0), one stream, cached device scratch.
+struct Handle {
+    bool ready = false;
+    bool initialised = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr, ev4 = nullptr;
+    int cu_count = 0;
+    int max_threads = 0;
+    size_t total_mem = 0;
+    std::string device_name;
+    std::string last_error;
+    std::map<std::string, Buf> bufs;
+    // framebuffer (IAccelerator::initialise / getFrameBuffer)
+    float* fb_dev = nullptr;
+    std::vector<float> fb_host;
+    int fb_w = 0, fb_h = 0;
+    uint8_t* bg_dev = nullptr;
+    int bg_w = 0, bg_h = 0;
+    // results of the last trace_batch held in device memory (sirius_hip_fetch_results)
+    DevOut* last = nullptr;
+    int last_bins = 0;
+    // surrogate network
+    MlpState* mlp = nullptr;
+
+    ~Handle();
+    void release();
+    int fail(int rc, const std::string& msg);
+    int hip_fail(hipError_t e, const char* what);
+    int ensure_device();
+    void* scratch(const char* name, size_t bytes, int* rc);
+};
+
+struct Batch {
+    std::vector<CamConst> cc;
+    CamConst* cams_dev = nullptr;
+    TraceParams tp;
+    int family = 0;
+    int64_t n = 0;
+};
+
+double isco(double a_star);
+double photon_orbit(double a, bool prograde);
+bool make_cam_const(const SiriusHipCamera& cam, const SiriusHipMetric& m, int64_t offset, CamConst& c,
+                    std::string& err);
+int prepare_batch(Handle& H, const SiriusHipCamera* cams, int n_cam, const SiriusHipMetric* metrics, int n_metric,
+                  const SiriusHipTraceOpts& o, Batch& B);
+int bind_outputs(Handle& H, const SiriusHipRayOutput* out, int64_t n, int n_bins, DevOut& o);
+int copy_outputs_to_host(Handle& H, const SiriusHipRayOutput* out, const DevOut& o, int n_bins);
+int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, const int32_t* list_dev,
+              const uint32_t* list_count_dev, int64_t list_cap, SiriusHipStats* stats, bool time_it);
+double flops_per_step(int family);
+
+// surrogate (mlp_kernels.hip / surrogate.cpp)
+void mlp_release(Handle& H);
+int surrogate_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, SiriusHipStats* stats);
+
+}  // namespace sirius

@@ -1,0 +1,155 @@
+// util_kernels.hip — small kernels around the tracer: statistics reduction, distance of each
+// ray from the critical curve (the surrogate's fallback band), and RGBA shading of the
+// framebuffer that IAccelerator::getFrameBuffer returns.
+#include <hip/hip_runtime.h>
+
+#include "../../include/sirius_hip.h"
+#include "geodesic.hpp"
+
+namespace sirius {
+
+// acc[0] attempts, acc[1] rejected, acc[2] landings, acc[3] max null drift (double bits),
+// acc[4] escalated rays, acc[5] surrogate rays, acc[6] audited rays, acc[7] attempts of the
+// escalated rays.
+__global__ __launch_bounds__(256) void k_reduce_stats(DevOut o, int64_t n, unsigned long long* acc) {
+    unsigned long long a0 = 0, a1 = 0, a2 = 0, a4 = 0, a5 = 0, a6 = 0, a7 = 0;
+    double dmax = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        a0 += o.steps[i];
+        a1 += o.rejected[i];
+        a2 += o.landings[i];
+        uint8_t f = o.flags[i];
+        a4 += (f & SIRIUS_FLAG_ESCALATED) ? 1 : 0;
+        a7 += (f & SIRIUS_FLAG_ESCALATED) ? o.steps[i] : 0;
+        a5 += (f & SIRIUS_FLAG_SURROGATE) ? 1 : 0;
+        a6 += (f & SIRIUS_FLAG_AUDITED) ? 1 : 0;
+        double d = (double)o.null_drift[i];
+        if (d > dmax) dmax = d;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off);
+        a1 += __shfl_xor(a1, off);
+        a2 += __shfl_xor(a2, off);
+        a4 += __shfl_xor(a4, off);
+        a5 += __shfl_xor(a5, off);
+        a6 += __shfl_xor(a6, off);
+        a7 += __shfl_xor(a7, off);
+        double t = __shfl_xor(dmax, off);
+        dmax = t > dmax ? t : dmax;
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(&acc[0], a0);
+        atomicAdd(&acc[1], a1);
+        atomicAdd(&acc[2], a2);
+        atomicAdd(&acc[4], a4);
+        atomicAdd(&acc[5], a5);
+        atomicAdd(&acc[6], a6);
+        atomicAdd(&acc[7], a7);
+        atomicMax(&acc[3], (unsigned long long)__double_as_longlong(dmax));
+    }
+}
+
+hipError_t launch_reduce_stats(const DevOut& o, int64_t n, unsigned long long* acc, hipStream_t st) {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_reduce_stats, dim3((unsigned)blocks), dim3(256), 0, st, o, n, acc);
+    return hipGetLastError();
+}
+
+// Distance in the (alpha, beta) plane from the camera's critical curve (sampled on the host,
+// curve_n[c] points of (alpha, |beta|), or a circle of radius -curve_n... see below), negative
+// inside. alpha = -L/(E sin th), beta = p_th/E at the ray's start, in FP64.
+__global__ __launch_bounds__(256) void k_critical_distance(const CamConst* __restrict__ cams, int n_cam,
+                                                           const float* __restrict__ curves,
+                                                           const int* __restrict__ curve_n, int stride, int64_t n,
+                                                           float* __restrict__ out) {
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    int cam = 0;
+    {
+        int lo = 0, hi = n_cam - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (cams[mid].offset <= id) lo = mid; else hi = mid - 1;
+        }
+        cam = lo;
+    }
+    const CamConst& c = cams[cam];
+    if (c.family != 0) { out[id] = 1e30f; return; }
+    int64_t pix = id - c.offset;
+    Spacetime<double> s = make_spacetime<double>(c);
+    double x[4], p[4];
+    init_state<double>(c, s, (int)(pix % c.width), (int)(pix / c.width), x, p);
+    double E = -p[0];
+    double L = x[1] * p[2] - x[2] * p[1];
+    double r = kerr_r(s, x[1], x[2], x[3]);
+    double ct = x[3] / r, st = sqrt(fmax(1.0 - ct * ct, 1e-300));
+    double pth = (ct / st) * (x[1] * p[1] + x[2] * p[2]) - r * st * p[3];
+    float al = (float)(-L / (E * st)), be = (float)fabs(pth / E);
+    const float* cv = curves + (int64_t)cam * stride * 2;
+    int m = curve_n[cam];
+    if (m < 0) {  // circle of radius cv[0]
+        out[id] = sqrtf(al * al + be * be) - cv[0];
+        return;
+    }
+    float best = 3.0e38f, cx = 0.f;
+    int ib = 0;
+    for (int i = 0; i < m; ++i) {
+        float da = al - cv[2 * i], db = be - cv[2 * i + 1];
+        float d = da * da + db * db;
+        if (d < best) { best = d; ib = i; }
+        cx += cv[2 * i];
+    }
+    cx /= (float)(m > 0 ? m : 1);
+    float rr = (al - cx) * (al - cx) + be * be;
+    float rc = (cv[2 * ib] - cx) * (cv[2 * ib] - cx) + cv[2 * ib + 1] * cv[2 * ib + 1];
+    float d = sqrtf(best);
+    out[id] = rr < rc ? -d : d;
+}
+
+hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float* curves, const int* curve_n,
+                                    int curve_stride, int64_t n, float* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_critical_distance, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cams, n_cam, curves,
+                       curve_n, curve_stride, n, out);
+    return hipGetLastError();
+}
+
+// RGBA shading of one camera's results: disk intensity bins split into red/green/blue thirds
+// by frequency, escaped rays sample the equirectangular background (black without one),
+// captured rays are black; x / (1 + x) tone curve.
+__global__ __launch_bounds__(256) void k_shade(DevOut o, int64_t n, int width, int height, int n_bins,
+                                               const uint8_t* __restrict__ bg, int bgw, int bgh,
+                                               float* __restrict__ fb) {
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    float rgb[3] = {0.f, 0.f, 0.f};
+    uint8_t cls = o.cls[id];
+    if (cls == SIRIUS_RAY_DISK && n_bins > 0) {
+        for (int k = 0; k < n_bins; ++k) {
+            int ch = (k * 3) / n_bins;
+            rgb[ch] += o.intensity[(int64_t)k * n + id];
+        }
+        for (int ch = 0; ch < 3; ++ch) rgb[ch] = rgb[ch] / (1.f + rgb[ch]);
+    } else if (cls == SIRIUS_RAY_ESCAPED && bg) {
+        float u = (o.phi[id] + 3.14159265f) / 6.2831853f, v = o.theta[id] / 3.14159265f;
+        int ix = min(bgw - 1, max(0, (int)(u * bgw))), iy = min(bgh - 1, max(0, (int)(v * bgh)));
+        const uint8_t* px = bg + 4 * ((int64_t)iy * bgw + ix);
+        rgb[0] = px[0] / 255.f;
+        rgb[1] = px[1] / 255.f;
+        rgb[2] = px[2] / 255.f;
+    }
+    fb[4 * id + 0] = rgb[0];
+    fb[4 * id + 1] = rgb[1];
+    fb[4 * id + 2] = rgb[2];
+    fb[4 * id + 3] = 1.f;
+}
+
+hipError_t launch_shade(const DevOut& o, int64_t n, int width, int height, int n_bins, const uint8_t* bg, int bgw,
+                        int bgh, float* fb, hipStream_t st) {
+    hipLaunchKernelGGL(k_shade, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, o, n, width, height, n_bins, bg,
+                       bgw, bgh, fb);
+    return hipGetLastError();
+}
+
+}  // namespace sirius

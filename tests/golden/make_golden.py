@@ -1,0 +1,109 @@
+"""Writes the golden fixtures in tests/golden/ from the reference's own CPU code
+(oracle/_ref/libsirius_ref.so, built by `make -C oracle ref` from the reference sources in
+place) and from the FP64 oracle (the build's "pure FP64" columns).
+
+Run in the build container (it needs the reference tree): python tests/golden/make_golden.py
+Every fixture records its seed. The reference's GeodesicTracer::trace cannot be used as-is
+(it ends a ray on its first rejected step, GTRC001A.cpp:269-277), so the trace fixtures use the
+reference's Geodesic::integrateStepRK45 inside the harness loop of oracle/ref_harness.cpp.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle_py as O  # noqa: E402
+from oracle import ref_py as RF  # noqa: E402
+from sirius_amd import abi  # noqa: E402
+
+
+def metric_points(seed=0x51121001):
+    rng = np.random.default_rng(seed)
+    ks_params, ks_x, ks_g, ks_dg = [], [], [], []
+    while len(ks_x) < 256:
+        a = rng.uniform(-0.99, 0.99)
+        Q = rng.uniform(0, 0.5) * np.sqrt(max(1 - a * a, 0))
+        L = rng.choice([0.0, rng.uniform(0, 1e-3)])
+        x = np.r_[rng.uniform(-5, 5), rng.uniform(-25, 25, 3)]
+        if np.linalg.norm(x[1:]) < 2.5:
+            continue
+        g, dg = RF.ks_evaluate(1.0, a, Q, L, x)
+        ks_params.append((1.0, a, Q, L)); ks_x.append(x); ks_g.append(g); ks_dg.append(dg)
+    mt_x, mt_g, mt_dg = [], [], []
+    for _ in range(64):
+        x = np.r_[0.0, rng.uniform(1.1, 20.0), rng.uniform(0.1, np.pi - 0.1), rng.uniform(0, 2 * np.pi)]
+        g, dg = RF.mt_evaluate(1.0, x)
+        mt_x.append(x); mt_g.append(g); mt_dg.append(dg)
+    wp_x, wp_g, wp_dg = [], [], []
+    for _ in range(64):
+        x = np.r_[0.0, rng.uniform(-3, 3, 3)]
+        g, dg = RF.warp_evaluate(0.5, 1.0, 8.0, x)
+        wp_x.append(x); wp_g.append(g); wp_dg.append(dg)
+    np.savez_compressed(os.path.join(HERE, "ref_metric.npz"), seed=seed,
+                        ks_params=np.array(ks_params), ks_x=np.array(ks_x), ks_g=np.array(ks_g),
+                        ks_dg=np.array(ks_dg), mt_b0=1.0, mt_x=np.array(mt_x), mt_g=np.array(mt_g),
+                        mt_dg=np.array(mt_dg), warp_params=np.array([0.5, 1.0, 8.0]),
+                        warp_x=np.array(wp_x), warp_g=np.array(wp_g), warp_dg=np.array(wp_dg))
+
+
+def camera_and_closed_forms():
+    cams = [(30.0, np.deg2rad(80), 0.0, 30.0, 256, 256, 0.0),
+            (50.0, np.deg2rad(75), 0.0, 20.0, 1920, 1080, 0.9)]
+    rows = []
+    for r, th, ph, fov, w, h, a in cams:
+        for px, py in [(0, 0), (w - 1, 0), (0, h - 1), (w - 1, h - 1), (w // 2, h // 2), (w // 3, 2 * h // 3),
+                       (17, 5), (w - 40, h // 7)]:
+            o, d = RF.camera_ray(r, th, ph, fov, w, h, px, py)
+            x, k = RF.init_state(1.0, a, 0.0, 0.0, r, th, ph, fov, w, h, px, py)
+            rows.append(np.r_[r, th, ph, fov, w, h, a, px, py, o, d, x, k])
+    spins = np.array([0.0, 0.1, 0.5, 0.9, 0.99, -0.5, -0.9])
+    isco = np.array([RF.isco(a) for a in spins])
+    circ_r, circ = [], []
+    for a in (0.0, 0.5, 0.9):
+        for r in (7.0, 10.0, 25.0):
+            circ_r.append((a, r)); circ.append(RF.circular(a, r))
+    ptq = np.array([[a, r, RF.pt_q(a, r)] for a in (0.0, 0.9) for r in (9.0, 15.0, 30.0)])
+    np.savez_compressed(os.path.join(HERE, "ref_camera_closed.npz"), camera_rows=np.array(rows),
+                        spins=spins, isco=isco, circular_ar=np.array(circ_r), circular_E_L_Om=np.array(circ),
+                        pt_q=ptq)
+
+
+def trace_fixture(name, M, a, r_obs, incl, fov, w, h, disk, n_bins, pix, seed):
+    R = RF.trace(M, a, 0.0, 0.0, r_obs, np.deg2rad(incl), 0.0, fov, w, h, pix, r_cap=1.01 * (1 + np.sqrt(1 - a * a)),
+                 r_esc=1000.0, disk=(disk[0] if disk[0] > 0 else RF.isco(a), disk[1]))
+    cam = abi.camera(r_obs, incl, fov, w, h, disk=disk)
+    m = abi.metric_ks(M, a)
+    opts = O.default_opts(abi.PRECISION_FP64)
+    opts.n_bins = n_bins
+    out, _ = O.trace([cam], [m], opts, ray_ids=pix, with_state=True)
+    crit = np.array([O.critical_distance(cam, m, int(i % w), int(i // w)) for i in pix])
+    np.savez_compressed(os.path.join(HERE, name), seed=seed, pix=pix, width=w, height=h, a=a, r_obs=r_obs,
+                        incl_deg=incl, fov=fov, disk=np.array(disk), n_bins=n_bins,
+                        ref_cls=R["cls"].astype(np.int8), ref_steps=R["steps"], ref_x=R["x"], ref_k=R["k"],
+                        ref_disk_r=R["disk_r"], ref_lam=R["lam"],
+                        fp64_cls=out.cls, fp64_theta=out.theta, fp64_phi=out.phi, fp64_lam=out.lam,
+                        fp64_disk_r=out.disk_r, fp64_g=out.redshift, fp64_I=out.intensity,
+                        fp64_steps=out.steps, fp64_state=out.state, crit=crit)
+
+
+def main():
+    if not RF.available():
+        sys.exit("oracle/_ref/libsirius_ref.so missing: run make -C oracle ref (needs the reference tree)")
+    metric_points()
+    camera_and_closed_forms()
+    # C1: Schwarzschild, r=30, 80 deg, FOV 30, 256^2, disk [6, 20], 8 bins: every 4th pixel per axis
+    yy, xx = np.meshgrid(np.arange(0, 256, 4), np.arange(0, 256, 4), indexing="ij")
+    trace_fixture("ref_trace_c1.npz", 1.0, 0.0, 30.0, 80.0, 30.0, 256, 256, (6.0, 20.0), 8,
+                  (yy * 256 + xx).ravel().astype(np.int64), seed=0)
+    # C2: Kerr a=0.9, r=50, 75 deg, FOV 20, 1920x1080, disk [ISCO, 30], 16 bins: 3000 random pixels
+    seed = 0x51121002
+    pix = np.sort(np.random.default_rng(seed).choice(1920 * 1080, 3000, replace=False)).astype(np.int64)
+    trace_fixture("ref_trace_c2.npz", 1.0, 0.9, 50.0, 75.0, 20.0, 1920, 1080, (0.0, 30.0), 16, pix, seed=seed)
+
+
+if __name__ == "__main__":
+    main()
