@@ -109,6 +109,8 @@ typedef struct SiriusHipTraceOpts {
     int32_t use_surrogate;   /* 1: MLP far-field surrogate with integrator fallback         */
     double x_min, x_max;     /* bin range in h nu / k T_max                                 */
     double escalate_margin;  /* FP32 path: escalate inside r < escalate_margin * r_ph(retro)*/
+    double escalate_band;    /* FP32 path: rays within this distance (M) of the critical
+                              * curve are traced in FP64 from the start (Kerr-Schild only)   */
     double surrogate_band;   /* fallback band around the critical curve (M)                 */
     double surrogate_pmin;   /* fallback when max class probability is below this           */
     double audit_fraction;   /* fraction of surrogate rays re-traced in FP64                */
@@ -204,8 +206,10 @@ int sirius_hip_trace_batch(SiriusHipHandle h, const SiriusHipCamera* cams, int n
                            const SiriusHipMetric* metrics, int n_metric,
                            const SiriusHipTraceOpts* opts, SiriusHipRayOutput* out,
                            SiriusHipStats* stats);
-/* Distance of each ray's (alpha, beta) from the critical curve (M; negative inside the
- * shadow); rays are all pixels of the cameras, output n_rays floats (device or host). */
+/* Distance (M) of each ray from the critical curve in the (alpha, beta) plane, measured
+ * radially from the curve's centre (alpha_0, 0): rho - rho_c(psi), negative inside the shadow.
+ * alpha = -L/(E sin th), beta = p_th/E at the ray's start. Rays are all pixels of the cameras,
+ * output n_rays floats (device or host). */
 int sirius_hip_critical_distance(SiriusHipHandle h, const SiriusHipCamera* cams, int n_cam,
                                  const SiriusHipMetric* metrics, int n_metric, float* out,
                                  int on_device);

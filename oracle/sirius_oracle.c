@@ -314,6 +314,7 @@ void oracle_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->x_min = 0.1;
     o->x_max = 10.0;
     o->escalate_margin = 1.0;
+    o->escalate_band = 0.5;
     o->surrogate_band = 0.5;
     o->surrogate_pmin = 0.99;
     o->audit_fraction = 0.01;
@@ -450,25 +451,32 @@ void oracle_camera_state(const SiriusHipCamera* c, const SiriusHipMetric* m, int
 /* ========================================================================================
  * Critical curve (Bardeen 1973) and a ray's distance from it
  * ======================================================================================== */
-#define CC_N 2048
-typedef struct { int n; double al[CC_N], be[CC_N]; double circle_r; } crit_curve;
+#define CC_SAMPLES 4096
+#define CC_TABLE 512
 
-/* Spherical photon orbits of Kerr-Newman: xi(r) from R = R' = 0 with
- * Delta = r^2 - 2Mr + a^2 + Q^2, eta = (r^2 + a^2 - a xi)^2 / Delta - (xi - a)^2. */
-static void build_crit_curve(const SiriusHipMetric* m, double theta_o, crit_curve* cc) {
+/* Radial table of the critical curve seen from polar angle theta_o: rho_c(psi) on psi in
+ * [0, pi] (uniform, CC_TABLE samples) about the centre (alpha0, 0), where alpha0 is the
+ * midpoint of the curve's alpha range. Spherical photon orbits of Kerr-Newman: with
+ * D = r^2 - 2Mr + a^2 + Q^2, xi(r) = ((r^2 + a^2) D' - 4 r D) / (a D'),
+ * eta = (r^2 + a^2 - a xi)^2 / D - (xi - a)^2, alpha = -xi / sin th, beta^2 = eta +
+ * a^2 cos^2 th - xi^2 cot^2 th. A non-rotating hole gives a circle of radius
+ * r_ph^2 / sqrt(D(r_ph)). */
+static void crit_table(const SiriusHipMetric* m, double theta_o, double* tab, double* alpha0) {
     double M = m->M, a = m->a, Q2 = m->Q * m->Q;
-    cc->n = 0;
-    cc->circle_r = -1;
     if (fabs(a) < 1e-6) {
         double rph = 0.5 * (3 * M + sqrt(fmax(9 * M * M - 8 * Q2, 0)));
         double D = rph * rph - 2 * M * rph + Q2;
-        cc->circle_r = rph * rph / sqrt(D);
+        for (int j = 0; j < CC_TABLE; ++j) tab[j] = rph * rph / sqrt(D);
+        *alpha0 = 0.0;
         return;
     }
+    static __thread double al[CC_SAMPLES], be[CC_SAMPLES], ps[CC_SAMPLES], rh[CC_SAMPLES];
     double rp = oracle_horizon(m);
     double st = sin(theta_o), ct = cos(theta_o);
-    for (int i = 0; i < CC_N; ++i) {
-        double r = rp + (5.0 * M - rp) * (i + 0.5) / CC_N;
+    int n = 0;
+    double amin = 1e300, amax = -1e300;
+    for (int i = 0; i < CC_SAMPLES; ++i) {
+        double r = rp + (5.0 * M - rp) * (i + 0.5) / CC_SAMPLES;
         double D = r * r - 2 * M * r + a * a + Q2, Dp = 2 * r - 2 * M;
         if (D <= 0 || fabs(Dp) < 1e-12) continue;
         double xi = ((r * r + a * a) * Dp - 4 * r * D) / (a * Dp);
@@ -477,16 +485,38 @@ static void build_crit_curve(const SiriusHipMetric* m, double theta_o, crit_curv
         if (eta < 0) continue;
         double b2 = eta + a * a * ct * ct - xi * xi * ct * ct / (st * st);
         if (b2 < 0) continue;
-        cc->al[cc->n] = -xi / st;
-        cc->be[cc->n] = sqrt(b2);
-        cc->n++;
+        al[n] = -xi / st;
+        be[n] = sqrt(b2);
+        if (al[n] < amin) amin = al[n];
+        if (al[n] > amax) amax = al[n];
+        ++n;
+    }
+    double c0 = 0.5 * (amin + amax);
+    *alpha0 = c0;
+    for (int i = 0; i < n; ++i) { ps[i] = atan2(be[i], al[i] - c0); rh[i] = hypot(al[i] - c0, be[i]); }
+    /* insertion sort by psi (the samples are already nearly ordered) */
+    for (int i = 1; i < n; ++i) {
+        double p0 = ps[i], r0 = rh[i];
+        int j = i - 1;
+        while (j >= 0 && ps[j] > p0) { ps[j + 1] = ps[j]; rh[j + 1] = rh[j]; --j; }
+        ps[j + 1] = p0; rh[j + 1] = r0;
+    }
+    int k = 0;
+    for (int j = 0; j < CC_TABLE; ++j) {
+        double psi = M_PI * j / (CC_TABLE - 1);
+        if (n == 0) { tab[j] = 0; continue; }
+        if (psi <= ps[0]) { tab[j] = rh[0]; continue; }
+        if (psi >= ps[n - 1]) { tab[j] = rh[n - 1]; continue; }
+        while (k < n - 2 && ps[k + 1] < psi) ++k;
+        double w = (psi - ps[k]) / (ps[k + 1] - ps[k]);
+        tab[j] = rh[k] + w * (rh[k + 1] - rh[k]);
     }
 }
 
 /* (alpha, beta) of a ray from its constants of motion: alpha = -L / (E sin th),
  * beta = p_th / E at the ray's start (exact for the theta potential of Kerr-Newman). */
 static void ray_alpha_beta(const SiriusHipMetric* m, const double x[4], const double p[4],
-                           double* al, double* be) {
+                           double* al, double* be, double* theta) {
     double E = -p[0];
     double L = x[1] * p[2] - x[2] * p[1];
     double r = oracle_kerr_radius(m, x[1], x[2], x[3]);
@@ -494,37 +524,27 @@ static void ray_alpha_beta(const SiriusHipMetric* m, const double x[4], const do
     double pth = (ct / st) * (x[1] * p[1] + x[2] * p[2]) - r * st * p[3];
     *al = -L / (E * st);
     *be = pth / E;
+    *theta = acos(ct);
 }
 
-static double crit_distance_ab(const crit_curve* cc, double al, double be) {
-    double b = fabs(be);
-    if (cc->circle_r > 0) return sqrt(al * al + be * be) - cc->circle_r;
-    if (cc->n == 0) return 1e30;
-    double best = 1e300;
-    int ib = 0;
-    for (int i = 0; i < cc->n; ++i) {
-        double d = (al - cc->al[i]) * (al - cc->al[i]) + (b - cc->be[i]) * (b - cc->be[i]);
-        if (d < best) { best = d; ib = i; }
-    }
-    /* inside when closer to the curve's centroid than the nearest curve point is */
-    double cx = 0;
-    for (int i = 0; i < cc->n; ++i) cx += cc->al[i];
-    cx /= cc->n;
-    double rr = (al - cx) * (al - cx) + b * b;
-    double rc = (cc->al[ib] - cx) * (cc->al[ib] - cx) + cc->be[ib] * cc->be[ib];
-    double d = sqrt(best);
-    return rr < rc ? -d : d;
+static double crit_lookup(const double* tab, double alpha0, double al, double be) {
+    double da = al - alpha0, b = fabs(be);
+    double psi = atan2(b, da), rho = hypot(da, b);
+    double u = psi / M_PI * (CC_TABLE - 1);
+    int j = (int)u;
+    if (j < 0) j = 0;
+    if (j > CC_TABLE - 2) j = CC_TABLE - 2;
+    double w = u - j;
+    return rho - (tab[j] + w * (tab[j + 1] - tab[j]));
 }
 
 double oracle_critical_distance(const SiriusHipCamera* c, const SiriusHipMetric* m, int px, int py) {
     if (m->family != SIRIUS_METRIC_KERR_SCHILD) return 1e30;
-    crit_curve cc;
-    double x[4], p[4], al, be;
+    double x[4], p[4], al, be, th, tab[CC_TABLE], a0;
     oracle_camera_state(c, m, px, py, x, p);
-    double r = oracle_kerr_radius(m, x[1], x[2], x[3]);
-    build_crit_curve(m, acos(x[3] / r), &cc);
-    ray_alpha_beta(m, x, p, &al, &be);
-    return crit_distance_ab(&cc, al, be);
+    ray_alpha_beta(m, x, p, &al, &be, &th);
+    crit_table(m, th, tab, &a0);
+    return crit_lookup(tab, a0, al, be);
 }
 
 /* ========================================================================================
@@ -668,7 +688,6 @@ typedef struct {
     const SiriusHipMetric* m;
     double r_cap, r_in, r_out, fmax, obs_norm;
     double bins[64];
-    crit_curve* cc;
 } cam_ctx;
 
 static void setup_cam(const SiriusHipCamera* c, const SiriusHipMetric* m,
@@ -833,12 +852,17 @@ static void trace_dp45(const cam_ctx* cx, const SiriusHipTraceOpts* o, const st8
         }
         if (fam == SIRIUS_METRIC_KERR_SCHILD && o->emission == SIRIUS_EMISSION_THIN_DISK &&
             s0.y[3] * s1.y[3] < 0) {
+            /* The hit is decided on the Hermite-interpolated crossing radius; the reported
+             * radius and the emission come from the landed state. */
             double th = locate_event(m, EV_Z, 0, &s0, &d0, &s1, &d1, h);
-            st8 sl; der8 dl;
-            th = land(m, EV_Z, 0, &s0, &d0, h, th, &sl, &dl, 1);
-            R->landings += 2;
-            double rd = oracle_kerr_radius(m, sl.y[1], sl.y[2], sl.y[3]);
-            if (rd >= cx->r_in && rd <= cx->r_out) {
+            double xc[4];
+            hermite(&s0, &d0, &s1, &d1, h, th, xc);
+            double rc = oracle_kerr_radius(m, xc[1], xc[2], xc[3]);
+            if (rc >= cx->r_in && rc <= cx->r_out) {
+                st8 sl; der8 dl;
+                th = land(m, EV_Z, 0, &s0, &d0, h, th, &sl, &dl, 1);
+                R->landings += 2;
+                double rd = oracle_kerr_radius(m, sl.y[1], sl.y[2], sl.y[3]);
                 R->cls = SIRIUS_RAY_DISK;
                 R->disk_r = rd;
                 disk_emission(cx, o, &sl, rd, &R->g, R->I);

@@ -53,7 +53,7 @@ class TraceOpts(C.Structure):
         ("r_escape", C.c_double), ("horizon_factor", C.c_double),
         ("n_bins", C.c_int32), ("use_surrogate", C.c_int32),
         ("x_min", C.c_double), ("x_max", C.c_double),
-        ("escalate_margin", C.c_double), ("surrogate_band", C.c_double),
+        ("escalate_margin", C.c_double), ("escalate_band", C.c_double), ("surrogate_band", C.c_double),
         ("surrogate_pmin", C.c_double), ("audit_fraction", C.c_double),
         ("audit_seed", C.c_uint64),
         ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),

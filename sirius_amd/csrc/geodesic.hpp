@@ -153,7 +153,7 @@ SIRIUS_HD __forceinline__ T event_radius(const Spacetime<T>& s, const T x[4]) {
 
 // Covariant metric values (no gradient), for initial conditions and emitter velocities.
 template <class T>
-SIRIUS_HD inline void metric_cov(const Spacetime<T>& s, const T x[4], T g[4][4]) {
+SIRIUS_HD __forceinline__ void metric_cov(const Spacetime<T>& s, const T x[4], T g[4][4]) {
     for (int i = 0; i < 4; ++i)
         for (int j = 0; j < 4; ++j) g[i][j] = (i == j) ? (i == 0 ? T(-1) : T(1)) : T(0);
     if (s.family == 1) {
@@ -268,6 +268,41 @@ SIRIUS_HD __forceinline__ T rhs(const Spacetime<T>& s, const T x[4], const T p[4
     else return rhs_ks<T>(s, x, p, k, dp);
 }
 
+// k = g^{mu nu} p_nu from metric values only (no gradient): the FP64 null-constraint
+// re-normalisation and drift report of the FP32 path use this lighter form.
+template <class T, int FAM>
+SIRIUS_HD __forceinline__ void raise_index(const Spacetime<T>& s, const T x[4], const T p[4], T k[4]) {
+    if constexpr (FAM == 1) {
+        k[0] = -p[0]; k[1] = p[1]; k[2] = p[2] / (x[1] * x[1] + s.b02); k[3] = p[3];
+    } else if constexpr (FAM == 2) {
+        T rs = sqrt(x[1] * x[1] + x[2] * x[2] + x[3] * x[3]);
+        T vf = s.wv * (tanh(s.wsig * (rs + s.wR)) - tanh(s.wsig * (rs - s.wR))) * s.winv2tanh;
+        k[0] = -p[0] - vf * p[1];
+        k[1] = -vf * p[0] + (T(1) - vf * vf) * p[1];
+        k[2] = p[2];
+        k[3] = p[3];
+    } else {
+        T r = kerr_r(s, x[1], x[2], x[3]);
+        T r2 = r * r, den = T(1) / (r2 + s.a2);
+        T l1 = (r * x[1] + s.a * x[2]) * den, l2 = (r * x[2] - s.a * x[1]) * den, l3 = x[3] / r;
+        T H = (T(2) * s.M * r - s.Q2) * r2 / (r2 * r2 + s.a2 * x[3] * x[3]);
+        T Hl = H * (-p[0] + l1 * p[1] + l2 * p[2] + l3 * p[3]);
+        k[0] = -p[0] + Hl;
+        k[1] = p[1] - Hl * l1;
+        k[2] = p[2] - Hl * l2;
+        k[3] = p[3] - Hl * l3;
+        if (s.Lambda != T(0)) {
+            T c = -s.Lambda * (x[1] * x[1] + x[2] * x[2] + x[3] * x[3]) / T(3);
+            T u0 = -T(1) - H;
+            T fac = c * k[0] / (T(1) + c * u0);
+            k[0] -= fac * u0;
+            k[1] -= fac * H * l1;
+            k[2] -= fac * H * l2;
+            k[3] -= fac * H * l3;
+        }
+    }
+}
+
 // Derivative of the state: dx (= k), dp (spatial), and p.k.
 template <class T>
 struct Der {
@@ -284,8 +319,10 @@ SIRIUS_HD __forceinline__ void eval(const Spacetime<T>& s, const T x[4], const T
 // ------------------------------------------------------------------------------------------
 // Dormand-Prince 5(4), PHGD001A.cpp:16-79. p_t is a constant of motion and is not stepped.
 // ------------------------------------------------------------------------------------------
-template <class T, int FAM>
-SIRIUS_HD inline void dp45_step(const Spacetime<T>& s, const T x0[4], const T p0[4], const Der<T>& d1,
+// TIME = false skips the coordinate time t (x[0]), which no equation depends on: the FP32
+// path integrates the 6 components (x, y, z, p_x, p_y, p_z) and measures its error on x, y, z.
+template <class T, int FAM, bool TIME = true>
+SIRIUS_HD __forceinline__ void dp45_step(const Spacetime<T>& s, const T x0[4], const T p0[4], const Der<T>& d1,
                                  T h, T x1[4], T p1[4], Der<T>& d7, T err[4], bool want_err) {
     constexpr double a21 = 1.0 / 5.0;
     constexpr double a31 = 3.0 / 40.0, a32 = 9.0 / 40.0;
@@ -300,31 +337,33 @@ SIRIUS_HD inline void dp45_step(const Spacetime<T>& s, const T x0[4], const T p0
     Der<T> d2, d3, d4, d5, d6;
     T xt[4], pt[4];
     pt[0] = p0[0];
-    xt[0] = x0[0] + h * T(a21) * d1.k[0];
+    constexpr int I0 = TIME ? 0 : 1;
+    xt[0] = x0[0];
+    x1[0] = x0[0];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) xt[i] = x0[i] + h * (T(a21) * d1.k[i]);
+    for (int i = I0; i < 4; ++i) xt[i] = x0[i] + h * (T(a21) * d1.k[i]);
 #pragma unroll
     for (int i = 0; i < 3; ++i) pt[i + 1] = p0[i + 1] + h * (T(a21) * d1.f[i]);
     eval<T, FAM>(s, xt, pt, d2);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) xt[i] = x0[i] + h * (T(a31) * d1.k[i] + T(a32) * d2.k[i]);
+    for (int i = I0; i < 4; ++i) xt[i] = x0[i] + h * (T(a31) * d1.k[i] + T(a32) * d2.k[i]);
 #pragma unroll
     for (int i = 0; i < 3; ++i) pt[i + 1] = p0[i + 1] + h * (T(a31) * d1.f[i] + T(a32) * d2.f[i]);
     eval<T, FAM>(s, xt, pt, d3);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) xt[i] = x0[i] + h * (T(a41) * d1.k[i] + T(a42) * d2.k[i] + T(a43) * d3.k[i]);
+    for (int i = I0; i < 4; ++i) xt[i] = x0[i] + h * (T(a41) * d1.k[i] + T(a42) * d2.k[i] + T(a43) * d3.k[i]);
 #pragma unroll
     for (int i = 0; i < 3; ++i) pt[i + 1] = p0[i + 1] + h * (T(a41) * d1.f[i] + T(a42) * d2.f[i] + T(a43) * d3.f[i]);
     eval<T, FAM>(s, xt, pt, d4);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = I0; i < 4; ++i)
         xt[i] = x0[i] + h * (T(a51) * d1.k[i] + T(a52) * d2.k[i] + T(a53) * d3.k[i] + T(a54) * d4.k[i]);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         pt[i + 1] = p0[i + 1] + h * (T(a51) * d1.f[i] + T(a52) * d2.f[i] + T(a53) * d3.f[i] + T(a54) * d4.f[i]);
     eval<T, FAM>(s, xt, pt, d5);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = I0; i < 4; ++i)
         xt[i] = x0[i] + h * (T(a61) * d1.k[i] + T(a62) * d2.k[i] + T(a63) * d3.k[i] + T(a64) * d4.k[i] +
                              T(a65) * d5.k[i]);
 #pragma unroll
@@ -334,7 +373,7 @@ SIRIUS_HD inline void dp45_step(const Spacetime<T>& s, const T x0[4], const T p0
     eval<T, FAM>(s, xt, pt, d6);
     p1[0] = p0[0];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = I0; i < 4; ++i)
         x1[i] = x0[i] + h * (T(b1) * d1.k[i] + T(b3) * d3.k[i] + T(b4) * d4.k[i] + T(b5) * d5.k[i] + T(b6) * d6.k[i]);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -342,8 +381,9 @@ SIRIUS_HD inline void dp45_step(const Spacetime<T>& s, const T x0[4], const T p0
                                      T(b6) * d6.f[i]);
     eval<T, FAM>(s, x1, p1, d7);
     if (want_err) {
+        err[0] = T(0);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = I0; i < 4; ++i)
             err[i] = h * (T(e1) * d1.k[i] + T(e3) * d3.k[i] + T(e4) * d4.k[i] + T(e5) * d5.k[i] + T(e6) * d6.k[i] +
                           T(e7) * d7.k[i]);
     }
@@ -383,7 +423,7 @@ SIRIUS_HD __forceinline__ T event_value(const Spacetime<T>& s, int kind, const T
 }
 
 template <class T>
-SIRIUS_HD inline T locate_event(const Spacetime<T>& s, int kind, T target, const T x0[4], const T k0[4],
+SIRIUS_HD __forceinline__ T locate_event(const Spacetime<T>& s, int kind, T target, const T x0[4], const T k0[4],
                                  const T x1[4], const T k1[4], T h) {
     constexpr int iters = sizeof(T) == 8 ? 52 : 24;
     T lo = T(0), hi = T(1), xm[4];
@@ -401,7 +441,7 @@ SIRIUS_HD inline T locate_event(const Spacetime<T>& s, int kind, T target, const
 // Camera and initial state (CMBS001A.h:120-185, GTRC001A.cpp:15-106)
 // ------------------------------------------------------------------------------------------
 template <class T>
-SIRIUS_HD inline void camera_dir(const CamConst& c, int px, int py, T nsph[3]) {
+SIRIUS_HD __forceinline__ void camera_dir(const CamConst& c, int px, int py, T nsph[3]) {
     T sx = (T(2) * (T(px) + T(0.5)) / T(c.width) - T(1)) * T(c.aspect);
     T sy = T(1) - T(2) * (T(py) + T(0.5)) / T(c.height);
     T dx = sx * T(c.tan_half), dy = sy * T(c.tan_half), dz = T(-1);
@@ -420,7 +460,7 @@ SIRIUS_HD inline void camera_dir(const CamConst& c, int px, int py, T nsph[3]) {
 // instead: inside the ergoregion g_00 > 0 and both roots can be positive, where the
 // reference's first-positive-root rule can jump to the wrong branch.
 template <class T>
-SIRIUS_HD inline T null_k0(const T g[4][4], const T k[4], T near = T(0)) {
+SIRIUS_HD __forceinline__ T null_k0(const T g[4][4], const T k[4], T near = T(0)) {
     T A = g[0][0], B = T(0), C = T(0);
     for (int i = 1; i < 4; ++i) B += T(2) * g[0][i] * k[i];
     for (int i = 1; i < 4; ++i)
@@ -438,13 +478,13 @@ SIRIUS_HD inline T null_k0(const T g[4][4], const T k[4], T near = T(0)) {
 
 // Wormhole orbital-plane frame of a ray: n_r (radial) and n_t (tangential magnitude).
 template <class T>
-SIRIUS_HD inline void mt_radial_tangential(const CamConst& c, const T ns[3], T& nr, T& nt) {
+SIRIUS_HD __forceinline__ void mt_radial_tangential(const CamConst& c, const T ns[3], T& nr, T& nt) {
     nr = ns[0];
     nt = sqrt(ns[1] * ns[1] + ns[2] * ns[2]);
 }
 
 template <class T>
-SIRIUS_HD inline void init_state(const CamConst& c, const Spacetime<T>& s, int px, int py, T x[4], T p[4]) {
+SIRIUS_HD __forceinline__ void init_state(const CamConst& c, const Spacetime<T>& s, int px, int py, T x[4], T p[4]) {
     T ns[3];
     camera_dir<T>(c, px, py, ns);
     T k[4], g[4][4];
@@ -476,7 +516,7 @@ SIRIUS_HD inline void init_state(const CamConst& c, const Spacetime<T>& s, int p
 // Emission (evaluated in FP64 on both paths; once per ray or per volume node)
 // ------------------------------------------------------------------------------------------
 // Page & Thorne (1974) closed-form f(r) with the camera's precomputed roots (M = 1).
-SIRIUS_HD inline double page_thorne_f(const CamConst& c, double r) {
+SIRIUS_HD __forceinline__ double page_thorne_f(const CamConst& c, double r) {
     double x = sqrt(r), x0 = c.pt_x0;
     if (x <= x0) return 0.0;
     double br = x - x0 - 1.5 * c.a * log(x / x0);
@@ -486,13 +526,13 @@ SIRIUS_HD inline double page_thorne_f(const CamConst& c, double r) {
 }
 
 // Kepler angular velocity of a prograde circular Kerr-Newman orbit at BL radius r.
-SIRIUS_HD inline double kepler_omega(const CamConst& c, double r) {
+SIRIUS_HD __forceinline__ double kepler_omega(const CamConst& c, double r) {
     double sq = sqrt(fmax(c.M * r - c.Q2, 0.0));
     return sq / (r * r + c.a * sq);
 }
 
 // u^t of the circular emitter at x with angular velocity Om (u = u^t (d_t + Om d_phi)).
-SIRIUS_HD inline double emitter_ut(const CamConst& c, const Spacetime<double>& s, const double x[4], double Om) {
+SIRIUS_HD __forceinline__ double emitter_ut(const CamConst& c, const Spacetime<double>& s, const double x[4], double Om) {
     double g[4][4];
     metric_cov(s, x, g);
     double v[4] = {1.0, -Om * x[2], Om * x[1], 0.0};
@@ -503,7 +543,7 @@ SIRIUS_HD inline double emitter_ut(const CamConst& c, const Spacetime<double>& s
 }
 
 // Thin disk: g = (p.u_obs)/(p.u_emit), I_k = x_k^3 / (e^{x_k/(g tau)} - 1).
-SIRIUS_HD inline void disk_emission(const CamConst& c, const TraceParams& tp, const double x[4], const double p[4],
+SIRIUS_HD __forceinline__ void disk_emission(const CamConst& c, const TraceParams& tp, const double x[4], const double p[4],
                                      double r, double& gf, float* I, int stride) {
     Spacetime<double> s = make_spacetime<double>(c);
     double Om = kepler_omega(c, r);
@@ -518,6 +558,29 @@ SIRIUS_HD inline void disk_emission(const CamConst& c, const TraceParams& tp, co
         double v = (tau > 0 && gf > 0 && ut > 0) ? xk * xk * xk / expm1(xk / (gf * tau)) : 0.0;
         I[(int64_t)k * stride] = (float)v;
     }
+}
+
+// ------------------------------------------------------------------------------------------
+// Critical-curve distance. Per camera the host builds a radial table of the curve in the
+// (alpha, beta) plane: tab[0] = alpha0 (the centre), tab[1 + j] = rho_c(pi j / (N - 1)).
+// alpha = -L/(E sin th), beta = p_th/E at the ray's start; distance = rho - rho_c(psi).
+// ------------------------------------------------------------------------------------------
+#define SIRIUS_CC_TABLE 512
+
+SIRIUS_HD __forceinline__ double crit_distance(const float* tab, const Spacetime<double>& s, const double x[4],
+                                               const double p[4]) {
+    double E = -p[0];
+    double L = x[1] * p[2] - x[2] * p[1];
+    double r = kerr_r(s, x[1], x[2], x[3]);
+    double ct = x[3] / r, st = sqrt(fmax(1.0 - ct * ct, 1e-300));
+    double pth = (ct / st) * (x[1] * p[1] + x[2] * p[2]) - r * st * p[3];
+    double da = -L / (E * st) - (double)tab[0], b = fabs(pth / E);
+    double psi = atan2(b, da), rho = sqrt(da * da + b * b);
+    double u = psi / 3.141592653589793 * (SIRIUS_CC_TABLE - 1);
+    int j = (int)u;
+    j = j < 0 ? 0 : (j > SIRIUS_CC_TABLE - 2 ? SIRIUS_CC_TABLE - 2 : j);
+    double w = u - j;
+    return rho - ((double)tab[1 + j] + w * ((double)tab[2 + j] - (double)tab[1 + j]));
 }
 
 }  // namespace sirius

@@ -15,31 +15,33 @@
 namespace sirius {
 hipError_t launch_shade(const DevOut& o, int64_t n, int width, int height, int n_bins, const uint8_t* bg, int bgw,
                         int bgh, float* fb, hipStream_t st);
-hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float* curves, const int* curve_n,
-                                    int curve_stride, int64_t n, float* out, hipStream_t st);
-int persistent_blocks_per_cu(bool fp64, int family, int block);
+hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float* tabs, int64_t n, float* out,
+                                    hipStream_t st);
+int persistent_blocks_per_cu(bool fp64, int family, int emission, bool multi, int block);
 
-// Critical curve of a Kerr-Newman camera in (alpha, |beta|): spherical photon orbits with
-// xi(r) = ((r^2 + a^2) D' - 4 r D) / (a D'), eta = (r^2 + a^2 - a xi)^2 / D - (xi - a)^2,
-// D = r^2 - 2 M r + a^2 + Q^2 (Bardeen 1973 generalised). Returns the number of points, or -1
-// with pts[0] = radius for a non-rotating hole.
-int critical_curve(const CamConst& c, float* pts, int cap) {
+// Radial table of the critical curve (same construction as the oracle's crit_table): spherical
+// photon orbits of Kerr-Newman, D = r^2 - 2Mr + a^2 + Q^2, xi(r) = ((r^2 + a^2) D' - 4 r D) /
+// (a D'), eta = (r^2 + a^2 - a xi)^2 / D - (xi - a)^2 (Bardeen 1973 with charge), seen from the
+// polar angle of the rays' start; out[0] = alpha0, out[1 + j] = rho_c(pi j / (N - 1)).
+void critical_table(const CamConst& c, float* out) {
+    const int N = SIRIUS_CC_TABLE, S = 4096;
     double M = c.M, a = c.a, Q2 = c.Q2;
     if (std::fabs(a) < 1e-6) {
         double rph = 0.5 * (3 * M + std::sqrt(std::max(9 * M * M - 8 * Q2, 0.0)));
         double D = rph * rph - 2 * M * rph + Q2;
-        pts[0] = (float)(rph * rph / std::sqrt(D));
-        return -1;
+        out[0] = 0.f;
+        for (int j = 0; j < N; ++j) out[1 + j] = (float)(rph * rph / std::sqrt(D));
+        return;
     }
     double rp = M + std::sqrt(std::max(M * M - a * a - Q2, 0.0));
-    // observer polar angle as the ray start sees it (Kerr r of the flat-mapped position)
     double x = c.r_obs * c.er[0], y = c.r_obs * c.er[1], z = c.r_obs * c.er[2];
     double R2 = x * x + y * y + z * z, w = R2 - a * a;
-    double r0 = std::sqrt(0.5 * (w + std::sqrt(w * w + 4 * a * a * z * z)));
-    double ct = z / r0, st = std::sqrt(1 - ct * ct);
-    int n = 0;
-    for (int i = 0; i < cap; ++i) {
-        double r = rp + (5.0 * M - rp) * (i + 0.5) / cap;
+    double r0 = std::sqrt(0.5 * (w + std::sqrt(std::max(w * w + 4 * a * a * z * z, 1e-20))));
+    double ct = z / r0, st = std::sqrt(std::max(1 - ct * ct, 1e-300));
+    std::vector<double> al, be;
+    double amin = 1e300, amax = -1e300;
+    for (int i = 0; i < S; ++i) {
+        double r = rp + (5.0 * M - rp) * (i + 0.5) / S;
         double D = r * r - 2 * M * r + a * a + Q2, Dp = 2 * r - 2 * M;
         if (D <= 0 || std::fabs(Dp) < 1e-12) continue;
         double xi = ((r * r + a * a) * Dp - 4 * r * D) / (a * Dp);
@@ -48,33 +50,54 @@ int critical_curve(const CamConst& c, float* pts, int cap) {
         if (eta < 0) continue;
         double b2 = eta + a * a * ct * ct - xi * xi * ct * ct / (st * st);
         if (b2 < 0) continue;
-        pts[2 * n] = (float)(-xi / st);
-        pts[2 * n + 1] = (float)std::sqrt(b2);
-        ++n;
+        al.push_back(-xi / st);
+        be.push_back(std::sqrt(b2));
+        amin = std::min(amin, al.back());
+        amax = std::max(amax, al.back());
     }
-    return n;
+    double c0 = 0.5 * (amin + amax);
+    out[0] = (float)c0;
+    std::vector<std::pair<double, double>> pr;
+    for (size_t i = 0; i < al.size(); ++i) pr.emplace_back(std::atan2(be[i], al[i] - c0), std::hypot(al[i] - c0, be[i]));
+    std::stable_sort(pr.begin(), pr.end(), [](auto& u, auto& v) { return u.first < v.first; });
+    size_t k = 0, n = pr.size();
+    for (int j = 0; j < N; ++j) {
+        double psi = M_PI * j / (N - 1), v = 0;
+        if (n == 0) v = 0;
+        else if (psi <= pr[0].first) v = pr[0].second;
+        else if (psi >= pr[n - 1].first) v = pr[n - 1].second;
+        else {
+            while (k + 2 < n && pr[k + 1].first < psi) ++k;
+            double t = (psi - pr[k].first) / (pr[k + 1].first - pr[k].first);
+            v = pr[k].second + t * (pr[k + 1].second - pr[k].second);
+        }
+        out[1 + j] = (float)v;
+    }
+}
+
+// Uploads the critical tables of a batch (n_cam x (N + 1) floats) and returns the device copy.
+float* upload_critical_tables(Handle& H, Batch& B, int* rc) {
+    int n_cam = (int)B.cc.size();
+    std::vector<float> tabs((size_t)n_cam * (SIRIUS_CC_TABLE + 1), 0.f);
+    for (int i = 0; i < n_cam; ++i)
+        if (B.cc[i].family == 0) critical_table(B.cc[i], tabs.data() + (size_t)i * (SIRIUS_CC_TABLE + 1));
+    float* d = (float*)H.scratch("crit_tabs", tabs.size() * 4, rc);
+    if (*rc) return nullptr;
+    hipError_t e = hipMemcpyAsync(d, tabs.data(), tabs.size() * 4, hipMemcpyHostToDevice, H.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(H.stream);  // the host vector dies here
+    if (e != hipSuccess) {
+        *rc = H.hip_fail(e, "upload critical tables");
+        return nullptr;
+    }
+    return d;
 }
 
 int critical_distance_dev(Handle& H, Batch& B, float* out_dev) {
-    const int stride = 2048;
-    int n_cam = (int)B.cc.size();
-    std::vector<float> curves((size_t)n_cam * stride * 2, 0.f);
-    std::vector<int> cn(n_cam, 0);
-    for (int i = 0; i < n_cam; ++i)
-        cn[i] = B.cc[i].family == 0 ? critical_curve(B.cc[i], curves.data() + (size_t)i * stride * 2, stride) : 0;
     int rc = 0;
-    float* cd = (float*)H.scratch("crit_curves", curves.size() * 4, &rc);
-    int* nd = (int*)H.scratch("crit_n", cn.size() * 4, &rc);
+    float* tabs = upload_critical_tables(H, B, &rc);
     if (rc) return rc;
-    hipError_t e;
-    if ((e = hipMemcpyAsync(cd, curves.data(), curves.size() * 4, hipMemcpyHostToDevice, H.stream)) != hipSuccess)
-        return H.hip_fail(e, "upload curves");
-    if ((e = hipMemcpyAsync(nd, cn.data(), cn.size() * 4, hipMemcpyHostToDevice, H.stream)) != hipSuccess)
-        return H.hip_fail(e, "upload curves");
-    if ((e = launch_critical_distance(B.cams_dev, n_cam, cd, nd, stride, B.n, out_dev, H.stream)) != hipSuccess)
-        return H.hip_fail(e, "critical distance");
-    // the host vectors die here: wait for the async copies
-    if ((e = hipStreamSynchronize(H.stream)) != hipSuccess) return H.hip_fail(e, "synchronize");
+    hipError_t e = launch_critical_distance(B.cams_dev, (int)B.cc.size(), tabs, B.n, out_dev, H.stream);
+    if (e != hipSuccess) return H.hip_fail(e, "critical distance");
     return SIRIUS_HIP_OK;
 }
 
@@ -209,7 +232,7 @@ int sirius_hip_autotune(SiriusHipHandle hh, const SiriusHipCamera* cams, int n_c
     const int blocks[3] = {64, 128, 256};
     const int refills[4] = {1, 8, 16, 32};
     for (int bs : blocks) {
-        int occ = persistent_blocks_per_cu(false, B.family, bs);
+        int occ = persistent_blocks_per_cu(false, B.family, B.tp.emission, B.tp.n_cam > 1, bs);
         int bpcs[3] = {occ, std::max(1, occ / 2), std::max(1, occ / 4)};
         for (int j = 0; j < 3; ++j) {
             if (j > 0 && bpcs[j] == bpcs[j - 1]) continue;

@@ -24,20 +24,25 @@ namespace sirius {
 
 // ---- kernels defined in trace_kernels.hip / util_kernels.hip ----
 hipError_t launch_fill(uint32_t* p, int n, uint32_t v, hipStream_t st);
-hipError_t launch_trace_grid(bool fp64, int family, const CamConst* cams, const TraceParams& tp, const DevOut& o,
-                             const int32_t* list, int64_t n, const uint32_t* count_dev, uint32_t flag_or, int block,
-                             hipStream_t st);
-hipError_t launch_trace_persistent(bool fp64, int family, const CamConst* cams, const TraceParams& tp,
+hipError_t launch_finish(int family, int emission, const CamConst* cams, const TraceParams& tp, const DevOut& o,
+                         hipStream_t st);
+hipError_t launch_trace_grid(bool fp64, int family, int emission, bool multi, const CamConst* cams, const TraceParams& tp,
+                             const DevOut& o, const int32_t* list, int64_t n, const uint32_t* count_dev,
+                             uint32_t flag_or, int block, hipStream_t st);
+hipError_t launch_trace_persistent(bool fp64, int family, int emission, bool multi, const CamConst* cams,
+                                   const TraceParams& tp,
                                    const DevOut& o, const int32_t* list, const uint32_t* list_count, int64_t n,
                                    uint32_t* head, uint32_t* esc_count, int32_t* esc_list,
                                    unsigned long long* attempts_out, int grid, int block, int refill_min,
                                    uint32_t flag_or, hipStream_t st);
 hipError_t launch_symplectic(bool fp64, int family, const CamConst* cams, const TraceParams& tp, const DevOut& o,
                              int block, hipStream_t st);
-int persistent_blocks_per_cu(bool fp64, int family, int block);
+int persistent_blocks_per_cu(bool fp64, int family, int emission, bool multi, int block);
 hipError_t launch_reduce_stats(const DevOut& o, int64_t n, unsigned long long* acc, hipStream_t st);
-hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float* curves, const int* curve_n,
-                                    int curve_stride, int64_t n, float* out, hipStream_t st);
+hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float* tabs, int64_t n, float* out,
+                                    hipStream_t st);
+hipError_t launch_gate(const CamConst* cams, int n_cam, const float* tabs, int64_t n, double band, int32_t* list_a,
+                       int32_t* list_b, uint32_t* counts, hipStream_t st);
 hipError_t launch_shade(const DevOut& o, int64_t n, int width, int height, int n_bins, const uint8_t* bg, int bgw,
                         int bgh, float* fb, hipStream_t st);
 
@@ -209,6 +214,12 @@ void Handle::release() {
     ev0 = ev1 = ev2 = ev3 = ev4 = nullptr;
     delete last;
     last = nullptr;
+    for (hipEvent_t* ev : {&ev_gate, &ev_band0, &ev_band1}) {
+        if (*ev) (void)hipEventDestroy(*ev);
+        *ev = nullptr;
+    }
+    if (stream2) (void)hipStreamDestroy(stream2);
+    stream2 = nullptr;
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
     ready = false;
@@ -241,6 +252,9 @@ int Handle::ensure_device() {
     device_name = prop.name;
     total_mem = prop.totalGlobalMem;
     if ((e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "stream");
+    if ((e = hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "stream");
+    for (hipEvent_t* ev : {&ev_gate, &ev_band0, &ev_band1})
+        if ((e = hipEventCreate(ev)) != hipSuccess) return hip_fail(e, "event");
     if ((e = hipEventCreate(&ev0)) != hipSuccess) return hip_fail(e, "event");
     if ((e = hipEventCreate(&ev1)) != hipSuccess) return hip_fail(e, "event");
     if ((e = hipEventCreate(&ev2)) != hipSuccess) return hip_fail(e, "event");
@@ -319,7 +333,7 @@ int prepare_batch(Handle& H, const SiriusHipCamera* cams, int n_cam, const Siriu
     TraceParams& tp = B.tp;
     std::memset(&tp, 0, sizeof(tp));
     tp.rtol = o.rtol; tp.atol = o.atol; tp.h_init = o.h_init; tp.h_min = o.h_min; tp.h_max = o.h_max;
-    tp.far_slope = o.precision == SIRIUS_PRECISION_FP32_ESCALATE ? o.far_slope : 0.0;
+    tp.far_slope = o.far_slope;
     tp.r_escape = o.r_escape;
     tp.x_min = o.x_min; tp.x_max = o.x_max;
     tp.escalate_margin = o.escalate_margin;
@@ -363,8 +377,8 @@ int bind_outputs(Handle& H, const SiriusHipRayOutput* out, int64_t n, int n_bins
     o.rejected = (uint32_t*)H.scratch("o_rej", 4 * n, &rc);
     o.landings = (uint32_t*)H.scratch("o_land", 4 * n, &rc);
     o.intensity = (float*)get("o_int", out ? out->intensity : nullptr, 4 * n * std::max(n_bins, 1));
-    o.state = nullptr;
-    if (out && out->state) o.state = (double*)get("o_state", out->state, 64 * n);
+    // the final state is always kept: the finish kernel derives every output from it
+    o.state = (double*)get("o_state", out ? out->state : nullptr, 64 * n);
     return rc;
 }
 
@@ -407,10 +421,13 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
     int64_t n_work = list_dev ? list_cap : B.n;
     bool persistent = o.schedule == SIRIUS_SCHEDULE_PERSISTENT && o.integrator == SIRIUS_INTEGRATOR_DP45;
     // work[0] bulk queue head, work[1] escalation count, work[2..3] bulk attempts (u64),
-    // work[4] escalation queue head, work[5] unused, work[6..7] escalation attempts (u64)
+    // work[4] escalation queue head, work[5] unused, work[6..7] escalation attempts (u64),
+    // work[8..9] gate counts (A, B), work[12] band queue head, work[13] unused,
+    // work[14..15] band attempts (u64)
     uint32_t* work = (uint32_t*)H.scratch("work", 64, &rc);
     if (rc) return rc;
     if ((e = launch_fill(work, 16, 0u, H.stream)) != hipSuccess) return H.hip_fail(e, "fill");
+    H.last_band = 0;
     if (time_it) (void)hipEventRecord(H.ev0, H.stream);
     if (o.integrator == SIRIUS_INTEGRATOR_SYMPLECTIC) {
         if (list_dev) return H.fail(SIRIUS_HIP_E_UNSUPPORTED, "symplectic integrator with a ray list");
@@ -420,47 +437,96 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
         // the grid schedule has no escalation queue: FP32 lanes run to the end in FP32
         TraceParams tp = B.tp;
         tp.escalate = 0;
-        if ((e = launch_trace_grid(fp64, B.family, B.cams_dev, tp, dout, list_dev, n_work, list_count_dev, 0,
+        if ((e = launch_trace_grid(fp64, B.family, B.tp.emission, B.tp.n_cam > 1, B.cams_dev, tp, dout, list_dev, n_work, list_count_dev, 0,
                                    block, H.stream)) != hipSuccess)
             return H.hip_fail(e, "grid launch");
-    } else {
-        int32_t* esc = (int32_t*)H.scratch("esc_list", 4 * (size_t)std::max<int64_t>(n_work, 1), &rc);
-        if (rc) return rc;
-        int bpc = o.blocks_per_cu > 0 ? o.blocks_per_cu : persistent_blocks_per_cu(fp64, B.family, block);
-        int grid = std::max(1, bpc * H.cu_count);
-        int refill = o.refill_min > 0 ? std::min(o.refill_min, 64) : 16;
+    } else if (fp64) {
         TraceParams tp = B.tp;
-        if (fp64) tp.escalate = 0;
-        if ((e = launch_trace_persistent(fp64, B.family, B.cams_dev, tp, dout, list_dev, list_count_dev, n_work, work,
-                                         work + 1, esc, (unsigned long long*)(work + 2), grid, block, refill, 0,
-                                         H.stream)) != hipSuccess)
+        tp.escalate = 0;
+        int bpc = o.blocks_per_cu > 0 ? o.blocks_per_cu
+                                      : persistent_blocks_per_cu(true, B.family, B.tp.emission, B.tp.n_cam > 1, block);
+        int refill = o.refill_min > 0 ? std::min(o.refill_min, 64) : 16;
+        if ((e = launch_trace_persistent(true, B.family, B.tp.emission, B.tp.n_cam > 1, B.cams_dev, tp, dout, list_dev,
+                                         list_count_dev, n_work, work, work + 1, nullptr,
+                                         (unsigned long long*)(work + 2), std::max(1, bpc * H.cu_count), block, refill,
+                                         0, H.stream)) != hipSuccess)
             return H.hip_fail(e, "persistent launch");
         if (time_it) (void)hipEventRecord(H.ev1, H.stream);
-        if (tp.escalate) {
-            // Escalated rays are re-traced from the camera by the FP64 persistent kernel at the
-            // FP64 contract's tolerances, so they carry exactly the plain FP64 answer.
-            SiriusHipTraceOpts d64;
-            sirius_hip_default_opts(&d64, SIRIUS_PRECISION_FP64);
-            TraceParams t64 = B.tp;
-            t64.rtol = d64.rtol;
-            t64.atol = d64.atol;
-            t64.h_init = d64.h_init;
-            t64.h_min = d64.h_min;
-            t64.h_max = d64.h_max;
-            t64.far_slope = 0.0;
-            t64.escalate = 0;
-            int bpc64 = persistent_blocks_per_cu(true, B.family, 256);
-            if ((e = launch_trace_persistent(true, B.family, B.cams_dev, t64, dout, esc, work + 1, n_work, work + 4,
-                                             work + 5, esc, (unsigned long long*)(work + 6),
-                                             std::max(1, bpc64 * H.cu_count), 256, 16, SIRIUS_FLAG_ESCALATED,
-                                             H.stream)) != hipSuccess)
+    } else {
+        // FP32 path. (1) Gate: rays within escalate_band of the critical curve go to list B and
+        // are traced in FP64 on the second stream while (2) the FP32 persistent tracer runs list
+        // A; (3) FP32 rays that still reach a turning point inside the photon region are re-traced
+        // in FP64 from their camera pixel after the bulk. Escalated rays keep the path's step
+        // policy in FP64 arithmetic.
+        int32_t* esc = (int32_t*)H.scratch("esc_list", 4 * (size_t)std::max<int64_t>(n_work, 1), &rc);
+        if (rc) return rc;
+        const int32_t* list_a = list_dev;
+        const uint32_t* count_a = list_count_dev;
+        int64_t cap_a = n_work;
+        uint32_t n_band = 0;
+        int32_t* list_b = nullptr;
+        bool gate = !list_dev && B.family == SIRIUS_METRIC_KERR_SCHILD && B.tp.escalate && o.escalate_band > 0;
+        if (gate) {
+            float* tabs = upload_critical_tables(H, B, &rc);
+            int32_t* la = (int32_t*)H.scratch("gate_a", 4 * (size_t)B.n, &rc);
+            list_b = (int32_t*)H.scratch("gate_b", 4 * (size_t)B.n, &rc);
+            if (rc) return rc;
+            uint32_t* counts = work + 8;
+            if ((e = launch_gate(B.cams_dev, (int)B.cc.size(), tabs, B.n, o.escalate_band, la, list_b, counts,
+                                 H.stream)) != hipSuccess)
+                return H.hip_fail(e, "gate launch");
+            uint32_t hc[2] = {0, 0};
+            if ((e = hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, H.stream)) != hipSuccess ||
+                (e = hipStreamSynchronize(H.stream)) != hipSuccess)
+                return H.hip_fail(e, "gate counts");
+            list_a = la;
+            count_a = counts;
+            cap_a = B.n;
+            n_band = hc[1];
+        }
+        TraceParams t64 = B.tp;
+        t64.escalate = 0;
+        if (n_band > 0) {
+            // FP64 band on the second stream, small blocks so the long rays spread over the chip
+            (void)hipEventRecord(H.ev_gate, H.stream);
+            (void)hipStreamWaitEvent(H.stream2, H.ev_gate, 0);
+            int bpc64 = persistent_blocks_per_cu(true, B.family, B.tp.emission, B.tp.n_cam > 1, 64);
+            int grid64 = (int)std::min<int64_t>((int64_t)bpc64 * H.cu_count, ((int64_t)n_band + 63) / 64);
+            (void)hipEventRecord(H.ev_band0, H.stream2);
+            if ((e = launch_trace_persistent(true, B.family, B.tp.emission, B.tp.n_cam > 1, B.cams_dev, t64, dout, list_b,
+                                             work + 9, B.n, work + 12, work + 13, nullptr,
+                                             (unsigned long long*)(work + 14), std::max(1, grid64), 64, 1,
+                                             SIRIUS_FLAG_ESCALATED, H.stream2)) != hipSuccess)
+                return H.hip_fail(e, "band launch");
+            (void)hipEventRecord(H.ev_band1, H.stream2);
+        }
+        int bpc = o.blocks_per_cu > 0 ? o.blocks_per_cu
+                                      : persistent_blocks_per_cu(false, B.family, B.tp.emission, B.tp.n_cam > 1, block);
+        int grid = std::max(1, bpc * H.cu_count);
+        int refill = o.refill_min > 0 ? std::min(o.refill_min, 64) : 16;
+        if (time_it) (void)hipEventRecord(H.ev0, H.stream);
+        if ((e = launch_trace_persistent(false, B.family, B.tp.emission, B.tp.n_cam > 1, B.cams_dev, B.tp, dout, list_a,
+                                         count_a, cap_a, work, work + 1, esc, (unsigned long long*)(work + 2), grid,
+                                         block, refill, 0, H.stream)) != hipSuccess)
+            return H.hip_fail(e, "persistent launch");
+        if (time_it) (void)hipEventRecord(H.ev1, H.stream);
+        if (B.tp.escalate) {
+            int bpc64 = persistent_blocks_per_cu(true, B.family, B.tp.emission, B.tp.n_cam > 1, 64);
+            if ((e = launch_trace_persistent(true, B.family, B.tp.emission, B.tp.n_cam > 1, B.cams_dev, t64, dout, esc,
+                                             work + 1, cap_a, work + 4, work + 5, nullptr,
+                                             (unsigned long long*)(work + 6), std::max(1, bpc64 * H.cu_count), 64, 1,
+                                             SIRIUS_FLAG_ESCALATED, H.stream)) != hipSuccess)
                 return H.hip_fail(e, "escalation launch");
         }
+        if (n_band > 0) (void)hipStreamWaitEvent(H.stream, H.ev_band1, 0);
+        H.last_band = n_band;
     }
     if (time_it) {
         if (!persistent) (void)hipEventRecord(H.ev1, H.stream);
         (void)hipEventRecord(H.ev4, H.stream);
     }
+    if ((e = launch_finish(B.family, B.tp.emission, B.cams_dev, B.tp, dout, H.stream)) != hipSuccess)
+        return H.hip_fail(e, "finish launch");
     if (stats) {
         unsigned long long* acc = (unsigned long long*)H.scratch("stat_acc", 8 * 8, &rc);
         if (rc) return rc;
@@ -469,7 +535,7 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
         unsigned long long h[8] = {0};
         if ((e = hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, H.stream)) != hipSuccess)
             return H.hip_fail(e, "stats copy");
-        uint32_t wk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        uint32_t wk[16] = {0};
         if (persistent) {
             if ((e = hipMemcpyAsync(wk, H.bufs["work"].ptr, sizeof(wk), hipMemcpyDeviceToHost, H.stream)) != hipSuccess)
                 return H.hip_fail(e, "stats copy");
@@ -487,20 +553,22 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
         stats->rays_audited += h[6];
         unsigned long long bulk = h[0];
         if (persistent) {
-            unsigned long long esc_att = 0;
+            unsigned long long esc_att = 0, band_att = 0;
             std::memcpy(&bulk, &wk[2], sizeof(bulk));
             std::memcpy(&esc_att, &wk[6], sizeof(esc_att));
-            stats->steps_escalated += esc_att;
-            stats->flops_escalated += (double)esc_att * flops_per_step(B.family);
+            std::memcpy(&band_att, &wk[14], sizeof(band_att));
+            stats->steps_escalated += esc_att + band_att;
+            stats->flops_escalated += (double)(esc_att + band_att) * flops_per_step(B.family);
         }
         stats->steps_bulk += bulk;
         stats->flops += (double)bulk * flops_per_step(B.family);
         if (time_it) {
-            float ms = 0, ms2 = 0;
+            float ms = 0, ms2 = 0, ms3 = 0;
             (void)hipEventElapsedTime(&ms, H.ev0, H.ev1);
             (void)hipEventElapsedTime(&ms2, H.ev1, H.ev4);
+            if (H.last_band > 0) (void)hipEventElapsedTime(&ms3, H.ev_band0, H.ev_band1);
             stats->kernel_ms += ms;
-            stats->escalate_ms += ms2;
+            stats->escalate_ms += std::max(ms2, ms3);
         }
     }
     return SIRIUS_HIP_OK;
@@ -630,6 +698,7 @@ void sirius_hip_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->x_min = 0.1;
     o->x_max = 10.0;
     o->escalate_margin = 1.0;
+    o->escalate_band = 0.5;
     o->surrogate_band = 0.5;
     o->surrogate_pmin = 0.99;
     o->audit_fraction = 0.01;

@@ -26,7 +26,9 @@ struct Handle {
     bool ready = false;
     bool initialised = false;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // second stream: the FP64 band runs beside the FP32 bulk
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr, ev4 = nullptr;
+    hipEvent_t ev_gate = nullptr, ev_band0 = nullptr, ev_band1 = nullptr;
     int cu_count = 0;
     int max_threads = 0;
     size_t total_mem = 0;
@@ -42,6 +44,7 @@ struct Handle {
     // results of the last trace_batch held in device memory (sirius_hip_fetch_results)
     DevOut* last = nullptr;
     int last_bins = 0;
+    uint32_t last_band = 0;  // rays the last FP32 trace sent to the FP64 band
     // surrogate network
     MlpState* mlp = nullptr;
 
@@ -72,6 +75,8 @@ int copy_outputs_to_host(Handle& H, const SiriusHipRayOutput* out, const DevOut&
 int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, const int32_t* list_dev,
               const uint32_t* list_count_dev, int64_t list_cap, SiriusHipStats* stats, bool time_it);
 double flops_per_step(int family);
+
+float* upload_critical_tables(Handle& H, Batch& B, int* rc);
 
 // surrogate (mlp_kernels.hip / surrogate.cpp)
 void mlp_release(Handle& H);

@@ -28,28 +28,30 @@ SIRIUS_HD __forceinline__ int find_camera(const CamConst* cams, int n_cam, int64
     return lo;
 }
 
-// Null drift |g_{mu nu} k^mu k^nu| of the final state, evaluated in FP64.
+// Null drift |g_{mu nu} k^mu k^nu| = |p.k| of the final state, evaluated in FP64.
 template <int FAM>
-SIRIUS_HD inline double null_drift64(const CamConst& c, const double x[4], const double p[4]) {
+SIRIUS_HD __forceinline__ double null_drift64(const CamConst& c, const double x[4], const double p[4]) {
     Spacetime<double> s = make_spacetime<double>(c);
-    Der<double> d;
-    eval<double, FAM>(s, x, p, d);
-    return fabs(d.pk);
+    double k[4];
+    raise_index<double, FAM>(s, x, p, k);
+    return fabs(p[0] * k[0] + p[1] * k[1] + p[2] * k[2] + p[3] * k[3]);
 }
 
 // FP64 re-imposition of the null constraint on an FP32 state, keeping E = -p_t and the
-// direction of the spatial momentum: p_i -> sigma p_i with H(sigma) = 0.
+// direction of the spatial momentum: p_i -> sigma p_i with H(sigma) = 0, i.e.
+// A sigma^2 + B sigma + C = 0, A = q.G q, B = 2 p_t (G q)^t, C = p_t^2 G^{tt}.
 template <int FAM>
-SIRIUS_HD inline void renorm_energy64(const Spacetime<double>& s, const float xf[4], float pf[4]) {
+SIRIUS_HD __forceinline__ void renorm_energy64(const Spacetime<double>& s, const float xf[4], float pf[4]) {
     double x[4] = {xf[0], xf[1], xf[2], xf[3]};
     double q[4] = {0.0, pf[1], pf[2], pf[3]};
-    double e[4] = {pf[0], 0.0, 0.0, 0.0};
-    Der<double> dq, de;
-    eval<double, FAM>(s, x, q, dq);
-    eval<double, FAM>(s, x, e, de);
-    double A = q[1] * dq.k[1] + q[2] * dq.k[2] + q[3] * dq.k[3];
-    double B = 2.0 * e[0] * dq.k[0];
-    double C = e[0] * de.k[0];
+    double e[4] = {1.0, 0.0, 0.0, 0.0};
+    double kq[4], ke[4];
+    raise_index<double, FAM>(s, x, q, kq);
+    raise_index<double, FAM>(s, x, e, ke);
+    double pt = pf[0];
+    double A = q[1] * kq[1] + q[2] * kq[2] + q[3] * kq[3];
+    double B = 2.0 * pt * kq[0];
+    double C = pt * pt * ke[0];
     double disc = B * B - 4.0 * A * C;
     if (A <= 0.0 || disc < 0.0) return;
     double sq = sqrt(disc);
@@ -63,6 +65,9 @@ SIRIUS_HD inline void renorm_energy64(const Spacetime<double>& s, const float xf
 
 // ------------------------------------------------------------------------------------------
 // Per-lane ray state and one step attempt.
+// EMIT is the emission model (SIRIUS_EMISSION_*), a template parameter so each kernel only
+// carries the code (and registers) of its own model. TIME (FP64 only) integrates the
+// coordinate time, which enters the reference's error norm (PHGD001A.cpp:508-517).
 // ------------------------------------------------------------------------------------------
 template <class T, int FAM>
 struct Ray {
@@ -71,97 +76,110 @@ struct Ray {
     T h, lam;
     T tau;          // volumetric optical depth
     T vr_prev;      // radial velocity sign tracking (FP32 escalation)
+    T rdisk;        // disk-hit radius of a finished ray
     int64_t id;
     int cam;
-    int px, py;
+    int cls;        // termination class of a finished ray
     uint32_t attempts, rejected, landings, flags;
 };
 
-template <class T, int FAM>
-SIRIUS_HD inline void ray_start(Ray<T, FAM>& R, const CamConst* cams, int cam, int64_t id, const TraceParams& tp,
-                                 const DevOut& o) {
+template <class T, int FAM, int EMIT>
+SIRIUS_HD __forceinline__ void ray_start(Ray<T, FAM>& R, const CamConst* cams, int cam, int64_t id, const TraceParams& tp,
+                                         const DevOut& o) {
     const CamConst& c = cams[cam];
     int64_t pix = id - c.offset;
     R.id = id;
     R.cam = cam;
-    R.px = (int)(pix % c.width);
-    R.py = (int)(pix / c.width);
     Spacetime<T> s = make_spacetime<T>(c);
-    init_state<T>(c, s, R.px, R.py, R.x, R.p);
+    init_state<T>(c, s, (int)(pix % c.width), (int)(pix / c.width), R.x, R.p);
     eval<T, FAM>(s, R.x, R.p, R.d);
     R.h = T(tp.h_init);
     R.lam = T(0);
     R.tau = T(0);
     R.vr_prev = T(-1);
     R.attempts = R.rejected = R.landings = R.flags = 0;
-    if (tp.emission == SIRIUS_EMISSION_VOLUMETRIC)
+    if constexpr (EMIT == SIRIUS_EMISSION_VOLUMETRIC)
         for (int k = 0; k < tp.n_bins; ++k) o.intensity[(int64_t)k * o.n_rays + id] = 0.f;
 }
 
-template <class T, int FAM>
-SIRIUS_HD inline void ray_finish(Ray<T, FAM>& R, const CamConst* cams, const TraceParams& tp, const DevOut& o,
-                                  int cls, T disk_r) {
-    const CamConst& c = cams[R.cam];
+// A finished ray leaves its class, counters and final state (x^mu, p_mu in FP64) in the
+// result arrays; k_finish turns them into directions, emission and null drift afterwards, so
+// the tracing kernels carry none of that FP64 code.
+template <class T, int FAM, int EMIT>
+SIRIUS_HD __forceinline__ void ray_store(const Ray<T, FAM>& R, const DevOut& o) {
     int64_t id = R.id;
-    double xd[4], pd[4];
-    for (int i = 0; i < 4; ++i) { xd[i] = (double)R.x[i]; pd[i] = (double)R.p[i]; }
-    float th = 0.f, ph = 0.f, g = 0.f;
+    o.cls[id] = (uint8_t)R.cls;
+    o.flags[id] = (uint8_t)R.flags;
+    o.lam[id] = (float)R.lam;
+    o.disk_r[id] = (float)R.rdisk;
+    o.redshift[id] = EMIT == SIRIUS_EMISSION_VOLUMETRIC ? (float)R.tau : 0.f;
+    o.steps[id] = R.attempts;
+    o.rejected[id] = R.rejected;
+    o.landings[id] = R.landings;
+    for (int i = 0; i < 4; ++i) { o.state[8 * id + i] = (double)R.x[i]; o.state[8 * id + 4 + i] = (double)R.p[i]; }
+}
+
+// Outputs of one finished ray from its stored class and state (all FP64).
+template <int FAM, int EMIT>
+SIRIUS_HD __forceinline__ void ray_outputs(const CamConst& c, const TraceParams& tp, const DevOut& o, int64_t id) {
+    const double* st = o.state + 8 * id;
+    double xd[4] = {st[0], st[1], st[2], st[3]}, pd[4] = {st[4], st[5], st[6], st[7]};
+    Spacetime<double> sd = make_spacetime<double>(c);
+    double kd[4];
+    raise_index<double, FAM>(sd, xd, pd, kd);
+    int cls = o.cls[id];
+    float th = 0.f, ph = 0.f;
     if (cls == SIRIUS_RAY_ESCAPED) {
         double dir[3];
-        if (FAM == 1) {
+        if constexpr (FAM == 1) {
             // wormhole: map the orbital-plane end state back to the exit side's sky
-            T ns[3];
-            camera_dir<T>(c, R.px, R.py, ns);
-            double nt = sqrt((double)ns[1] * ns[1] + (double)ns[2] * ns[2]);
+            int64_t pix = id - c.offset;
+            double ns[3];
+            camera_dir<double>(c, (int)(pix % c.width), (int)(pix / c.width), ns);
+            double nt = sqrt(ns[1] * ns[1] + ns[2] * ns[2]);
             double e2[3];
-            for (int i = 0; i < 3; ++i)
-                e2[i] = nt > 1e-300 ? ((double)ns[1] * c.et[i] + (double)ns[2] * c.ep[i]) / nt : c.et[i];
+            for (int i = 0; i < 3; ++i) e2[i] = nt > 1e-300 ? (ns[1] * c.et[i] + ns[2] * c.ep[i]) / nt : c.et[i];
             double l = xd[1], psi = xd[2], r = sqrt(l * l + c.b0 * c.b0);
-            double drdl = (l / r) * (double)R.d.k[1], rdpsi = pd[2] / r;
+            double drdl = (l / r) * kd[1], rdpsi = pd[2] / r;
             double cp = cos(psi), sp = sin(psi);
             for (int i = 0; i < 3; ++i) {
                 double e = cp * c.er[i] + sp * e2[i], t = -sp * c.er[i] + cp * e2[i];
                 dir[i] = drdl * e + rdpsi * t;
             }
-            if (l < 0) R.flags |= SIRIUS_FLAG_FAR_SIDE;
+            if (l < 0) o.flags[id] |= SIRIUS_FLAG_FAR_SIDE;
         } else {
-            dir[0] = (double)R.d.k[1];
-            dir[1] = (double)R.d.k[2];
-            dir[2] = (double)R.d.k[3];
+            dir[0] = kd[1];
+            dir[1] = kd[2];
+            dir[2] = kd[3];
         }
         double n = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
         th = (float)acos(fmin(1.0, fmax(-1.0, dir[2] / n)));
         ph = (float)atan2(dir[1], dir[0]);
     }
-    if (cls == SIRIUS_RAY_DISK && tp.emission == SIRIUS_EMISSION_THIN_DISK) {
-        double gf;
-        disk_emission(c, tp, xd, pd, (double)disk_r, gf, o.intensity + id, (int)o.n_rays);
-        g = (float)gf;
-    } else if (tp.emission == SIRIUS_EMISSION_VOLUMETRIC) {
-        g = (float)R.tau;
-    } else {
+    if constexpr (EMIT == SIRIUS_EMISSION_THIN_DISK) {
+        if (cls == SIRIUS_RAY_DISK) {
+            double gf;
+            double rd = kerr_r(sd, xd[1], xd[2], xd[3]);  // FP64 radius of the landed state
+            disk_emission(c, tp, xd, pd, rd, gf, o.intensity + id, (int)o.n_rays);
+            o.redshift[id] = (float)gf;
+        } else {
+            for (int k = 0; k < tp.n_bins; ++k) o.intensity[(int64_t)k * o.n_rays + id] = 0.f;
+        }
+    } else if constexpr (EMIT != SIRIUS_EMISSION_VOLUMETRIC) {
         for (int k = 0; k < tp.n_bins; ++k) o.intensity[(int64_t)k * o.n_rays + id] = 0.f;
     }
-    o.cls[id] = (uint8_t)cls;
-    o.flags[id] = (uint8_t)R.flags;
     o.theta[id] = th;
     o.phi[id] = ph;
-    o.lam[id] = (float)R.lam;
-    o.disk_r[id] = (float)disk_r;
-    o.redshift[id] = g;
-    o.null_drift[id] = (float)null_drift64<FAM>(c, xd, pd);
-    o.steps[id] = R.attempts;
-    o.rejected[id] = R.rejected;
-    o.landings[id] = R.landings;
-    if (o.state)
-        for (int i = 0; i < 4; ++i) { o.state[8 * id + i] = xd[i]; o.state[8 * id + 4 + i] = pd[i]; }
+    o.null_drift[id] = (float)fabs(pd[0] * kd[0] + pd[1] * kd[1] + pd[2] * kd[2] + pd[3] * kd[3]);
 }
 
-// Volumetric transfer over an accepted step (3-node Gauss-Legendre on the Hermite interpolant
-// of x, linear p). Emitter-frame j_nu = rho S(x_e), alpha = kappa rho, invariant form.
+// Volumetric transfer over an accepted step (3-node Gauss-Legendre on the cubic Hermite
+// interpolant of (x, p)). Emitter-frame j_nu = rho S(x_e), alpha = kappa rho, invariant form:
+//   dtau = kappa rho nu_e dlam,  dI_k = rho S(x_k / g) (nu_o^3 / nu_e^2) e^{-tau} dlam.
 template <class T, int FAM>
-SIRIUS_HD inline void vol_accumulate(Ray<T, FAM>& R, const CamConst& c, const Spacetime<T>& s, const TraceParams& tp,
-                                      const DevOut& o, const T x1[4], const T p1[4], const Der<T>& d1) {
+SIRIUS_HD __forceinline__ void vol_accumulate(Ray<T, FAM>& R, const CamConst& c, const Spacetime<T>& s,
+                                              const TraceParams& tp, const DevOut& o, const T x1[4], const T p1[4],
+                                              const Der<T>& d1) {
     const T gn[3] = {T(0.1127016653792583), T(0.5), T(0.8872983346207417)};
     const T gw[3] = {T(5.0 / 18.0), T(8.0 / 18.0), T(5.0 / 18.0)};
     Spacetime<double> sd = make_spacetime<double>(c);
@@ -172,10 +190,15 @@ SIRIUS_HD inline void vol_accumulate(Ray<T, FAM>& R, const CamConst& c, const Sp
         if (r < T(tp.vol_r_min) || r > T(tp.vol_r_max)) continue;
         T sig = T(tp.vol_h_over_r) * r;
         T rho = T(tp.vol_j0) / (r * r) * exp(-xq[3] * xq[3] / (T(2) * sig * sig));
-        // emitter velocity and frequencies in FP64
+        // emitter velocity and frequencies in FP64; p by cubic Hermite with dp/dlam
+        double th = (double)gn[q], t2 = th * th, t3 = t2 * th, hh = (double)R.h;
+        double h00 = 2 * t3 - 3 * t2 + 1, h10 = t3 - 2 * t2 + th, h01 = 3 * t2 - 2 * t3, h11 = t3 - t2;
         double xd[4] = {(double)xq[0], (double)xq[1], (double)xq[2], (double)xq[3]};
         double pd[4];
-        for (int i = 0; i < 4; ++i) pd[i] = (double)R.p[i] + (double)gn[q] * ((double)p1[i] - (double)R.p[i]);
+        pd[0] = (double)R.p[0];
+        for (int i = 1; i < 4; ++i)
+            pd[i] = h00 * (double)R.p[i] + h10 * hh * (double)R.d.f[i - 1] + h01 * (double)p1[i] +
+                    h11 * hh * (double)d1.f[i - 1];
         double rc = sqrt(fmax(xd[1] * xd[1] + xd[2] * xd[2] - c.a * c.a, 1e-12));
         double Om = kepler_omega(c, rc);
         double ut = emitter_ut(c, sd, xd, Om);
@@ -184,7 +207,7 @@ SIRIUS_HD inline void vol_accumulate(Ray<T, FAM>& R, const CamConst& c, const Sp
         double nuo = -pd[0] * c.obs_norm;
         if (nue <= 0 || nuo <= 0) continue;
         double gf = nuo / nue;
-        double dl = (double)gw[q] * (double)R.h;
+        double dl = (double)gw[q] * hh;
         double dtau = tp.vol_kappa * (double)rho * nue * dl;
         double w = (double)rho * (nuo * nuo * nuo / (nue * nue)) * dl * exp(-((double)R.tau + 0.5 * dtau));
         double tv = pow(6.0 / (double)r, 0.75);
@@ -207,138 +230,133 @@ SIRIUS_HD __forceinline__ T vol_step_cap(const Spacetime<T>& s, const TraceParam
     return cap > T(0.2) * sig ? cap : T(0.2) * sig;
 }
 
-// Land exactly on an event inside the step: Hermite bisection, one DP45 step of th*h, and
-// for the disk plane one Newton correction of th. Returns th; xl/pl/dl hold the landed state.
+// Land exactly on an event at fraction th of the step: one DP45 step of th*h from the step's
+// start, and for the disk plane one Newton correction of th. Landings always end the ray, so
+// the state is overwritten with the landed one. Returns the final th.
 template <class T, int FAM>
-SIRIUS_HD inline T land(Ray<T, FAM>& R, const Spacetime<T>& s, int kind, T target, const T x1[4], const Der<T>& d1,
-                         T xl[4], T pl[4], Der<T>& dl) {
-    T th = locate_event(s, kind, target, R.x, R.d.k, x1, d1.k, R.h);
-    T err[4];
-    dp45_step<T, FAM>(s, R.x, R.p, R.d, th * R.h, xl, pl, dl, err, false);
+SIRIUS_HD __forceinline__ T land(Ray<T, FAM>& R, const Spacetime<T>& s, bool plane, T th) {
+    constexpr bool TIME = sizeof(T) == 8;
+    T xl[4], pl[4], err[4];
+    Der<T> dl;
+    dp45_step<T, FAM, TIME>(s, R.x, R.p, R.d, th * R.h, xl, pl, dl, err, false);
     R.landings++;
-    if (kind == EV_Z && fabs(dl.k[3]) > T(1e-30)) {
+    if (plane && fabs(dl.k[3]) > T(1e-30)) {
         T th2 = th - xl[3] / (dl.k[3] * R.h);
         if (th2 > T(0) && th2 < T(1.5)) {
             th = th2;
-            dp45_step<T, FAM>(s, R.x, R.p, R.d, th * R.h, xl, pl, dl, err, false);
+            dp45_step<T, FAM, TIME>(s, R.x, R.p, R.d, th * R.h, xl, pl, dl, err, false);
             R.landings++;
         }
     }
+    R.lam += th * R.h;
+    for (int i = 0; i < 4; ++i) { R.x[i] = xl[i]; R.p[i] = pl[i]; }
+    R.d = dl;
     return th;
 }
 
-// One DP45 attempt with error control, events and acceptance. Returns ST_RUN, ST_DONE (ray
-// finished and written) or ST_ESCALATE (FP32 lane hands the ray to the FP64 queue).
-template <class T, int FAM>
-SIRIUS_HD inline int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, const Spacetime<T>& s, const TraceParams& tp,
-                                  const DevOut& o) {
-    const bool fp32 = sizeof(T) == 4;
-    const bool vol = (FAM == 0) && tp.emission == SIRIUS_EMISSION_VOLUMETRIC;
+// One DP45 attempt with error control, events and acceptance. Returns ST_RUN, ST_DONE (the
+// ray ended: R.cls, R.rdisk and the final state are set; the caller writes it out with
+// ray_finish) or ST_ESCALATE (an FP32 lane hands the ray to the FP64 queue). Every event is
+// located and landed through one code path, so the landing code exists once per kernel.
+template <class T, int FAM, int EMIT, bool MULTI>
+SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, const Spacetime<T>& s,
+                                          const TraceParams& tp, const DevOut& o) {
+    constexpr bool fp32 = sizeof(T) == 4;
+    constexpr bool TIME = !fp32;
+    constexpr bool vol = (FAM == 0) && EMIT == SIRIUS_EMISSION_VOLUMETRIC;
     T hmax = T(tp.h_max);
-    if (fp32 && tp.far_slope > 0) {
+    if (tp.far_slope > 0) {
         T rr = event_radius(s, R.x) * T(tp.far_slope);
         hmax = rr > hmax ? rr : hmax;
     }
-    if (vol) {
+    if constexpr (vol) {
         T cap = vol_step_cap(s, tp, R.x);
         if (R.h > cap) R.h = cap;
     }
     T x1[4], p1[4], err[4];
     Der<T> d1;
-    dp45_step<T, FAM>(s, R.x, R.p, R.d, R.h, x1, p1, d1, err, true);
+    dp45_step<T, FAM, TIME>(s, R.x, R.p, R.d, R.h, x1, p1, d1, err, true);
     R.attempts++;
     T e = T(0);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = TIME ? 0 : 1; i < 4; ++i) {
         T q = err[i] / (T(tp.atol) + T(tp.rtol) * fabs(x1[i]));
         e += q * q;
     }
-    e = sqrt(e * T(0.25));
+    e = sqrt(e * (TIME ? T(0.25) : T(1.0 / 3.0)));
     if (!(e <= T(1))) {
-        if (!sfinite(e)) { ray_finish(R, cams, tp, o, SIRIUS_RAY_FAILED, T(0)); return ST_DONE; }
         R.h = next_step(R.h, e, T(tp.h_min), hmax);
         R.rejected++;
-        if (R.h <= T(tp.h_min)) { ray_finish(R, cams, tp, o, SIRIUS_RAY_FAILED, T(0)); return ST_DONE; }
-        if (R.attempts >= (uint32_t)tp.max_steps) { ray_finish(R, cams, tp, o, SIRIUS_RAY_MAX_STEPS, T(0)); return ST_DONE; }
+        if (!sfinite(e) || R.h <= T(tp.h_min)) { R.cls = SIRIUS_RAY_FAILED; return ST_DONE; }
+        if (R.attempts >= (uint32_t)tp.max_steps) { R.cls = SIRIUS_RAY_MAX_STEPS; return ST_DONE; }
         return ST_RUN;
     }
     bool bad = false;
 #pragma unroll
     for (int i = 0; i < 4; ++i) bad |= !sfinite(x1[i]) || !sfinite(p1[i]);
-    if (bad) { ray_finish(R, cams, tp, o, SIRIUS_RAY_FAILED, T(0)); return ST_DONE; }
-    const CamConst& c = cams[R.cam];
+    if (bad) { R.cls = SIRIUS_RAY_FAILED; return ST_DONE; }
+    const CamConst& c = cams[MULTI ? R.cam : 0];
     // ---- volumetric transfer ----
-    if (vol) {
+    int ev = -1;        // pending termination class
+    int kind = EV_R;    // event function to locate (EV_Z plane or EV_R radius)
+    T target = T(0);
+    bool locate = false;
+    if constexpr (vol) {
         vol_accumulate<T, FAM>(R, c, s, tp, o, x1, p1, d1);
-        if (R.tau > T(tp.vol_tau_max)) {
-            R.lam += R.h;
-            for (int i = 0; i < 4; ++i) { R.x[i] = x1[i]; R.p[i] = p1[i]; }
-            R.d = d1;
-            ray_finish(R, cams, tp, o, SIRIUS_RAY_DISK, T(0));
-            return ST_DONE;
-        }
+        if (R.tau > T(tp.vol_tau_max)) ev = SIRIUS_RAY_DISK;
     }
-    // ---- thin disk: first crossing of the equatorial plane inside [r_in, r_out] ----
-    if (FAM == 0 && tp.emission == SIRIUS_EMISSION_THIN_DISK && R.x[3] * x1[3] < T(0)) {
-        T xl[4], pl[4];
-        Der<T> dl;
-        T th = land<T, FAM>(R, s, EV_Z, T(0), x1, d1, xl, pl, dl);
-        T rd = kerr_r(s, xl[1], xl[2], xl[3]);
-        if (rd >= T(c.r_in) && rd <= T(c.r_out)) {
-            R.lam += th * R.h;
-            for (int i = 0; i < 4; ++i) { R.x[i] = xl[i]; R.p[i] = pl[i]; }
-            R.d = dl;
-            ray_finish(R, cams, tp, o, SIRIUS_RAY_DISK, rd);
-            return ST_DONE;
+    // ---- thin disk: first crossing of the equatorial plane inside [r_in, r_out], decided on
+    // the Hermite-interpolated crossing radius, reported from the landed state ----
+    T th_disk = T(1);
+    if constexpr (FAM == 0 && EMIT == SIRIUS_EMISSION_THIN_DISK) {
+        if (R.x[3] * x1[3] < T(0)) {
+            th_disk = locate_event(s, EV_Z, T(0), R.x, R.d.k, x1, d1.k, R.h);
+            T xc[4];
+            hermite(R.x, R.d.k, x1, d1.k, R.h, th_disk, xc);
+            T rc = kerr_r(s, xc[1], xc[2], xc[3]);
+            if (rc >= T(c.r_in) && rc <= T(c.r_out)) ev = SIRIUS_RAY_DISK;
         }
     }
     T r1 = event_radius(s, x1);
-    // ---- horizon ----
-    if (FAM == 0 && r1 < T(c.r_cap)) {
-        T th = T(1);
-        if (event_radius(s, R.x) > T(c.r_cap)) {
-            T xl[4], pl[4];
-            Der<T> dl;
-            th = land<T, FAM>(R, s, EV_R, T(c.r_cap), x1, d1, xl, pl, dl);
-            for (int i = 0; i < 4; ++i) { x1[i] = xl[i]; p1[i] = pl[i]; }
-            d1 = dl;
-        }
-        R.lam += th * R.h;
-        for (int i = 0; i < 4; ++i) { R.x[i] = x1[i]; R.p[i] = p1[i]; }
-        R.d = d1;
-        ray_finish(R, cams, tp, o, SIRIUS_RAY_CAPTURED, T(0));
-        return ST_DONE;
-    }
-    // ---- escape sphere ----
-    if (r1 > T(tp.r_escape)) {
-        T radial = (FAM == 1) ? x1[1] * d1.k[1] : x1[1] * d1.k[1] + x1[2] * d1.k[2] + x1[3] * d1.k[3];
-        if (radial > T(0)) {
-            T th = T(1);
-            if (event_radius(s, R.x) < T(tp.r_escape)) {
-                T xl[4], pl[4];
-                Der<T> dl;
-                th = land<T, FAM>(R, s, EV_R, T(tp.r_escape), x1, d1, xl, pl, dl);
-                for (int i = 0; i < 4; ++i) { x1[i] = xl[i]; p1[i] = pl[i]; }
-                d1 = dl;
+    if (ev < 0) {
+        if (FAM == 0 && r1 < T(c.r_cap)) {
+            ev = SIRIUS_RAY_CAPTURED;
+            target = T(c.r_cap);
+            locate = event_radius(s, R.x) > target;
+        } else if (r1 > T(tp.r_escape)) {
+            T radial = (FAM == 1) ? x1[1] * d1.k[1] : x1[1] * d1.k[1] + x1[2] * d1.k[2] + x1[3] * d1.k[3];
+            if (radial > T(0)) {
+                ev = SIRIUS_RAY_ESCAPED;
+                target = T(tp.r_escape);
+                locate = event_radius(s, R.x) < target;
             }
-            R.lam += th * R.h;
+        }
+    }
+    if (ev >= 0) {
+        bool plane = (ev == SIRIUS_RAY_DISK) && EMIT == SIRIUS_EMISSION_THIN_DISK;
+        if (plane || locate) {
+            T th = plane ? th_disk : locate_event(s, kind, target, R.x, R.d.k, x1, d1.k, R.h);
+            land<T, FAM>(R, s, plane, th);
+        } else {
+            R.lam += R.h;
             for (int i = 0; i < 4; ++i) { R.x[i] = x1[i]; R.p[i] = p1[i]; }
             R.d = d1;
-            ray_finish(R, cams, tp, o, SIRIUS_RAY_ESCAPED, T(0));
-            return ST_DONE;
         }
+        R.cls = ev;
+        R.rdisk = plane ? kerr_r(s, R.x[1], R.x[2], R.x[3]) : T(0);
+        return ST_DONE;
     }
     // ---- accept ----
     R.lam += R.h;
     T hmax2 = T(tp.h_max);
-    if (fp32 && tp.far_slope > 0) {
+    if (tp.far_slope > 0) {
         T rr = r1 * T(tp.far_slope);
         hmax2 = rr > hmax2 ? rr : hmax2;
     }
     R.h = next_step(R.h, e, T(tp.h_min), hmax2);
     for (int i = 0; i < 4; ++i) { R.x[i] = x1[i]; R.p[i] = p1[i]; }
     R.d = d1;
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (fp32) {
         // escalation: a radial turning point inside the photon region
         if (FAM == 0 && tp.escalate) {
             T vr = x1[1] * d1.k[1] + x1[2] * d1.k[2] + x1[3] * d1.k[3];
@@ -353,7 +371,8 @@ SIRIUS_HD inline int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, const Spa
             R.flags |= SIRIUS_FLAG_RENORM;
         }
     } else if (fabs(R.d.pk) > T(1e-6)) {
-        // FP64 path: reference re-normalisation: re-solve k^0 with k^i fixed (PHGD001A.cpp:609-638)
+        // FP64 path: reference re-normalisation, k^0 re-solved with k^i fixed (PHGD001A.cpp:609-638),
+        // taking the root nearest the current k^0
         T g[4][4], k[4] = {R.d.k[0], R.d.k[1], R.d.k[2], R.d.k[3]};
         metric_cov(s, R.x, g);
         T k0 = null_k0(g, k, k[0]);
@@ -368,7 +387,7 @@ SIRIUS_HD inline int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, const Spa
             R.flags |= SIRIUS_FLAG_RENORM;
         }
     }
-    if (R.attempts >= (uint32_t)tp.max_steps) { ray_finish(R, cams, tp, o, SIRIUS_RAY_MAX_STEPS, T(0)); return ST_DONE; }
+    if (R.attempts >= (uint32_t)tp.max_steps) { R.cls = SIRIUS_RAY_MAX_STEPS; return ST_DONE; }
     return ST_RUN;
 }
 
@@ -378,7 +397,9 @@ SIRIUS_HD inline int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, const Spa
 
 // One lane per ray, run to completion. ray_list (optional) maps lane -> ray; count_dev
 // (optional) holds the list length on the device.
-template <class T, int FAM>
+// MULTI = false: one camera per launch, so the camera and spacetime constants are
+// wave-uniform (scalar registers); MULTI = true: every lane finds its own camera.
+template <class T, int FAM, int EMIT, bool MULTI>
 __global__ __launch_bounds__(256) void k_trace_grid(const CamConst* __restrict__ cams, TraceParams tp, DevOut o,
                                                     const int32_t* __restrict__ ray_list, int64_t n,
                                                     const uint32_t* __restrict__ count_dev, uint32_t flag_or) {
@@ -387,23 +408,22 @@ __global__ __launch_bounds__(256) void k_trace_grid(const CamConst* __restrict__
     if (lim > n) lim = n;
     if (i >= lim) return;
     int64_t id = ray_list ? (int64_t)ray_list[i] : i;
-    int cam = tp.n_cam == 1 ? 0 : find_camera(cams, tp.n_cam, id);
+    int cam = MULTI ? find_camera(cams, tp.n_cam, id) : 0;
     Ray<T, FAM> R;
-    ray_start<T, FAM>(R, cams, cam, id, tp, o);
+    ray_start<T, FAM, EMIT>(R, cams, cam, id, tp, o);
     R.flags |= flag_or;
     Spacetime<T> s = make_spacetime<T>(cams[cam]);
-    while (true) {
-        int st = ray_attempt<T, FAM>(R, cams, s, tp, o);
-        if (st != ST_RUN) break;
+    while (ray_attempt<T, FAM, EMIT, MULTI>(R, cams, s, tp, o) == ST_RUN) {
     }
+    ray_store<T, FAM, EMIT>(R, o);
 }
 
 // Persistent tracer. Each wave pulls batches of work-list slots from *head (one atomic per
 // refill) and refills its idle lanes when at least refill_min of them are idle; slot -> ray
 // through ray_list when given. FP32 lanes that must escalate append their ray to esc_list
-// (count in *esc_count) and add nothing else; each lane adds the attempts of every ray it
-// traced to *attempts_out when the wave retires.
-template <class T, int FAM>
+// (count in *esc_count); each lane adds the attempts of every ray it traced to
+// *attempts_out when the wave retires.
+template <class T, int FAM, int EMIT, bool MULTI>
 __global__ __launch_bounds__(256) void k_trace_persistent(const CamConst* __restrict__ cams, TraceParams tp, DevOut o,
                                                           const int32_t* __restrict__ ray_list,
                                                           const uint32_t* __restrict__ list_count, int64_t n,
@@ -416,6 +436,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(const CamConst* __rest
     if (total > n) total = n;
     Ray<T, FAM> R;
     Spacetime<T> s;
+    if (!MULTI) s = make_spacetime<T>(cams[0]);
     bool active = false;
     bool exhausted = false;
     unsigned long long my_attempts = 0;  // attempts of every ray this lane traced
@@ -434,19 +455,21 @@ __global__ __launch_bounds__(256) void k_trace_persistent(const CamConst* __rest
                 int64_t slot = (int64_t)base + __popcll(below);
                 if (slot < total) {
                     int64_t id = ray_list ? (int64_t)ray_list[slot] : slot;
-                    int cam = tp.n_cam == 1 ? 0 : find_camera(cams, tp.n_cam, id);
-                    ray_start<T, FAM>(R, cams, cam, id, tp, o);
+                    int cam = MULTI ? find_camera(cams, tp.n_cam, id) : 0;
+                    ray_start<T, FAM, EMIT>(R, cams, cam, id, tp, o);
                     R.flags |= flag_or;
-                    s = make_spacetime<T>(cams[cam]);
+                    if (MULTI) s = make_spacetime<T>(cams[cam]);
                     active = true;
                 }
             }
         }
         if (active) {
-            int st = ray_attempt<T, FAM>(R, cams, s, tp, o);
+            int st = ray_attempt<T, FAM, EMIT, MULTI>(R, cams, s, tp, o);
             if (st == ST_ESCALATE) {
                 uint32_t k = atomicAdd(esc_count, 1u);
                 esc_list[k] = (int32_t)R.id;
+            } else if (st == ST_DONE) {
+                ray_store<T, FAM, EMIT>(R, o);
             }
             if (st != ST_RUN) {
                 my_attempts += R.attempts;
@@ -456,6 +479,16 @@ __global__ __launch_bounds__(256) void k_trace_persistent(const CamConst* __rest
     }
     for (int off = 32; off > 0; off >>= 1) my_attempts += __shfl_xor(my_attempts, off);
     if (lane == 0) atomicAdd(attempts_out, my_attempts);
+}
+
+// Outputs of every traced ray (rays the surrogate answered are skipped).
+template <int FAM, int EMIT>
+__global__ __launch_bounds__(256) void k_finish(const CamConst* __restrict__ cams, TraceParams tp, DevOut o) {
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= o.n_rays) return;
+    if (o.flags[id] & SIRIUS_FLAG_SURROGATE) return;
+    int cam = tp.n_cam == 1 ? 0 : find_camera(cams, tp.n_cam, id);
+    ray_outputs<FAM, EMIT>(cams[cam], tp, o, id);
 }
 
 // Fill a small device array (graph-safe replacement for a memset).
@@ -470,21 +503,21 @@ __global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
 // (weights PHSI001A.h:43-47). Fixed step; disk crossing by linear interpolation.
 // ------------------------------------------------------------------------------------------
 template <class T, int FAM>
-SIRIUS_HD inline void tao_a(const Spacetime<T>& s, T d, T q[4], T p[4], T x[4], const T y[4]) {
+SIRIUS_HD __forceinline__ void tao_a(const Spacetime<T>& s, T d, T q[4], T p[4], T x[4], const T y[4]) {
     Der<T> D;
     eval<T, FAM>(s, q, y, D);
     for (int i = 0; i < 3; ++i) p[i + 1] += d * D.f[i];
     for (int i = 0; i < 4; ++i) x[i] += d * D.k[i];
 }
 template <class T, int FAM>
-SIRIUS_HD inline void tao_b(const Spacetime<T>& s, T d, T q[4], const T p[4], const T x[4], T y[4]) {
+SIRIUS_HD __forceinline__ void tao_b(const Spacetime<T>& s, T d, T q[4], const T p[4], const T x[4], T y[4]) {
     Der<T> D;
     eval<T, FAM>(s, x, p, D);
     for (int i = 0; i < 4; ++i) q[i] += d * D.k[i];
     for (int i = 0; i < 3; ++i) y[i + 1] += d * D.f[i];
 }
 template <class T>
-SIRIUS_HD inline void tao_c(T cw, T sw, T q[4], T p[4], T x[4], T y[4]) {
+SIRIUS_HD __forceinline__ void tao_c(T cw, T sw, T q[4], T p[4], T x[4], T y[4]) {
     for (int i = 0; i < 4; ++i) {
         T sq = q[i] + x[i], sp = p[i] + y[i], dq = q[i] - x[i], dp = p[i] - y[i];
         T ndq = cw * dq + sw * dp, ndp = -sw * dq + cw * dp;
@@ -499,7 +532,7 @@ __global__ __launch_bounds__(256) void k_symplectic_grid(const CamConst* __restr
     if (id >= tp.n_rays) return;
     int cam = tp.n_cam == 1 ? 0 : find_camera(cams, tp.n_cam, id);
     Ray<T, FAM> R;
-    ray_start<T, FAM>(R, cams, cam, id, tp, o);
+    ray_start<T, FAM, SIRIUS_EMISSION_THIN_DISK>(R, cams, cam, id, tp, o);
     const CamConst& c = cams[cam];
     Spacetime<T> s = make_spacetime<T>(c);
     T q[4], p[4], x[4], y[4];
@@ -551,7 +584,9 @@ __global__ __launch_bounds__(256) void k_symplectic_grid(const CamConst* __restr
     for (int i = 0; i < 4; ++i) { R.x[i] = q[i]; R.p[i] = p[i]; }
     eval<T, FAM>(s, R.x, R.p, R.d);
     R.attempts = n;
-    ray_finish(R, cams, tp, o, cls, rd);
+    R.cls = cls;
+    R.rdisk = rd;
+    ray_store<T, FAM, SIRIUS_EMISSION_NONE>(R, o);
 }
 
 }  // namespace sirius
@@ -561,24 +596,47 @@ __global__ __launch_bounds__(256) void k_symplectic_grid(const CamConst* __restr
 // ------------------------------------------------------------------------------------------
 namespace sirius {
 
-int persistent_blocks_per_cu(bool fp64, int family, int block) {
-    int n = 0;
-    hipError_t e;
-    if (fp64) {
-        switch (family) {
-            case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_persistent<double, 1>, block, 0); break;
-            case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_persistent<double, 2>, block, 0); break;
-            default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_persistent<double, 0>, block, 0); break;
-        }
-    } else {
-        switch (family) {
-            case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_persistent<float, 1>, block, 0); break;
-            case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_persistent<float, 2>, block, 0); break;
-            default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_persistent<float, 0>, block, 0); break;
-        }
+// Kernel variant for (precision, family, emission): emission only applies to Kerr-Schild.
+#define SIRIUS_VARIANTS_M(MACRO, T, M)                                                            \
+    switch (family) {                                                                             \
+        case 1: MACRO(T, 1, SIRIUS_EMISSION_NONE, M); break;                                      \
+        case 2: MACRO(T, 2, SIRIUS_EMISSION_NONE, M); break;                                      \
+        default:                                                                                  \
+            if (emission == SIRIUS_EMISSION_THIN_DISK) MACRO(T, 0, SIRIUS_EMISSION_THIN_DISK, M);    \
+            else if (emission == SIRIUS_EMISSION_VOLUMETRIC) MACRO(T, 0, SIRIUS_EMISSION_VOLUMETRIC, M); \
+            else MACRO(T, 0, SIRIUS_EMISSION_NONE, M);                                            \
+            break;                                                                                \
     }
+#define SIRIUS_VARIANTS(MACRO, T)                      \
+    if (multi) { SIRIUS_VARIANTS_M(MACRO, T, true) }   \
+    else { SIRIUS_VARIANTS_M(MACRO, T, false) }
+
+int persistent_blocks_per_cu(bool fp64, int family, int emission, bool multi, int block) {
+    int n = 0;
+    hipError_t e = hipSuccess;
+#define SIRIUS_OCC(T, F, E, M) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_persistent<T, F, E, M>, block, 0)
+    if (fp64) { SIRIUS_VARIANTS(SIRIUS_OCC, double) } else { SIRIUS_VARIANTS(SIRIUS_OCC, float) }
+#undef SIRIUS_OCC
     if (e != hipSuccess || n <= 0) n = 1;
     return n;
+}
+
+hipError_t launch_finish(int family, int emission, const CamConst* cams, const TraceParams& tp, const DevOut& o,
+                         hipStream_t st) {
+    dim3 g((unsigned)((o.n_rays + 255) / 256)), b(256);
+    switch (family) {
+        case 1: hipLaunchKernelGGL((k_finish<1, SIRIUS_EMISSION_NONE>), g, b, 0, st, cams, tp, o); break;
+        case 2: hipLaunchKernelGGL((k_finish<2, SIRIUS_EMISSION_NONE>), g, b, 0, st, cams, tp, o); break;
+        default:
+            if (emission == SIRIUS_EMISSION_THIN_DISK)
+                hipLaunchKernelGGL((k_finish<0, SIRIUS_EMISSION_THIN_DISK>), g, b, 0, st, cams, tp, o);
+            else if (emission == SIRIUS_EMISSION_VOLUMETRIC)
+                hipLaunchKernelGGL((k_finish<0, SIRIUS_EMISSION_VOLUMETRIC>), g, b, 0, st, cams, tp, o);
+            else
+                hipLaunchKernelGGL((k_finish<0, SIRIUS_EMISSION_NONE>), g, b, 0, st, cams, tp, o);
+            break;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_fill(uint32_t* p, int n, uint32_t v, hipStream_t st) {
@@ -586,50 +644,30 @@ hipError_t launch_fill(uint32_t* p, int n, uint32_t v, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <class T>
-static hipError_t grid_dispatch(int family, const CamConst* cams, const TraceParams& tp, const DevOut& o,
-                                const int32_t* list, int64_t n, const uint32_t* count_dev, uint32_t flag_or, int block,
-                                hipStream_t st) {
+hipError_t launch_trace_grid(bool fp64, int family, int emission, bool multi, const CamConst* cams, const TraceParams& tp,
+                             const DevOut& o, const int32_t* list, int64_t n, const uint32_t* count_dev,
+                             uint32_t flag_or, int block, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     dim3 g((unsigned)((n + block - 1) / block)), b(block);
-    switch (family) {
-        case 1: hipLaunchKernelGGL((k_trace_grid<T, 1>), g, b, 0, st, cams, tp, o, list, n, count_dev, flag_or); break;
-        case 2: hipLaunchKernelGGL((k_trace_grid<T, 2>), g, b, 0, st, cams, tp, o, list, n, count_dev, flag_or); break;
-        default: hipLaunchKernelGGL((k_trace_grid<T, 0>), g, b, 0, st, cams, tp, o, list, n, count_dev, flag_or); break;
-    }
+#define SIRIUS_GRID(T, F, E, M) \
+    hipLaunchKernelGGL((k_trace_grid<T, F, E, M>), g, b, 0, st, cams, tp, o, list, n, count_dev, flag_or)
+    if (fp64) { SIRIUS_VARIANTS(SIRIUS_GRID, double) } else { SIRIUS_VARIANTS(SIRIUS_GRID, float) }
+#undef SIRIUS_GRID
     return hipGetLastError();
 }
 
-hipError_t launch_trace_grid(bool fp64, int family, const CamConst* cams, const TraceParams& tp, const DevOut& o,
-                             const int32_t* list, int64_t n, const uint32_t* count_dev, uint32_t flag_or, int block,
-                             hipStream_t st) {
-    return fp64 ? grid_dispatch<double>(family, cams, tp, o, list, n, count_dev, flag_or, block, st)
-                : grid_dispatch<float>(family, cams, tp, o, list, n, count_dev, flag_or, block, st);
-}
-
-hipError_t launch_trace_persistent(bool fp64, int family, const CamConst* cams, const TraceParams& tp,
+hipError_t launch_trace_persistent(bool fp64, int family, int emission, bool multi, const CamConst* cams,
+                                   const TraceParams& tp,
                                    const DevOut& o, const int32_t* list, const uint32_t* list_count, int64_t n,
                                    uint32_t* head, uint32_t* esc_count, int32_t* esc_list,
                                    unsigned long long* attempts_out, int grid, int block, int refill_min,
                                    uint32_t flag_or, hipStream_t st) {
     dim3 g(grid), b(block);
-#define SIRIUS_LAUNCH_P(T, F)                                                                                   \
-    hipLaunchKernelGGL((k_trace_persistent<T, F>), g, b, 0, st, cams, tp, o, list, list_count, n, head, esc_count, \
-                       esc_list, attempts_out, refill_min, flag_or)
-    if (fp64) {
-        switch (family) {
-            case 1: SIRIUS_LAUNCH_P(double, 1); break;
-            case 2: SIRIUS_LAUNCH_P(double, 2); break;
-            default: SIRIUS_LAUNCH_P(double, 0); break;
-        }
-    } else {
-        switch (family) {
-            case 1: SIRIUS_LAUNCH_P(float, 1); break;
-            case 2: SIRIUS_LAUNCH_P(float, 2); break;
-            default: SIRIUS_LAUNCH_P(float, 0); break;
-        }
-    }
-#undef SIRIUS_LAUNCH_P
+#define SIRIUS_PERS(T, F, E, M)                                                                               \
+    hipLaunchKernelGGL((k_trace_persistent<T, F, E, M>), g, b, 0, st, cams, tp, o, list, list_count, n, head,    \
+                       esc_count, esc_list, attempts_out, refill_min, flag_or)
+    if (fp64) { SIRIUS_VARIANTS(SIRIUS_PERS, double) } else { SIRIUS_VARIANTS(SIRIUS_PERS, float) }
+#undef SIRIUS_PERS
     return hipGetLastError();
 }
 

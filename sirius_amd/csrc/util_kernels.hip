@@ -57,61 +57,83 @@ hipError_t launch_reduce_stats(const DevOut& o, int64_t n, unsigned long long* a
     return hipGetLastError();
 }
 
-// Distance in the (alpha, beta) plane from the camera's critical curve (sampled on the host,
-// curve_n[c] points of (alpha, |beta|), or a circle of radius -curve_n... see below), negative
-// inside. alpha = -L/(E sin th), beta = p_th/E at the ray's start, in FP64.
+// Critical-curve distance of every ray (radial, from the per-camera tables; 1e30 for metrics
+// without a photon sphere).
 __global__ __launch_bounds__(256) void k_critical_distance(const CamConst* __restrict__ cams, int n_cam,
-                                                           const float* __restrict__ curves,
-                                                           const int* __restrict__ curve_n, int stride, int64_t n,
+                                                           const float* __restrict__ tabs, int64_t n,
                                                            float* __restrict__ out) {
     int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= n) return;
-    int cam = 0;
-    {
-        int lo = 0, hi = n_cam - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (cams[mid].offset <= id) lo = mid; else hi = mid - 1;
-        }
-        cam = lo;
+    int lo = 0, hi = n_cam - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (cams[mid].offset <= id) lo = mid; else hi = mid - 1;
     }
-    const CamConst& c = cams[cam];
+    const CamConst& c = cams[lo];
     if (c.family != 0) { out[id] = 1e30f; return; }
     int64_t pix = id - c.offset;
     Spacetime<double> s = make_spacetime<double>(c);
     double x[4], p[4];
     init_state<double>(c, s, (int)(pix % c.width), (int)(pix / c.width), x, p);
-    double E = -p[0];
-    double L = x[1] * p[2] - x[2] * p[1];
-    double r = kerr_r(s, x[1], x[2], x[3]);
-    double ct = x[3] / r, st = sqrt(fmax(1.0 - ct * ct, 1e-300));
-    double pth = (ct / st) * (x[1] * p[1] + x[2] * p[2]) - r * st * p[3];
-    float al = (float)(-L / (E * st)), be = (float)fabs(pth / E);
-    const float* cv = curves + (int64_t)cam * stride * 2;
-    int m = curve_n[cam];
-    if (m < 0) {  // circle of radius cv[0]
-        out[id] = sqrtf(al * al + be * be) - cv[0];
-        return;
-    }
-    float best = 3.0e38f, cx = 0.f;
-    int ib = 0;
-    for (int i = 0; i < m; ++i) {
-        float da = al - cv[2 * i], db = be - cv[2 * i + 1];
-        float d = da * da + db * db;
-        if (d < best) { best = d; ib = i; }
-        cx += cv[2 * i];
-    }
-    cx /= (float)(m > 0 ? m : 1);
-    float rr = (al - cx) * (al - cx) + be * be;
-    float rc = (cv[2 * ib] - cx) * (cv[2 * ib] - cx) + cv[2 * ib + 1] * cv[2 * ib + 1];
-    float d = sqrtf(best);
-    out[id] = rr < rc ? -d : d;
+    out[id] = (float)crit_distance(tabs + (int64_t)lo * (SIRIUS_CC_TABLE + 1), s, x, p);
 }
 
-hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float* curves, const int* curve_n,
-                                    int curve_stride, int64_t n, float* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_critical_distance, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cams, n_cam, curves,
-                       curve_n, curve_stride, n, out);
+hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float* tabs, int64_t n, float* out,
+                                    hipStream_t st) {
+    hipLaunchKernelGGL(k_critical_distance, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cams, n_cam, tabs,
+                       n, out);
+    return hipGetLastError();
+}
+
+// Gate of the FP32 path: rays within `band` of the critical curve go to list B (traced in FP64
+// from the start, on a second stream), all others to list A (FP32 persistent tracer). Appends
+// are aggregated per wave: one atomic per wave and list. counts[0] = |A|, counts[1] = |B|.
+__global__ __launch_bounds__(256) void k_gate(const CamConst* __restrict__ cams, int n_cam,
+                                              const float* __restrict__ tabs, int64_t n, double band,
+                                              int32_t* __restrict__ list_a, int32_t* __restrict__ list_b,
+                                              uint32_t* __restrict__ counts) {
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = id < n;
+    bool to_b = false;
+    if (valid) {
+        int lo = 0, hi = n_cam - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (cams[mid].offset <= id) lo = mid; else hi = mid - 1;
+        }
+        const CamConst& c = cams[lo];
+        if (c.family == 0 && band > 0) {
+            int64_t pix = id - c.offset;
+            Spacetime<double> s = make_spacetime<double>(c);
+            double x[4], p[4];
+            init_state<double>(c, s, (int)(pix % c.width), (int)(pix / c.width), x, p);
+            double d = crit_distance(tabs + (int64_t)lo * (SIRIUS_CC_TABLE + 1), s, x, p);
+            to_b = fabs(d) < band;
+        }
+    }
+    const int lane = __lane_id();
+    unsigned long long below = lane ? ((1ull << lane) - 1ull) : 0ull;
+    unsigned long long ma = __ballot(valid && !to_b), mb = __ballot(valid && to_b);
+    int leader = __ffsll((long long)(ma | mb)) - 1;
+    uint32_t ba = 0, bb = 0;
+    if (leader >= 0 && lane == leader) {
+        if (ma) ba = atomicAdd(&counts[0], (uint32_t)__popcll(ma));
+        if (mb) bb = atomicAdd(&counts[1], (uint32_t)__popcll(mb));
+    }
+    if (leader >= 0) {
+        ba = __shfl(ba, leader);
+        bb = __shfl(bb, leader);
+    }
+    if (valid) {
+        if (to_b) list_b[bb + __popcll(mb & below)] = (int32_t)id;
+        else list_a[ba + __popcll(ma & below)] = (int32_t)id;
+    }
+}
+
+hipError_t launch_gate(const CamConst* cams, int n_cam, const float* tabs, int64_t n, double band, int32_t* list_a,
+                       int32_t* list_b, uint32_t* counts, hipStream_t st) {
+    hipLaunchKernelGGL(k_gate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cams, n_cam, tabs, n, band,
+                       list_a, list_b, counts);
     return hipGetLastError();
 }
 

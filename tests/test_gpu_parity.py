@@ -1,0 +1,245 @@
+"""GPU parity of the HIP tracer (through the C-ABI) against the FP64 oracle and its committed
+golden columns.
+
+Bars (the trace contract's tolerances):
+  plain FP64 kernel   classes identical; escape direction within 1e-6 rad; disk radius and
+                      intensity within 1e-6 relative (same algorithm and arithmetic width as
+                      the oracle; differences are FMA contraction and libm ulps)
+  FP32 + escalation   classes identical except rays within 0.5 M of the critical curve;
+                      escape direction within 1e-4 rad; intensity within 1e-3 relative to the
+                      pixel's brightest bin, floored at 1e-2 of the frame's median disk pixel
+                      (intensity vanishes at the ISCO, where a relative error means nothing)
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from sirius_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+
+def _angle(u, v):
+    u = u / np.linalg.norm(u, axis=1, keepdims=True)
+    v = v / np.linalg.norm(v, axis=1, keepdims=True)
+    return np.arccos(np.clip((u * v).sum(1), -1, 1))
+
+
+def _dirs(th, ph):
+    th, ph = np.asarray(th, float), np.asarray(ph, float)
+    return np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1)
+
+
+def _setup(d):
+    w, h = int(d["width"]), int(d["height"])
+    cam = abi.camera(float(d["r_obs"]), float(d["incl_deg"]), float(d["fov"]), w, h, disk=tuple(d["disk"]))
+    m = abi.metric_ks(1.0, float(d["a"]))
+    return cam, m
+
+
+def _compare(got, d, band, fp64):
+    """got: HostRays restricted to the fixture pixels; d: golden fixture."""
+    cls_ref = d["fp64_cls"]
+    if fp64:
+        assert np.array_equal(got.cls, cls_ref)
+        mask = np.ones_like(band)
+    else:
+        assert np.array_equal(got.cls[band], cls_ref[band])
+        mask = band
+    esc = mask & (got.cls == 0) & (cls_ref == 0)
+    ang = _angle(got.direction()[esc], _dirs(d["fp64_theta"][esc], d["fp64_phi"][esc])) if esc.any() else np.zeros(1)
+    dk = mask & (got.cls == 2) & (cls_ref == 2)
+    Iref = d["fp64_I"][:, dk]
+    Igot = got.intensity[:, dk]
+    scale = np.median(Iref.max(0))
+    rel = np.abs(Igot - Iref).max(0) / np.maximum(Iref.max(0), 1e-2 * scale)
+    rdisk = np.abs(got.disk_r[dk] - d["fp64_disk_r"][dk]) / d["fp64_disk_r"][dk]
+    return ang.max(), rel.max(), rdisk.max() if rdisk.size else 0.0
+
+
+@pytest.mark.parametrize("fixture", ["ref_trace_c1.npz", "ref_trace_c2.npz"])
+def test_fp64_full_frame_matches_oracle(acc, golden, fixture):
+    d = golden(fixture)
+    cam, m = _setup(d)
+    o = O.default_opts(abi.PRECISION_FP64)
+    o.n_bins = int(d["n_bins"])
+    print(f"[case] {fixture} fp64 {cam.width}x{cam.height}", flush=True)
+    out, st = acc.trace_batch([cam], [m], o)
+    got = out.take(d["pix"])
+    band = np.abs(d["crit"]) > 0.5
+    ang, rel, rd = _compare(got, d, band, fp64=True)
+    assert ang < 1e-6 and rel < 1e-6 and rd < 1e-6, (ang, rel, rd)
+    assert np.all(out.cls <= abi.RAY_DISK)
+    assert st["steps_bulk"] == out.steps.astype(np.int64).sum()
+    assert out.null_drift.max() < 2e-6
+
+
+@pytest.mark.parametrize("fixture", ["ref_trace_c1.npz", "ref_trace_c2.npz"])
+def test_fp32_escalation_meets_the_bar(acc, golden, fixture):
+    d = golden(fixture)
+    cam, m = _setup(d)
+    o = O.default_opts(abi.PRECISION_FP32_ESCALATE)
+    o.n_bins = int(d["n_bins"])
+    print(f"[case] {fixture} fp32 {cam.width}x{cam.height}", flush=True)
+    out, st = acc.trace_batch([cam], [m], o)
+    got = out.take(d["pix"])
+    band = np.abs(d["crit"]) > 0.5
+    ang, rel, rd = _compare(got, d, band, fp64=False)
+    assert ang < 1e-4 and rel < 1e-3, (ang, rel)
+    # rays gated into the FP64 band are those near the critical curve
+    e = (got.flags & abi.FLAG_ESCALATED) != 0
+    assert st["rays_escalated"] == int(((out.flags & abi.FLAG_ESCALATED) != 0).sum())
+    assert st["rays_escalated"] < 0.15 * out.n
+
+
+def test_persistent_queue_equals_grid_launch(acc):
+    """Same per-ray FP32 arithmetic under both schedules: the work queue traces every ray
+    exactly once, so all outputs are bit-identical."""
+    m = abi.metric_ks(1.0, 0.9)
+    cam = abi.camera(50.0, 75.0, 20.0, 320, 180, disk=(0.0, 30.0))
+    o = O.default_opts(abi.PRECISION_FP32_ESCALATE)
+    o.escalate_margin = 0.0
+    a, sa = acc.trace_batch([cam], [m], o)
+    o.schedule = abi.SCHEDULE_GRID
+    b, sb = acc.trace_batch([cam], [m], o)
+    for k in ("cls", "flags", "theta", "phi", "lam", "disk_r", "redshift", "steps", "intensity"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
+    assert sa["steps_bulk"] == sb["steps_bulk"]
+    # refill thresholds and launch shapes do not change results either
+    o.schedule = abi.SCHEDULE_PERSISTENT
+    for bs, bpc, rf in ((64, 1, 1), (128, 2, 32), (256, 0, 64)):
+        o.block_size, o.blocks_per_cu, o.refill_min = bs, bpc, rf
+        c, _ = acc.trace_batch([cam], [m], o)
+        assert np.array_equal(c.cls, a.cls) and np.array_equal(c.intensity, a.intensity)
+
+
+def test_escalated_rays_are_bitwise_fp64(acc):
+    """Escalated rays (the gate's critical band and in-flight turning points inside the photon
+    region) are traced by the FP64 persistent kernel with the FP32 path's step policy: they equal
+    an all-FP64 run with that policy bit for bit."""
+    m = abi.metric_ks(1.0, 0.9)
+    cam = abi.camera(50.0, 75.0, 20.0, 480, 270, disk=(0.0, 30.0))
+    o = O.default_opts(abi.PRECISION_FP32_ESCALATE)
+    got, st = acc.trace_batch([cam], [m], o)
+    o64 = O.default_opts(abi.PRECISION_FP32_ESCALATE)
+    o64.precision = abi.PRECISION_FP64
+    ref, _ = acc.trace_batch([cam], [m], o64)
+    e = (got.flags & abi.FLAG_ESCALATED) != 0
+    assert e.sum() > 10
+    for k in ("cls", "theta", "phi", "lam", "disk_r", "redshift", "steps", "intensity"):
+        assert np.array_equal(getattr(got, k)[..., e], getattr(ref, k)[..., e]), k
+    crit = acc.critical_distance([cam], [m])
+    gated = np.abs(crit) < o.escalate_band
+    assert np.all(e[gated])
+
+
+def test_multi_camera_batch_equals_single_cameras(acc):
+    """Cameras with different spacetimes in one persistent launch give exactly the results of
+    tracing each camera alone."""
+    metrics = [abi.metric_ks(1.0, 0.3, 0.2, 5e-5), abi.metric_ks(1.0, 0.95), abi.metric_ks(1.0, 0.0, 0.4)]
+    cams = [abi.camera(25.0, 30.0, 25.0, 64, 48, metric_index=0),
+            abi.camera(60.0, 85.0, 15.0, 37, 29, metric_index=1),
+            abi.camera(40.0, 60.0, 35.0, 50, 50, metric_index=2)]
+    crit = acc.critical_distance(cams, metrics)
+    for prec in (abi.PRECISION_FP64, abi.PRECISION_FP32_ESCALATE):
+        o = O.default_opts(prec)
+        o.n_bins = 8
+        batch, _ = acc.trace_batch(cams, metrics, o)
+        off = 0
+        for c in cams:
+            single_cam = abi.Camera.from_buffer_copy(c)
+            single_cam.metric_index = 0
+            one, _ = acc.trace_batch([single_cam], [metrics[c.metric_index]], o)
+            n = c.width * c.height
+            sl = slice(off, off + n)
+            if prec == abi.PRECISION_FP64:
+                # FP64: the multi-camera and single-camera kernel variants agree bit for bit
+                for k in ("cls", "theta", "phi", "disk_r", "steps"):
+                    assert np.array_equal(getattr(batch, k)[sl], getattr(one, k)), k
+                assert np.array_equal(batch.intensity[:, sl], one.intensity)
+            else:
+                # FP32: the two variants contract FMAs differently; they agree to the FP32 bar
+                band = np.abs(crit[sl]) > 0.5
+                assert np.array_equal(batch.cls[sl][band], one.cls[band])
+                esc = band & (one.cls == 0) & (batch.cls[sl] == 0)
+                assert _angle(batch.take(np.arange(off, off + n)).direction()[esc], one.direction()[esc]).max() < 2e-5
+                dk = band & (one.cls == 2) & (batch.cls[sl] == 2)
+                I1, I0 = batch.intensity[:, sl][:, dk], one.intensity[:, dk]
+                rel = np.abs(I1 - I0).max(0) / np.maximum(I0.max(0), 1e-2 * np.median(I0.max(0)))
+                assert rel.max() < 1e-3
+            off += n
+
+
+def test_charged_and_lambda_spacetimes_vs_oracle(acc):
+    """Kerr-Newman-de Sitter parameters (Q, Lambda) through the Sherman-Morrison inverse
+    against the oracle's general inverse."""
+    for M, a, Q, L in ((1.0, 0.6, 0.5, 0.0), (1.0, 0.2, 0.0, 1e-4), (1.0, 0.8, 0.3, 8e-5)):
+        m = abi.metric_ks(M, a, Q, L)
+        cam = abi.camera(40.0, 70.0, 30.0, 40, 30, disk=(0.0, 20.0))
+        o = O.default_opts(abi.PRECISION_FP64)
+        o.n_bins = 8
+        ref, _ = O.trace([cam], [m], o)
+        got, _ = acc.trace_batch([cam], [m], o)
+        assert np.array_equal(got.cls, ref.cls)
+        esc = ref.cls == 0
+        assert _angle(got.direction()[esc], ref.direction()[esc]).max() < 1e-6
+        dk = ref.cls == 2
+        assert np.allclose(got.intensity[:, dk], ref.intensity[:, dk], rtol=1e-6, atol=1e-12)
+
+
+def test_critical_distance_matches_oracle(acc):
+    for a in (0.0, 0.9):
+        m = abi.metric_ks(1.0, a)
+        cam = abi.camera(50.0, 75.0, 20.0, 64, 36)
+        got = acc.critical_distance([cam], [m])
+        ref = np.array([O.critical_distance(cam, m, i % 64, i // 64) for i in range(64 * 36)])
+        assert np.abs(got - ref).max() < 1e-4
+
+
+def test_edge_cases_and_errors(acc):
+    m = abi.metric_ks(1.0, 0.5)
+    o = O.default_opts(abi.PRECISION_FP64)
+    out, _ = acc.trace_batch([abi.camera(30.0, 60.0, 30.0, 1, 1)], [m], o)
+    assert out.cls[0] == abi.RAY_CAPTURED
+    o.max_steps = 5
+    out, _ = acc.trace_batch([abi.camera(30.0, 60.0, 30.0, 3, 2)], [m], o)
+    assert np.all(out.cls == abi.RAY_MAX_STEPS) and np.all(out.steps == 5)
+    from sirius_amd.accelerator import SiriusHipError
+    bad = O.default_opts(abi.PRECISION_FP64)
+    bad.n_bins = 65
+    with pytest.raises(SiriusHipError) as e:
+        acc.trace_batch([abi.camera(30.0, 60.0, 30.0, 4, 4)], [m], bad)
+    assert e.value.rc == abi.E_INVALID
+    with pytest.raises(SiriusHipError):
+        acc.trace_batch([abi.camera(30.0, 60.0, 30.0, 4, 4, metric_index=2)], [m], o)
+    with pytest.raises(SiriusHipError):
+        acc.trace_batch([abi.camera(30.0, 60.0, 30.0, 4, 4)], [abi.metric_ks(1.0, 1.2)], o)
+    with pytest.raises(SiriusHipError):
+        acc.trace_batch([abi.camera(30.0, 60.0, 30.0, 0, 4)], [m], o)
+
+
+def test_accelerator_launch_shades_framebuffer(acc):
+    """IAccelerator path: initialise -> launch -> getFrameBuffer gives a W x H RGBA image made
+    from the physical outputs (disk pixels lit, shadow black)."""
+    w, h = 64, 36
+    acc.initialise(w, h)
+    assert acc.isInitialised() and acc.getFrameBufferSize() == w * h * 16
+    m = abi.metric_ks(1.0, 0.9)
+    cam = abi.camera(50.0, 75.0, 20.0, w, h, disk=(0.0, 30.0))
+    o = O.default_opts(abi.PRECISION_FP32_ESCALATE)
+    acc.launch(cam, m, o)
+    fb = acc.getFrameBuffer().reshape(h, w, 4)
+    ref, _ = acc.trace_batch([cam], [m], o)
+    cls = ref.cls.reshape(h, w)
+    assert np.all(fb[..., 3] == 1.0)
+    assert np.all(fb[cls == abi.RAY_CAPTURED][:, :3] == 0.0)
+    assert fb[cls == abi.RAY_DISK][:, :3].max() > 0.0
+    bg = np.zeros((8, 16, 4), np.uint8)
+    bg[..., 1] = 200
+    acc.uploadBackground(bg)
+    acc.launch(cam, m, o)
+    fb2 = acc.getFrameBuffer().reshape(h, w, 4)
+    if (cls == abi.RAY_ESCAPED).any():
+        assert np.allclose(fb2[cls == abi.RAY_ESCAPED][:, 1], 200 / 255)
+    with pytest.raises(Exception):
+        acc.launch(abi.camera(50.0, 75.0, 20.0, w + 1, h), m, o)
