@@ -37,16 +37,18 @@ def workload():
     return cam, metric
 
 
-def time_frames(acc, cam, metric, opts, steps, warmup):
+def time_frames(acc, cam, metric, opts, steps, warmup, n_stats=3):
+    """Wall time per frame over `steps` frames (no statistics pass inside the timed region),
+    then `n_stats` frames with statistics for the HIP-event kernel times and step counts."""
     for _ in range(warmup):
-        acc.trace_resident([cam], [metric], opts)
+        acc.trace_resident([cam], [metric], opts, stats=False)
     acc.synchronise()
-    stats = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        stats.append(acc.trace_resident([cam], [metric], opts))
+        acc.trace_resident([cam], [metric], opts, stats=False)
     acc.synchronise()
     dt = time.perf_counter() - t0
+    stats = [acc.trace_resident([cam], [metric], opts) for _ in range(n_stats)]
     return dt / steps, stats
 
 
@@ -152,6 +154,7 @@ def main():
                      "traffic": None, "flops_per_launch": s["flops"], "avg_launch_ms": kern_ms},
         "geodesic_steps_per_s": s["steps_bulk"] / (kern_ms * 1e-3),
         "rays_escalated": s["rays_escalated"],
+        "escalate_band": o32.escalate_band,
         "escalate_ms": float(np.mean([x["escalate_ms"] for x in st32])),
         "fp64_plain": {"value": n_rays / ms64 / 1e6, "unit": "Mrays/s", "ms_per_step": ms64 * 1e3,
                        "kernel_ms": k64, "steps_per_ray": s64["steps_bulk"] / n_rays,

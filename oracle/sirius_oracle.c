@@ -314,7 +314,7 @@ void oracle_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->x_min = 0.1;
     o->x_max = 10.0;
     o->escalate_margin = 1.0;
-    o->escalate_band = 0.5;
+    o->escalate_band = 0.25;
     o->surrogate_band = 0.5;
     o->surrogate_pmin = 0.99;
     o->audit_fraction = 0.01;

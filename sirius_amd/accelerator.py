@@ -169,19 +169,21 @@ class HipAccelerator:
                                                    C.byref(opts), C.byref(ro), C.byref(st)))
         return out, st.as_dict()
 
-    def trace_resident(self, cams, metrics, opts):
+    def trace_resident(self, cams, metrics, opts, stats=True):
         """Trace with every result left in library device memory (nothing copied back).
-        Returns the stats dict; fetch_results() copies the results afterwards."""
+        Returns the stats dict (None with stats=False: no statistics pass and no read-back);
+        fetch_results() copies the results afterwards."""
         cams, metrics = list(cams), list(metrics)
         ca = (abi.Camera * len(cams))(*cams)
         ma = (abi.Metric * len(metrics))(*metrics)
         st = abi.Stats()
         ro = abi.RayOutput(on_device=1)
         self._check(self._L.sirius_hip_trace_batch(self._h, ca, len(cams), ma, len(metrics),
-                                                   C.byref(opts), C.byref(ro), C.byref(st)))
+                                                   C.byref(opts), C.byref(ro),
+                                                   C.byref(st) if stats else None))
         self._resident_n = sum(c.width * c.height for c in cams)
         self._resident_bins = opts.n_bins
-        return st.as_dict()
+        return st.as_dict() if stats else None
 
     def fetch_results(self, with_state=False):
         """Host copy of the results of the last trace_resident call."""

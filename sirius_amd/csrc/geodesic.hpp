@@ -567,20 +567,20 @@ SIRIUS_HD __forceinline__ void disk_emission(const CamConst& c, const TraceParam
 // ------------------------------------------------------------------------------------------
 #define SIRIUS_CC_TABLE 512
 
-SIRIUS_HD __forceinline__ double crit_distance(const float* tab, const Spacetime<double>& s, const double x[4],
-                                               const double p[4]) {
-    double E = -p[0];
-    double L = x[1] * p[2] - x[2] * p[1];
-    double r = kerr_r(s, x[1], x[2], x[3]);
-    double ct = x[3] / r, st = sqrt(fmax(1.0 - ct * ct, 1e-300));
-    double pth = (ct / st) * (x[1] * p[1] + x[2] * p[2]) - r * st * p[3];
-    double da = -L / (E * st) - (double)tab[0], b = fabs(pth / E);
-    double psi = atan2(b, da), rho = sqrt(da * da + b * b);
-    double u = psi / 3.141592653589793 * (SIRIUS_CC_TABLE - 1);
+template <class T>
+SIRIUS_HD __forceinline__ T crit_distance(const float* tab, const Spacetime<T>& s, const T x[4], const T p[4]) {
+    T E = -p[0];
+    T L = x[1] * p[2] - x[2] * p[1];
+    T r = kerr_r(s, x[1], x[2], x[3]);
+    T ct = x[3] / r, st = sqrt(fmax(T(1) - ct * ct, T(1e-30)));
+    T pth = (ct / st) * (x[1] * p[1] + x[2] * p[2]) - r * st * p[3];
+    T da = -L / (E * st) - T(tab[0]), b = fabs(pth / E);
+    T psi = atan2(b, da), rho = sqrt(da * da + b * b);
+    T u = psi / T(3.141592653589793) * T(SIRIUS_CC_TABLE - 1);
     int j = (int)u;
     j = j < 0 ? 0 : (j > SIRIUS_CC_TABLE - 2 ? SIRIUS_CC_TABLE - 2 : j);
-    double w = u - j;
-    return rho - ((double)tab[1 + j] + w * ((double)tab[2 + j] - (double)tab[1 + j]));
+    T w = u - T(j);
+    return rho - (T(tab[1 + j]) + w * (T(tab[2 + j]) - T(tab[1 + j])));
 }
 
 }  // namespace sirius

@@ -78,13 +78,15 @@ void critical_table(const CamConst& c, float* out) {
 // Uploads the critical tables of a batch (n_cam x (N + 1) floats) and returns the device copy.
 float* upload_critical_tables(Handle& H, Batch& B, int* rc) {
     int n_cam = (int)B.cc.size();
-    std::vector<float> tabs((size_t)n_cam * (SIRIUS_CC_TABLE + 1), 0.f);
+    // the host copy lives in the handle, so the asynchronous upload needs no synchronisation
+    // (the stream is synchronised before this buffer is rewritten by the next batch)
+    std::vector<float>& tabs = H.crit_host;
+    tabs.assign((size_t)n_cam * (SIRIUS_CC_TABLE + 1), 0.f);
     for (int i = 0; i < n_cam; ++i)
         if (B.cc[i].family == 0) critical_table(B.cc[i], tabs.data() + (size_t)i * (SIRIUS_CC_TABLE + 1));
     float* d = (float*)H.scratch("crit_tabs", tabs.size() * 4, rc);
     if (*rc) return nullptr;
     hipError_t e = hipMemcpyAsync(d, tabs.data(), tabs.size() * 4, hipMemcpyHostToDevice, H.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(H.stream);  // the host vector dies here
     if (e != hipSuccess) {
         *rc = H.hip_fail(e, "upload critical tables");
         return nullptr;

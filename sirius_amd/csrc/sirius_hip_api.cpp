@@ -352,7 +352,8 @@ int prepare_batch(Handle& H, const SiriusHipCamera* cams, int n_cam, const Siriu
     void* d = H.scratch("cams", sizeof(CamConst) * n_cam, &rc);
     if (!d) return rc;
     B.cams_dev = (CamConst*)d;
-    hipError_t e = hipMemcpyAsync(d, B.cc.data(), sizeof(CamConst) * n_cam, hipMemcpyHostToDevice, H.stream);
+    H.cam_host = B.cc;  // outlives the asynchronous copy
+    hipError_t e = hipMemcpyAsync(d, H.cam_host.data(), sizeof(CamConst) * n_cam, hipMemcpyHostToDevice, H.stream);
     if (e != hipSuccess) return H.hip_fail(e, "upload cameras");
     return SIRIUS_HIP_OK;
 }
@@ -698,7 +699,7 @@ void sirius_hip_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->x_min = 0.1;
     o->x_max = 10.0;
     o->escalate_margin = 1.0;
-    o->escalate_band = 0.5;
+    o->escalate_band = 0.25;
     o->surrogate_band = 0.5;
     o->surrogate_pmin = 0.99;
     o->audit_fraction = 0.01;

@@ -35,6 +35,8 @@ struct Handle {
     std::string device_name;
     std::string last_error;
     std::map<std::string, Buf> bufs;
+    std::vector<float> crit_host;       // host copies that async uploads read from
+    std::vector<CamConst> cam_host;
     // framebuffer (IAccelerator::initialise / getFrameBuffer)
     float* fb_dev = nullptr;
     std::vector<float> fb_host;

@@ -10,48 +10,51 @@ namespace sirius {
 
 // acc[0] attempts, acc[1] rejected, acc[2] landings, acc[3] max null drift (double bits),
 // acc[4] escalated rays, acc[5] surrogate rays, acc[6] audited rays, acc[7] attempts of the
-// escalated rays.
+// escalated rays. Grid-stride loop, wave then block reduction in LDS, one atomic per block
+// and counter.
 __global__ __launch_bounds__(256) void k_reduce_stats(DevOut o, int64_t n, unsigned long long* acc) {
-    unsigned long long a0 = 0, a1 = 0, a2 = 0, a4 = 0, a5 = 0, a6 = 0, a7 = 0;
+    unsigned long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double dmax = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        a0 += o.steps[i];
-        a1 += o.rejected[i];
-        a2 += o.landings[i];
+        uint32_t st = o.steps[i];
+        v[0] += st;
+        v[1] += o.rejected[i];
+        v[2] += o.landings[i];
         uint8_t f = o.flags[i];
-        a4 += (f & SIRIUS_FLAG_ESCALATED) ? 1 : 0;
-        a7 += (f & SIRIUS_FLAG_ESCALATED) ? o.steps[i] : 0;
-        a5 += (f & SIRIUS_FLAG_SURROGATE) ? 1 : 0;
-        a6 += (f & SIRIUS_FLAG_AUDITED) ? 1 : 0;
+        v[4] += (f & SIRIUS_FLAG_ESCALATED) ? 1 : 0;
+        v[7] += (f & SIRIUS_FLAG_ESCALATED) ? st : 0;
+        v[5] += (f & SIRIUS_FLAG_SURROGATE) ? 1 : 0;
+        v[6] += (f & SIRIUS_FLAG_AUDITED) ? 1 : 0;
         double d = (double)o.null_drift[i];
-        if (d > dmax) dmax = d;
+        dmax = d > dmax ? d : dmax;
     }
     for (int off = 32; off > 0; off >>= 1) {
-        a0 += __shfl_xor(a0, off);
-        a1 += __shfl_xor(a1, off);
-        a2 += __shfl_xor(a2, off);
-        a4 += __shfl_xor(a4, off);
-        a5 += __shfl_xor(a5, off);
-        a6 += __shfl_xor(a6, off);
-        a7 += __shfl_xor(a7, off);
+        for (int k = 0; k < 8; ++k) v[k] += __shfl_xor(v[k], off);
         double t = __shfl_xor(dmax, off);
         dmax = t > dmax ? t : dmax;
     }
+    __shared__ unsigned long long part[4][8];
+    __shared__ double pmax[4];
+    int w = threadIdx.x >> 6;
     if (__lane_id() == 0) {
-        atomicAdd(&acc[0], a0);
-        atomicAdd(&acc[1], a1);
-        atomicAdd(&acc[2], a2);
-        atomicAdd(&acc[4], a4);
-        atomicAdd(&acc[5], a5);
-        atomicAdd(&acc[6], a6);
-        atomicAdd(&acc[7], a7);
-        atomicMax(&acc[3], (unsigned long long)__double_as_longlong(dmax));
+        for (int k = 0; k < 8; ++k) part[w][k] = v[k];
+        pmax[w] = dmax;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8 && threadIdx.x != 3) {
+        unsigned long long t = 0;
+        for (int j = 0; j < (int)(blockDim.x >> 6); ++j) t += part[j][threadIdx.x];
+        atomicAdd(&acc[threadIdx.x], t);
+    } else if (threadIdx.x == 3) {
+        double t = 0.0;
+        for (int j = 0; j < (int)(blockDim.x >> 6); ++j) t = pmax[j] > t ? pmax[j] : t;
+        atomicMax(&acc[3], (unsigned long long)__double_as_longlong(t));
     }
 }
 
 hipError_t launch_reduce_stats(const DevOut& o, int64_t n, unsigned long long* acc, hipStream_t st) {
     int64_t blocks = (n + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
+    if (blocks > 512) blocks = 512;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_reduce_stats, dim3((unsigned)blocks), dim3(256), 0, st, o, n, acc);
     return hipGetLastError();
@@ -75,7 +78,7 @@ __global__ __launch_bounds__(256) void k_critical_distance(const CamConst* __res
     Spacetime<double> s = make_spacetime<double>(c);
     double x[4], p[4];
     init_state<double>(c, s, (int)(pix % c.width), (int)(pix / c.width), x, p);
-    out[id] = (float)crit_distance(tabs + (int64_t)lo * (SIRIUS_CC_TABLE + 1), s, x, p);
+    out[id] = (float)crit_distance<double>(tabs + (int64_t)lo * (SIRIUS_CC_TABLE + 1), s, x, p);
 }
 
 hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float* tabs, int64_t n, float* out,
@@ -103,12 +106,13 @@ __global__ __launch_bounds__(256) void k_gate(const CamConst* __restrict__ cams,
         }
         const CamConst& c = cams[lo];
         if (c.family == 0 && band > 0) {
+            // FP32 is ample for band membership (alpha, beta ~ 10 M, band ~ 0.1 M)
             int64_t pix = id - c.offset;
-            Spacetime<double> s = make_spacetime<double>(c);
-            double x[4], p[4];
-            init_state<double>(c, s, (int)(pix % c.width), (int)(pix / c.width), x, p);
-            double d = crit_distance(tabs + (int64_t)lo * (SIRIUS_CC_TABLE + 1), s, x, p);
-            to_b = fabs(d) < band;
+            Spacetime<float> s = make_spacetime<float>(c);
+            float x[4], p[4];
+            init_state<float>(c, s, (int)(pix % c.width), (int)(pix / c.width), x, p);
+            float d = crit_distance<float>(tabs + (int64_t)lo * (SIRIUS_CC_TABLE + 1), s, x, p);
+            to_b = fabsf(d) < (float)band;
         }
     }
     const int lane = __lane_id();

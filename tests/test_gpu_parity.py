@@ -129,7 +129,7 @@ def test_escalated_rays_are_bitwise_fp64(acc):
     for k in ("cls", "theta", "phi", "lam", "disk_r", "redshift", "steps", "intensity"):
         assert np.array_equal(getattr(got, k)[..., e], getattr(ref, k)[..., e]), k
     crit = acc.critical_distance([cam], [m])
-    gated = np.abs(crit) < o.escalate_band
+    gated = np.abs(crit) < o.escalate_band - 1e-3  # the gate decides in FP32
     assert np.all(e[gated])
 
 
