@@ -119,9 +119,14 @@ typedef struct SiriusHipTraceOpts {
     int32_t blocks_per_cu;   /* 0: autotuned                                                */
     int32_t refill_min;      /* persistent queue: refill a wave when >= this many lanes idle*/
     int32_t reserved0;
-    /* volumetric emission (emitter frame): j_nu = j0 r^-2 exp(-z^2 / (2 (h_over_r r)^2))
-     * * x^3 / (e^{x / tau_v(r)} - 1), tau_v = (6 / r)^{3/4}; alpha = kappa j0 r^-2 exp(...) */
+    /* volumetric emission (emitter frame): j_nu = rho(x) x^3 / (e^{x / tau_v(r)} - 1),
+     * rho = j0 r^-2 exp(-z^2 / (2 (h_over_r r)^2)) W(r), tau_v = (6 / r)^{3/4},
+     * alpha = kappa rho; W is a C1 smoothstep window, 0 at vol_r_min and vol_r_max and 1 on
+     * [1.1 vol_r_min, 0.9 vol_r_max] */
     double vol_j0, vol_h_over_r, vol_kappa, vol_r_min, vol_r_max, vol_tau_max;
+    double escalate_disk_edge; /* FP32 path, thin disk: a disk crossing whose radius lies within
+                              * this fraction of r_in (|r/r_in - 1| < edge) is re-traced in FP64,
+                              * where the emission's steep edge magnifies FP32 position error */
 } SiriusHipTraceOpts;
 
 /* Structure-of-arrays result buffers. Every array holds n_rays entries, except intensity

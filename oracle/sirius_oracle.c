@@ -325,6 +325,7 @@ void oracle_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->vol_r_min = 2.0;
     o->vol_r_max = 50.0;
     o->vol_tau_max = 10.0;
+    o->escalate_disk_edge = 0.02;
     if (precision == SIRIUS_PRECISION_FP32_ESCALATE) {
         o->rtol = 1e-6;
         o->atol = 1e-8;
@@ -750,10 +751,20 @@ static double vol_step_cap(const SiriusHipMetric* m, const SiriusHipTraceOpts* o
     return c > 0.2 * sig ? c : 0.2 * sig;
 }
 
+/* Smooth (C1) radial window of the emitting shell: 0 at r_min and r_max, 1 on
+ * [1.1 r_min, 0.9 r_max], cubic smoothstep between. */
+static double vol_window(double r, double r_min, double r_max) {
+    double u = (r - r_min) / (0.1 * r_min), v = (r_max - r) / (0.1 * r_max);
+    u = u < 0 ? 0 : (u > 1 ? 1 : u);
+    v = v < 0 ? 0 : (v > 1 ? 1 : v);
+    return u * u * (3 - 2 * u) * v * v * (3 - 2 * v);
+}
+
 /* Volumetric transfer over one accepted step, 3-node Gauss-Legendre on the Hermite
  * interpolant. Invariant form (I/nu^3, j/nu^2, nu alpha): with g = nu_obs/nu_emit,
  *   dtau = kappa rho nu_e dlam,  dI_k = rho S(x_k / g) (nu_o^3 / nu_e^2) e^{-tau} dlam,
- * rho = j0 r^-2 exp(-z^2/(2 sigma^2)), S(x) = x^3/(e^{x/tau_v} - 1), tau_v = (6/r)^{3/4},
+ * rho = j0 r^-2 exp(-z^2/(2 sigma^2)) W(r) (W: the shell window above),
+ * S(x) = x^3/(e^{x/tau_v} - 1), tau_v = (6/r)^{3/4},
  * emitter on a Keplerian (Kerr-Newman) circle at r_c = sqrt(x^2 + y^2 - a^2). This is the
  * build's definition of the volumetric case; the reference's gray straight-chord march
  * (GTRC001A.cpp:914-998) is the role it replaces. */
@@ -771,7 +782,8 @@ static void vol_accumulate(const cam_ctx* cx, const SiriusHipTraceOpts* o, const
         double r = oracle_kerr_radius(m, x[1], x[2], x[3]);
         if (r < o->vol_r_min || r > o->vol_r_max) continue;
         double sig = o->vol_h_over_r * r;
-        double rho = o->vol_j0 / (r * r) * exp(-x[3] * x[3] / (2 * sig * sig));
+        double rho = o->vol_j0 / (r * r) * exp(-x[3] * x[3] / (2 * sig * sig)) *
+                     vol_window(r, o->vol_r_min, o->vol_r_max);
         double M = m->M, a = m->a, Q2 = m->Q * m->Q;
         double rc = sqrt(fmax(x[1] * x[1] + x[2] * x[2] - a * a, 1e-12));
         double sq = sqrt(fmax(M * rc - Q2, 0.0));

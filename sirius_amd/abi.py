@@ -60,6 +60,7 @@ class TraceOpts(C.Structure):
         ("refill_min", C.c_int32), ("reserved0", C.c_int32),
         ("vol_j0", C.c_double), ("vol_h_over_r", C.c_double), ("vol_kappa", C.c_double),
         ("vol_r_min", C.c_double), ("vol_r_max", C.c_double), ("vol_tau_max", C.c_double),
+        ("escalate_disk_edge", C.c_double),
     ]
 
 

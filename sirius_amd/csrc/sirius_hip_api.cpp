@@ -340,6 +340,7 @@ int prepare_batch(Handle& H, const SiriusHipCamera* cams, int n_cam, const Siriu
     tp.sym_h = o.symplectic_h;
     tp.vol_j0 = o.vol_j0; tp.vol_h_over_r = o.vol_h_over_r; tp.vol_kappa = o.vol_kappa;
     tp.vol_r_min = o.vol_r_min; tp.vol_r_max = o.vol_r_max; tp.vol_tau_max = o.vol_tau_max;
+    tp.esc_edge = o.escalate_disk_edge;
     tp.max_steps = o.max_steps;
     tp.n_bins = o.n_bins;
     tp.emission = B.family == SIRIUS_METRIC_KERR_SCHILD ? o.emission : SIRIUS_EMISSION_NONE;
@@ -428,7 +429,7 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
     uint32_t* work = (uint32_t*)H.scratch("work", 64, &rc);
     if (rc) return rc;
     if ((e = launch_fill(work, 16, 0u, H.stream)) != hipSuccess) return H.hip_fail(e, "fill");
-    H.last_band = 0;
+    H.last_band = false;
     if (time_it) (void)hipEventRecord(H.ev0, H.stream);
     if (o.integrator == SIRIUS_INTEGRATOR_SYMPLECTIC) {
         if (list_dev) return H.fail(SIRIUS_HIP_E_UNSUPPORTED, "symplectic integrator with a ray list");
@@ -464,7 +465,6 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
         const int32_t* list_a = list_dev;
         const uint32_t* count_a = list_count_dev;
         int64_t cap_a = n_work;
-        uint32_t n_band = 0;
         int32_t* list_b = nullptr;
         bool gate = !list_dev && B.family == SIRIUS_METRIC_KERR_SCHILD && B.tp.escalate && o.escalate_band > 0;
         if (gate) {
@@ -476,27 +476,23 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
             if ((e = launch_gate(B.cams_dev, (int)B.cc.size(), tabs, B.n, o.escalate_band, la, list_b, counts,
                                  H.stream)) != hipSuccess)
                 return H.hip_fail(e, "gate launch");
-            uint32_t hc[2] = {0, 0};
-            if ((e = hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, H.stream)) != hipSuccess ||
-                (e = hipStreamSynchronize(H.stream)) != hipSuccess)
-                return H.hip_fail(e, "gate counts");
             list_a = la;
             count_a = counts;
             cap_a = B.n;
-            n_band = hc[1];
         }
         TraceParams t64 = B.tp;
         t64.escalate = 0;
-        if (n_band > 0) {
-            // FP64 band on the second stream, small blocks so the long rays spread over the chip
+        if (gate) {
+            // FP64 band on the second stream, small blocks so the long rays spread over the chip.
+            // The band's length stays on the device (no host round trip): the persistent grid is
+            // sized for the whole chip and waves with nothing to trace retire at once.
             (void)hipEventRecord(H.ev_gate, H.stream);
             (void)hipStreamWaitEvent(H.stream2, H.ev_gate, 0);
             int bpc64 = persistent_blocks_per_cu(true, B.family, B.tp.emission, B.tp.n_cam > 1, 64);
-            int grid64 = (int)std::min<int64_t>((int64_t)bpc64 * H.cu_count, ((int64_t)n_band + 63) / 64);
             (void)hipEventRecord(H.ev_band0, H.stream2);
             if ((e = launch_trace_persistent(true, B.family, B.tp.emission, B.tp.n_cam > 1, B.cams_dev, t64, dout, list_b,
                                              work + 9, B.n, work + 12, work + 13, nullptr,
-                                             (unsigned long long*)(work + 14), std::max(1, grid64), 64, 1,
+                                             (unsigned long long*)(work + 14), std::max(1, bpc64 * H.cu_count), 64, 1,
                                              SIRIUS_FLAG_ESCALATED, H.stream2)) != hipSuccess)
                 return H.hip_fail(e, "band launch");
             (void)hipEventRecord(H.ev_band1, H.stream2);
@@ -519,8 +515,8 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
                                              SIRIUS_FLAG_ESCALATED, H.stream)) != hipSuccess)
                 return H.hip_fail(e, "escalation launch");
         }
-        if (n_band > 0) (void)hipStreamWaitEvent(H.stream, H.ev_band1, 0);
-        H.last_band = n_band;
+        if (gate) (void)hipStreamWaitEvent(H.stream, H.ev_band1, 0);
+        H.last_band = gate;
     }
     if (time_it) {
         if (!persistent) (void)hipEventRecord(H.ev1, H.stream);
@@ -567,7 +563,7 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
             float ms = 0, ms2 = 0, ms3 = 0;
             (void)hipEventElapsedTime(&ms, H.ev0, H.ev1);
             (void)hipEventElapsedTime(&ms2, H.ev1, H.ev4);
-            if (H.last_band > 0) (void)hipEventElapsedTime(&ms3, H.ev_band0, H.ev_band1);
+            if (H.last_band) (void)hipEventElapsedTime(&ms3, H.ev_band0, H.ev_band1);
             stats->kernel_ms += ms;
             stats->escalate_ms += std::max(ms2, ms3);
         }
@@ -710,6 +706,7 @@ void sirius_hip_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->vol_r_min = 2.0;
     o->vol_r_max = 50.0;
     o->vol_tau_max = 10.0;
+    o->escalate_disk_edge = 0.02;
     if (precision == SIRIUS_PRECISION_FP32_ESCALATE) {
         o->rtol = 1e-6;
         o->atol = 1e-8;

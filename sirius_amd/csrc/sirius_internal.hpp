@@ -46,7 +46,7 @@ struct Handle {
     // results of the last trace_batch held in device memory (sirius_hip_fetch_results)
     DevOut* last = nullptr;
     int last_bins = 0;
-    uint32_t last_band = 0;  // rays the last FP32 trace sent to the FP64 band
+    bool last_band = false;  // the last FP32 trace ran the FP64 band on the second stream
     // surrogate network
     MlpState* mlp = nullptr;
 

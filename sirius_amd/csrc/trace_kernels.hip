@@ -176,6 +176,17 @@ SIRIUS_HD __forceinline__ void ray_outputs(const CamConst& c, const TraceParams&
 // Volumetric transfer over an accepted step (3-node Gauss-Legendre on the cubic Hermite
 // interpolant of (x, p)). Emitter-frame j_nu = rho S(x_e), alpha = kappa rho, invariant form:
 //   dtau = kappa rho nu_e dlam,  dI_k = rho S(x_k / g) (nu_o^3 / nu_e^2) e^{-tau} dlam.
+// Smooth (C1) radial window of the emitting shell: zero at r_min and r_max, one between
+// 1.1 r_min and 0.9 r_max (cubic smoothstep), so the emissivity has no jump for the per-step
+// quadrature to straddle.
+template <class T>
+SIRIUS_HD __forceinline__ T vol_window(T r, T r_min, T r_max) {
+    T u = (r - r_min) / (T(0.1) * r_min), v = (r_max - r) / (T(0.1) * r_max);
+    u = u < T(0) ? T(0) : (u > T(1) ? T(1) : u);
+    v = v < T(0) ? T(0) : (v > T(1) ? T(1) : v);
+    return u * u * (T(3) - T(2) * u) * v * v * (T(3) - T(2) * v);
+}
+
 template <class T, int FAM>
 SIRIUS_HD __forceinline__ void vol_accumulate(Ray<T, FAM>& R, const CamConst& c, const Spacetime<T>& s,
                                               const TraceParams& tp, const DevOut& o, const T x1[4], const T p1[4],
@@ -189,7 +200,8 @@ SIRIUS_HD __forceinline__ void vol_accumulate(Ray<T, FAM>& R, const CamConst& c,
         T r = kerr_r(s, xq[1], xq[2], xq[3]);
         if (r < T(tp.vol_r_min) || r > T(tp.vol_r_max)) continue;
         T sig = T(tp.vol_h_over_r) * r;
-        T rho = T(tp.vol_j0) / (r * r) * exp(-xq[3] * xq[3] / (T(2) * sig * sig));
+        T rho = T(tp.vol_j0) / (r * r) * exp(-xq[3] * xq[3] / (T(2) * sig * sig)) *
+                vol_window(r, T(tp.vol_r_min), T(tp.vol_r_max));
         // emitter velocity and frequencies in FP64; p by cubic Hermite with dp/dlam
         double th = (double)gn[q], t2 = th * th, t3 = t2 * th, hh = (double)R.h;
         double h00 = 2 * t3 - 3 * t2 + 1, h10 = t3 - 2 * t2 + th, h01 = 3 * t2 - 2 * t3, h11 = t3 - t2;
@@ -314,6 +326,11 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
             T xc[4];
             hermite(R.x, R.d.k, x1, d1.k, R.h, th_disk, xc);
             T rc = kerr_r(s, xc[1], xc[2], xc[3]);
+            if constexpr (fp32) {
+                // near the inner edge the emission falls steeply to zero: an FP32 position error
+                // there is a large relative intensity error, so such rays go to FP64
+                if (tp.escalate && fabs(rc - T(c.r_in)) < T(tp.esc_edge * c.r_in)) return ST_ESCALATE;
+            }
             if (rc >= T(c.r_in) && rc <= T(c.r_out)) ev = SIRIUS_RAY_DISK;
         }
     }

@@ -89,55 +89,83 @@ hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float
 }
 
 // Gate of the FP32 path: rays within `band` of the critical curve go to list B (traced in FP64
-// from the start, on a second stream), all others to list A (FP32 persistent tracer). Appends
-// are aggregated per wave: one atomic per wave and list. counts[0] = |A|, counts[1] = |B|.
+// from the start, on a second stream), all others to list A (FP32 persistent tracer). A block
+// of 256 threads classifies a contiguous chunk of GATE_RAYS rays and reserves its slots with
+// one atomic per list (device-scope atomics on one address serialise, so one per wave is too
+// many at 1080p); within a chunk the lists keep ray order. counts[0] = |A|, counts[1] = |B|.
+constexpr int GATE_PASSES = 8;
+constexpr int GATE_RAYS = 256 * GATE_PASSES;
+
+__device__ __forceinline__ bool gate_in_band(const CamConst* __restrict__ cams, int n_cam,
+                                             const float* __restrict__ tabs, int64_t id, float band) {
+    int lo = 0, hi = n_cam - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (cams[mid].offset <= id) lo = mid; else hi = mid - 1;
+    }
+    const CamConst& c = cams[lo];
+    if (c.family != 0) return false;
+    // FP32 is ample for band membership (alpha, beta ~ 10 M, band ~ 0.1 M)
+    int64_t pix = id - c.offset;
+    Spacetime<float> s = make_spacetime<float>(c);
+    float x[4], p[4];
+    init_state<float>(c, s, (int)(pix % c.width), (int)(pix / c.width), x, p);
+    float d = crit_distance<float>(tabs + (int64_t)lo * (SIRIUS_CC_TABLE + 1), s, x, p);
+    return fabsf(d) < band;
+}
+
 __global__ __launch_bounds__(256) void k_gate(const CamConst* __restrict__ cams, int n_cam,
-                                              const float* __restrict__ tabs, int64_t n, double band,
+                                              const float* __restrict__ tabs, int64_t n, float band,
                                               int32_t* __restrict__ list_a, int32_t* __restrict__ list_b,
                                               uint32_t* __restrict__ counts) {
-    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = id < n;
-    bool to_b = false;
-    if (valid) {
-        int lo = 0, hi = n_cam - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (cams[mid].offset <= id) lo = mid; else hi = mid - 1;
-        }
-        const CamConst& c = cams[lo];
-        if (c.family == 0 && band > 0) {
-            // FP32 is ample for band membership (alpha, beta ~ 10 M, band ~ 0.1 M)
-            int64_t pix = id - c.offset;
-            Spacetime<float> s = make_spacetime<float>(c);
-            float x[4], p[4];
-            init_state<float>(c, s, (int)(pix % c.width), (int)(pix / c.width), x, p);
-            float d = crit_distance<float>(tabs + (int64_t)lo * (SIRIUS_CC_TABLE + 1), s, x, p);
-            to_b = fabsf(d) < (float)band;
+    __shared__ uint32_t off_a[GATE_PASSES][4], off_b[GATE_PASSES][4];
+    __shared__ uint32_t base[2];
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    const int64_t chunk = (int64_t)blockIdx.x * GATE_RAYS;
+    const unsigned long long below = lane ? ((1ull << lane) - 1ull) : 0ull;
+    uint32_t in_b = 0;  // bit k: ray of pass k is in the band
+#pragma unroll
+    for (int k = 0; k < GATE_PASSES; ++k) {
+        int64_t id = chunk + k * 256 + threadIdx.x;
+        bool valid = id < n;
+        bool to_b = valid && band > 0.f && gate_in_band(cams, n_cam, tabs, id, band);
+        in_b |= (uint32_t)to_b << k;
+        unsigned long long ma = __ballot(valid && !to_b), mb = __ballot(to_b);
+        if (lane == 0) {
+            off_a[k][wave] = (uint32_t)__popcll(ma);
+            off_b[k][wave] = (uint32_t)__popcll(mb);
         }
     }
-    const int lane = __lane_id();
-    unsigned long long below = lane ? ((1ull << lane) - 1ull) : 0ull;
-    unsigned long long ma = __ballot(valid && !to_b), mb = __ballot(valid && to_b);
-    int leader = __ffsll((long long)(ma | mb)) - 1;
-    uint32_t ba = 0, bb = 0;
-    if (leader >= 0 && lane == leader) {
-        if (ma) ba = atomicAdd(&counts[0], (uint32_t)__popcll(ma));
-        if (mb) bb = atomicAdd(&counts[1], (uint32_t)__popcll(mb));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sa = 0, sb = 0;
+        for (int k = 0; k < GATE_PASSES; ++k)
+            for (int w = 0; w < 4; ++w) {
+                uint32_t ta = off_a[k][w], tb = off_b[k][w];
+                off_a[k][w] = sa;
+                off_b[k][w] = sb;
+                sa += ta;
+                sb += tb;
+            }
+        base[0] = sa ? atomicAdd(&counts[0], sa) : 0u;
+        base[1] = sb ? atomicAdd(&counts[1], sb) : 0u;
     }
-    if (leader >= 0) {
-        ba = __shfl(ba, leader);
-        bb = __shfl(bb, leader);
-    }
-    if (valid) {
-        if (to_b) list_b[bb + __popcll(mb & below)] = (int32_t)id;
-        else list_a[ba + __popcll(ma & below)] = (int32_t)id;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < GATE_PASSES; ++k) {
+        int64_t id = chunk + k * 256 + threadIdx.x;
+        bool valid = id < n;
+        bool to_b = (in_b >> k) & 1u;
+        unsigned long long ma = __ballot(valid && !to_b), mb = __ballot(to_b);
+        if (to_b) list_b[base[1] + off_b[k][wave] + __popcll(mb & below)] = (int32_t)id;
+        else if (valid) list_a[base[0] + off_a[k][wave] + __popcll(ma & below)] = (int32_t)id;
     }
 }
 
 hipError_t launch_gate(const CamConst* cams, int n_cam, const float* tabs, int64_t n, double band, int32_t* list_a,
                        int32_t* list_b, uint32_t* counts, hipStream_t st) {
-    hipLaunchKernelGGL(k_gate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cams, n_cam, tabs, n, band,
-                       list_a, list_b, counts);
+    hipLaunchKernelGGL(k_gate, dim3((unsigned)((n + GATE_RAYS - 1) / GATE_RAYS)), dim3(256), 0, st, cams, n_cam, tabs,
+                       n, (float)band, list_a, list_b, counts);
     return hipGetLastError();
 }
 
