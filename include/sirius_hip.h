@@ -70,7 +70,8 @@ enum {
     SIRIUS_FLAG_SURROGATE = 2,   /* class/direction came from the MLP, not the integrator  */
     SIRIUS_FLAG_RENORM = 4,      /* the null constraint was re-imposed at least once       */
     SIRIUS_FLAG_AUDITED = 8,     /* ray belongs to the random audit set                    */
-    SIRIUS_FLAG_FAR_SIDE = 16    /* wormhole: escaped on the far side of the throat        */
+    SIRIUS_FLAG_FAR_SIDE = 16,   /* wormhole: escaped on the far side of the throat        */
+    SIRIUS_FLAG_CRITICAL = 32    /* FP32 path: within critical_band of the critical curve  */
 };
 
 /* ---- descriptors ---------------------------------------------------------------------- */
@@ -127,6 +128,11 @@ typedef struct SiriusHipTraceOpts {
     double escalate_disk_edge; /* FP32 path, thin disk: a disk crossing whose radius lies within
                               * this fraction of r_in (|r/r_in - 1| < edge) is re-traced in FP64,
                               * where the emission's steep edge magnifies FP32 position error */
+    double escalate_rtol;    /* relative tolerance of escalated FP64 re-traces (atol scales with
+                              * it); they keep the FP32 path's step caps. 0: the path's rtol    */
+    double critical_band;    /* FP32 path, Kerr-Schild: rays within this distance (M) of the
+                              * critical curve are flagged SIRIUS_FLAG_CRITICAL; the accuracy
+                              * contract exempts them and the inner-edge escalation skips them */
 } SiriusHipTraceOpts;
 
 /* Structure-of-arrays result buffers. Every array holds n_rays entries, except intensity
@@ -227,24 +233,34 @@ int sirius_hip_autotune(SiriusHipHandle h, const SiriusHipCamera* cams, int n_ca
                         double* best_ms);
 
 /* ---- MLP far-field surrogate (new) ------------------------------------------------------ */
-/* Network: 5 inputs -> 4 x 256 GELU -> 6 outputs (3 class logits, 3 direction residual).
- * Weights are FP32 masters on the device; forward/backward run in BF16 MFMA with FP32
- * accumulation. */
+/* Network: 5 inputs -> 4 x 256 GELU (tanh form) -> 6 outputs: 3 class logits (escaped,
+ * captured, disk) and the escape direction residual d_esc - n_0 (n_0: the ray's initial
+ * Cartesian direction at the camera). Inputs: a, theta_obs / (pi/2), (r_obs - 60) / 40,
+ * alpha / 15, beta / 15 (alpha, beta: the ray's impact parameters on the observer's sky, M).
+ * Parameters are FP32 masters on the device, laid out layer by layer as W_l (out x in,
+ * row-major) then b_l; forward and backward run on BF16 MFMA with FP32 accumulation (GEMM
+ * operands rounded to BF16: activations, weights, and the output/pre-activation gradients).
+ * Loss per sample: softmax cross-entropy + SIRIUS_MLP_DIR_WEIGHT * |residual error|^2 for
+ * escaped rays; the batch loss is the mean. */
 #define SIRIUS_MLP_IN 5
 #define SIRIUS_MLP_HIDDEN 256
 #define SIRIUS_MLP_LAYERS 4
 #define SIRIUS_MLP_OUT 6
+#define SIRIUS_MLP_DIR_WEIGHT 100.0
 size_t sirius_hip_mlp_param_count(void);
+/* Deterministic initialisation from a seed (uniform +-sqrt(6 / fan_in) weights, zero biases). */
 int sirius_hip_mlp_init(SiriusHipHandle h, uint64_t seed);
 int sirius_hip_mlp_get_params(SiriusHipHandle h, float* host_params, size_t n);
 int sirius_hip_mlp_set_params(SiriusHipHandle h, const float* host_params, size_t n);
-/* Forward pass on n samples x[n*5] (device), logits+direction y[n*6] (device). */
+/* Forward pass on n samples x[n*5] (device), outputs y[n*6] (device). */
 int sirius_hip_mlp_forward(SiriusHipHandle h, const float* x, float* y, int n);
-/* Loss and parameter gradient for a batch; targets cls[n] (int32) and dir[n*3];
- * grad is FP32, length param_count (device). Returns the mean loss in *loss. */
+/* Loss and parameter gradient for a batch; targets cls[n] (int32: 0 escaped, 1 captured,
+ * 2 disk) and dir[n*3] (residuals, read for escaped rays); grad is FP32, length param_count
+ * (device). The mean loss is returned in *loss (host). */
 int sirius_hip_mlp_loss_grad(SiriusHipHandle h, const float* x, const int32_t* cls,
                              const float* dir, int n, float* grad, float* loss);
-/* One Adam step over a batch (forward + backward + update), on device buffers. */
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) over a batch on device buffers; *loss (host)
+ * receives the batch loss before the update. */
 int sirius_hip_mlp_train_step(SiriusHipHandle h, const float* x, const int32_t* cls,
                               const float* dir, int n, float lr, float* loss);
 

@@ -124,3 +124,42 @@ def trace(cams, metrics, opts, ray_ids=None, threads=0, with_state=False):
     if rc != 0:
         raise RuntimeError(f"oracle_trace failed: {rc}")
     return out, st.as_dict()
+
+
+def _mlp_lib():
+    L = lib()
+    if not getattr(L, "_mlp_ready", False):
+        fp, dp = C.POINTER(C.c_float), C.POINTER(C.c_double)
+        L.oracle_mlp_param_count.restype = C.c_size_t
+        L.oracle_mlp_forward.argtypes = [fp, fp, C.c_int, dp, C.c_int]
+        L.oracle_mlp_loss_grad.restype = C.c_double
+        L.oracle_mlp_loss_grad.argtypes = [fp, fp, C.POINTER(C.c_int32), fp, C.c_int, dp, C.c_int]
+        L._mlp_ready = True
+    return L
+
+
+def mlp_param_count():
+    return int(_mlp_lib().oracle_mlp_param_count())
+
+
+def mlp_forward(params, x, round_bf16=False):
+    L = _mlp_lib()
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros((len(x), 6))
+    L.oracle_mlp_forward(params.ctypes.data_as(C.POINTER(C.c_float)), x.ctypes.data_as(C.POINTER(C.c_float)),
+                         len(x), _d(out), int(round_bf16))
+    return out
+
+
+def mlp_loss_grad(params, x, cls, dirs, round_bf16=False):
+    L = _mlp_lib()
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    cls = np.ascontiguousarray(cls, np.int32)
+    dirs = np.ascontiguousarray(dirs, np.float32)
+    g = np.zeros(len(params))
+    loss = L.oracle_mlp_loss_grad(params.ctypes.data_as(C.POINTER(C.c_float)), x.ctypes.data_as(C.POINTER(C.c_float)),
+                                  cls.ctypes.data_as(C.POINTER(C.c_int32)), dirs.ctypes.data_as(C.POINTER(C.c_float)),
+                                  len(x), _d(g), int(round_bf16))
+    return loss, g

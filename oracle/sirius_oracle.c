@@ -326,6 +326,8 @@ void oracle_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->vol_r_max = 50.0;
     o->vol_tau_max = 10.0;
     o->escalate_disk_edge = 0.02;
+    o->escalate_rtol = 1e-8;
+    o->critical_band = 0.5;
     if (precision == SIRIUS_PRECISION_FP32_ESCALATE) {
         o->rtol = 1e-6;
         o->atol = 1e-8;

@@ -55,12 +55,12 @@ int oracle_trace(const SiriusHipCamera* cams, int n_cam, const SiriusHipMetric* 
 
 int oracle_abi_layout(int64_t* out, int n);
 
-/* MLP oracle (FP64): forward and backward of the surrogate network on n samples. Parameter
- * layout matches sirius_hip_mlp_get_params. logits_dir[n*6]. */
+/* MLP oracle (FP64): the surrogate network of sirius_hip.h (same parameter layout, inputs,
+ * outputs and loss). round_bf16 != 0 rounds every GEMM operand to BF16 (round to nearest
+ * even) exactly where the device does; 0 is the plain FP64 network. */
 size_t oracle_mlp_param_count(void);
-void oracle_mlp_forward(const float* params, const float* x, int n, double* out);
-/* Mean loss over the batch and its gradient w.r.t. params (length param_count). Inputs are
- * rounded to bf16 exactly as the device path rounds them when round_bf16 != 0. */
+void oracle_mlp_forward(const float* params, const float* x, int n, double* out, int round_bf16);
+/* Mean loss over the batch and its gradient w.r.t. params (length param_count). */
 double oracle_mlp_loss_grad(const float* params, const float* x, const int32_t* cls,
                             const float* dir, int n, double* grad, int round_bf16);
 

@@ -16,7 +16,7 @@ PRECISION_FP64, PRECISION_FP32_ESCALATE = 0, 1
 EMISSION_NONE, EMISSION_THIN_DISK, EMISSION_VOLUMETRIC = 0, 1, 2
 SCHEDULE_GRID, SCHEDULE_PERSISTENT = 0, 1
 RAY_ESCAPED, RAY_CAPTURED, RAY_DISK, RAY_MAX_STEPS, RAY_SPIRAL, RAY_FAILED = 0, 1, 2, 3, 4, 5
-FLAG_ESCALATED, FLAG_SURROGATE, FLAG_RENORM, FLAG_AUDITED, FLAG_FAR_SIDE = 1, 2, 4, 8, 16
+FLAG_ESCALATED, FLAG_SURROGATE, FLAG_RENORM, FLAG_AUDITED, FLAG_FAR_SIDE, FLAG_CRITICAL = 1, 2, 4, 8, 16, 32
 
 MLP_IN, MLP_HIDDEN, MLP_LAYERS, MLP_OUT = 5, 256, 4, 6
 
@@ -60,7 +60,7 @@ class TraceOpts(C.Structure):
         ("refill_min", C.c_int32), ("reserved0", C.c_int32),
         ("vol_j0", C.c_double), ("vol_h_over_r", C.c_double), ("vol_kappa", C.c_double),
         ("vol_r_min", C.c_double), ("vol_r_max", C.c_double), ("vol_tau_max", C.c_double),
-        ("escalate_disk_edge", C.c_double),
+        ("escalate_disk_edge", C.c_double), ("escalate_rtol", C.c_double), ("critical_band", C.c_double),
     ]
 
 

@@ -64,8 +64,10 @@ def lib():
     L.sirius_hip_mlp_get_params.argtypes = [H, C.c_void_p, C.c_size_t]
     L.sirius_hip_mlp_set_params.argtypes = [H, C.c_void_p, C.c_size_t]
     L.sirius_hip_mlp_forward.argtypes = [H, C.c_void_p, C.c_void_p, i]
-    L.sirius_hip_mlp_loss_grad.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, i, C.c_void_p, C.c_void_p]
-    L.sirius_hip_mlp_train_step.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, i, C.c_float, C.c_void_p]
+    L.sirius_hip_mlp_loss_grad.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, i, C.c_void_p,
+                                           C.POINTER(C.c_float)]
+    L.sirius_hip_mlp_train_step.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, i, C.c_float,
+                                            C.POINTER(C.c_float)]
     _lib = L
     return L
 
@@ -201,6 +203,52 @@ class HipAccelerator:
         self._check(self._L.sirius_hip_critical_distance(self._h, ca, len(cams), ma, len(metrics),
                                                          out.ctypes.data, 0))
         return out
+
+    # ---- surrogate network (device buffers are torch CUDA tensors: device-memory plumbing) ----
+    def mlp_init(self, seed):
+        self._check(self._L.sirius_hip_mlp_init(self._h, seed))
+
+    def mlp_get_params(self):
+        p = np.zeros(self._L.sirius_hip_mlp_param_count(), np.float32)
+        self._check(self._L.sirius_hip_mlp_get_params(self._h, p.ctypes.data, p.size))
+        return p
+
+    def mlp_set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self._check(self._L.sirius_hip_mlp_set_params(self._h, p.ctypes.data, p.size))
+
+    @staticmethod
+    def _dev(t, dtype):
+        import torch
+        assert t.is_cuda and t.dtype == dtype and t.is_contiguous()
+        return t.data_ptr()
+
+    def mlp_forward(self, x):
+        """x: CUDA float32 tensor (n, 5) -> CUDA float32 tensor (n, 6)."""
+        import torch
+        y = torch.empty((x.shape[0], 6), dtype=torch.float32, device=x.device)
+        torch.cuda.synchronize()
+        self._check(self._L.sirius_hip_mlp_forward(self._h, self._dev(x, torch.float32), y.data_ptr(), x.shape[0]))
+        return y
+
+    def mlp_loss_grad(self, x, cls, dirs):
+        import torch
+        g = torch.empty(self._L.sirius_hip_mlp_param_count(), dtype=torch.float32, device=x.device)
+        loss = C.c_float()
+        torch.cuda.synchronize()
+        self._check(self._L.sirius_hip_mlp_loss_grad(self._h, self._dev(x, torch.float32), self._dev(cls, torch.int32),
+                                                     self._dev(dirs, torch.float32), x.shape[0], g.data_ptr(),
+                                                     C.byref(loss)))
+        return loss.value, g
+
+    def mlp_train_step(self, x, cls, dirs, lr):
+        import torch
+        loss = C.c_float()
+        torch.cuda.synchronize()
+        self._check(self._L.sirius_hip_mlp_train_step(self._h, self._dev(x, torch.float32), self._dev(cls, torch.int32),
+                                                      self._dev(dirs, torch.float32), x.shape[0], C.c_float(lr),
+                                                      C.byref(loss)))
+        return loss.value
 
     def autotune(self, cams, metrics, opts):
         cams, metrics = list(cams), list(metrics)

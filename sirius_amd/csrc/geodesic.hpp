@@ -59,7 +59,7 @@ struct TraceParams {
     double vol_j0, vol_h_over_r, vol_kappa, vol_r_min, vol_r_max, vol_tau_max;
     double esc_edge;
     int32_t max_steps, n_bins, emission, sym_order;
-    int32_t n_cam, escalate, pad0, pad1;
+    int32_t n_cam, escalate, gate_flags, pad1;  // gate_flags: the gate wrote per-ray flags
     int64_t n_rays;
 };
 

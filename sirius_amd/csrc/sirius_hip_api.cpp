@@ -41,8 +41,8 @@ int persistent_blocks_per_cu(bool fp64, int family, int emission, bool multi, in
 hipError_t launch_reduce_stats(const DevOut& o, int64_t n, unsigned long long* acc, hipStream_t st);
 hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float* tabs, int64_t n, float* out,
                                     hipStream_t st);
-hipError_t launch_gate(const CamConst* cams, int n_cam, const float* tabs, int64_t n, double band, int32_t* list_a,
-                       int32_t* list_b, uint32_t* counts, hipStream_t st);
+hipError_t launch_gate(const CamConst* cams, int n_cam, const float* tabs, int64_t n, double band, double crit_band,
+                       int32_t* list_a, int32_t* list_b, uint32_t* counts, uint8_t* flags, hipStream_t st);
 hipError_t launch_shade(const DevOut& o, int64_t n, int width, int height, int n_bins, const uint8_t* bg, int bgw,
                         int bgh, float* fb, hipStream_t st);
 
@@ -341,12 +341,15 @@ int prepare_batch(Handle& H, const SiriusHipCamera* cams, int n_cam, const Siriu
     tp.vol_j0 = o.vol_j0; tp.vol_h_over_r = o.vol_h_over_r; tp.vol_kappa = o.vol_kappa;
     tp.vol_r_min = o.vol_r_min; tp.vol_r_max = o.vol_r_max; tp.vol_tau_max = o.vol_tau_max;
     tp.esc_edge = o.escalate_disk_edge;
+    tp.gate_flags = 0;
     tp.max_steps = o.max_steps;
     tp.n_bins = o.n_bins;
     tp.emission = B.family == SIRIUS_METRIC_KERR_SCHILD ? o.emission : SIRIUS_EMISSION_NONE;
     tp.sym_order = o.symplectic_order;
     tp.n_cam = n_cam;
-    tp.escalate = (o.precision == SIRIUS_PRECISION_FP32_ESCALATE && o.escalate_margin > 0) ? 1 : 0;
+    // escalation machinery of the FP32 path: on when any criterion is (a zero criterion never fires)
+    tp.escalate = (o.precision == SIRIUS_PRECISION_FP32_ESCALATE &&
+                   (o.escalate_margin > 0 || o.escalate_band > 0 || o.escalate_disk_edge > 0)) ? 1 : 0;
     tp.n_rays = off;
     if (tp.emission == SIRIUS_EMISSION_VOLUMETRIC && !(o.vol_h_over_r > 0 && o.vol_r_max > o.vol_r_min))
         return H.fail(SIRIUS_HIP_E_INVALID, "volumetric parameters");
@@ -455,34 +458,44 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
             return H.hip_fail(e, "persistent launch");
         if (time_it) (void)hipEventRecord(H.ev1, H.stream);
     } else {
-        // FP32 path. (1) Gate: rays within escalate_band of the critical curve go to list B and
-        // are traced in FP64 on the second stream while (2) the FP32 persistent tracer runs list
-        // A; (3) FP32 rays that still reach a turning point inside the photon region are re-traced
-        // in FP64 from their camera pixel after the bulk. Escalated rays keep the path's step
-        // policy in FP64 arithmetic.
+        // FP32 path. (1) Gate (Kerr-Schild): every ray's distance from the critical curve; rays
+        // within critical_band are flagged, rays within escalate_band go to list B and are traced
+        // in FP64 on the second stream while (2) the FP32 persistent tracer runs list A; (3) FP32
+        // rays that meet an in-flight criterion (inner-edge disk crossing, turning point inside
+        // the photon region, warp wall) are re-traced in FP64 from their camera pixel after the
+        // bulk. Escalated rays keep the path's step caps with the escalation tolerance.
         int32_t* esc = (int32_t*)H.scratch("esc_list", 4 * (size_t)std::max<int64_t>(n_work, 1), &rc);
         if (rc) return rc;
         const int32_t* list_a = list_dev;
         const uint32_t* count_a = list_count_dev;
         int64_t cap_a = n_work;
         int32_t* list_b = nullptr;
-        bool gate = !list_dev && B.family == SIRIUS_METRIC_KERR_SCHILD && B.tp.escalate && o.escalate_band > 0;
+        bool gate = !list_dev && B.family == SIRIUS_METRIC_KERR_SCHILD && B.tp.escalate &&
+                    (o.escalate_band > 0 || o.critical_band > 0);
+        bool band = gate && o.escalate_band > 0;
+        TraceParams t32 = B.tp;
+        t32.gate_flags = gate ? 1 : 0;
         if (gate) {
             float* tabs = upload_critical_tables(H, B, &rc);
             int32_t* la = (int32_t*)H.scratch("gate_a", 4 * (size_t)B.n, &rc);
             list_b = (int32_t*)H.scratch("gate_b", 4 * (size_t)B.n, &rc);
             if (rc) return rc;
             uint32_t* counts = work + 8;
-            if ((e = launch_gate(B.cams_dev, (int)B.cc.size(), tabs, B.n, o.escalate_band, la, list_b, counts,
-                                 H.stream)) != hipSuccess)
+            if ((e = launch_gate(B.cams_dev, (int)B.cc.size(), tabs, B.n, std::max(o.escalate_band, 0.0),
+                                 std::max(o.critical_band, 0.0), la, list_b, counts, dout.flags, H.stream)) != hipSuccess)
                 return H.hip_fail(e, "gate launch");
             list_a = la;
             count_a = counts;
             cap_a = B.n;
         }
-        TraceParams t64 = B.tp;
+        // escalated rays: FP64 arithmetic, the path's step caps, the escalation tolerance
+        TraceParams t64 = t32;
         t64.escalate = 0;
-        if (gate) {
+        if (o.escalate_rtol > 0) {
+            t64.rtol = o.escalate_rtol;
+            t64.atol = o.atol * (o.escalate_rtol / o.rtol);
+        }
+        if (band) {
             // FP64 band on the second stream, small blocks so the long rays spread over the chip.
             // The band's length stays on the device (no host round trip): the persistent grid is
             // sized for the whole chip and waves with nothing to trace retire at once.
@@ -502,7 +515,7 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
         int grid = std::max(1, bpc * H.cu_count);
         int refill = o.refill_min > 0 ? std::min(o.refill_min, 64) : 16;
         if (time_it) (void)hipEventRecord(H.ev0, H.stream);
-        if ((e = launch_trace_persistent(false, B.family, B.tp.emission, B.tp.n_cam > 1, B.cams_dev, B.tp, dout, list_a,
+        if ((e = launch_trace_persistent(false, B.family, B.tp.emission, B.tp.n_cam > 1, B.cams_dev, t32, dout, list_a,
                                          count_a, cap_a, work, work + 1, esc, (unsigned long long*)(work + 2), grid,
                                          block, refill, 0, H.stream)) != hipSuccess)
             return H.hip_fail(e, "persistent launch");
@@ -515,8 +528,8 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
                                              SIRIUS_FLAG_ESCALATED, H.stream)) != hipSuccess)
                 return H.hip_fail(e, "escalation launch");
         }
-        if (gate) (void)hipStreamWaitEvent(H.stream, H.ev_band1, 0);
-        H.last_band = gate;
+        if (band) (void)hipStreamWaitEvent(H.stream, H.ev_band1, 0);
+        H.last_band = band;
     }
     if (time_it) {
         if (!persistent) (void)hipEventRecord(H.ev1, H.stream);
@@ -707,6 +720,8 @@ void sirius_hip_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->vol_r_max = 50.0;
     o->vol_tau_max = 10.0;
     o->escalate_disk_edge = 0.02;
+    o->escalate_rtol = 1e-8;
+    o->critical_band = 0.5;
     if (precision == SIRIUS_PRECISION_FP32_ESCALATE) {
         o->rtol = 1e-6;
         o->atol = 1e-8;

@@ -1,30 +1,349 @@
-// surrogate.cpp — host orchestration of the MLP far-field surrogate (placeholder until the
-// network kernels land: the entry points report SIRIUS_HIP_E_UNSUPPORTED).
+// surrogate.cpp — host orchestration of the MLP far-field surrogate: parameters, forward,
+// loss and gradient, Adam training steps (the network contract is in sirius_hip.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "../../include/sirius_hip.h"
+#include "mlp.hpp"
 #include "sirius_internal.hpp"
 
 namespace sirius {
 
-void mlp_release(Handle&) {}
+size_t mlp_param_offset_w(int l) {
+    size_t o = 0;
+    for (int i = 0; i < l; ++i) o += (size_t)MLP_OUTD[i] * MLP_IN[i] + MLP_OUTD[i];
+    return o;
+}
+size_t mlp_param_offset_b(int l) { return mlp_param_offset_w(l) + (size_t)MLP_OUTD[l] * MLP_IN[l]; }
+size_t mlp_param_count() { return mlp_param_offset_w(MLP_NL); }
+
+void mlp_release(Handle& H) {
+    if (!H.mlp) return;
+    MlpState& S = *H.mlp;
+    for (void* p : {(void*)S.params, (void*)S.adam_m, (void*)S.adam_v, (void*)S.grad})
+        if (p) (void)hipFree(p);
+    for (int l = 0; l < MLP_NL; ++l) {
+        if (S.w[l]) (void)hipFree(S.w[l]);
+        if (S.b[l]) (void)hipFree(S.b[l]);
+    }
+    delete H.mlp;
+    H.mlp = nullptr;
+}
+
+static int mlp_alloc(Handle& H) {
+    if (H.mlp) return SIRIUS_HIP_OK;
+    int rc = H.ensure_device();
+    if (rc) return rc;
+    H.mlp = new MlpState();
+    MlpState& S = *H.mlp;
+    size_t P = mlp_param_count();
+    hipError_t e;
+    for (float** p : {&S.params, &S.adam_m, &S.adam_v, &S.grad})
+        if ((e = hipMalloc((void**)p, P * sizeof(float))) != hipSuccess) return H.hip_fail(e, "mlp alloc");
+    for (int l = 0; l < MLP_NL; ++l) {
+        if ((e = hipMalloc((void**)&S.w[l], (size_t)MLP_NP[l] * MLP_KP[l] * 2)) != hipSuccess ||
+            (e = hipMalloc((void**)&S.b[l], (size_t)MLP_NP[l] * 4)) != hipSuccess)
+            return H.hip_fail(e, "mlp alloc");
+    }
+    return SIRIUS_HIP_OK;
+}
+
+static int mlp_reset_adam(Handle& H) {
+    MlpState& S = *H.mlp;
+    size_t P = mlp_param_count();
+    hipError_t e;
+    if ((e = hipMemsetAsync(S.adam_m, 0, P * 4, H.stream)) != hipSuccess ||
+        (e = hipMemsetAsync(S.adam_v, 0, P * 4, H.stream)) != hipSuccess)
+        return H.hip_fail(e, "adam reset");
+    S.step = 0;
+    if ((e = launch_pack(S.params, S.w, S.b, H.stream)) != hipSuccess) return H.hip_fail(e, "pack");
+    if ((e = hipStreamSynchronize(H.stream)) != hipSuccess) return H.hip_fail(e, "synchronize");
+    S.ready = true;
+    return SIRIUS_HIP_OK;
+}
+
+static inline int64_t pad64(int64_t n) { return (n + 63) / 64 * 64; }
+
+// Workspace of a training batch of n_pad rows
+struct MlpWork {
+    uint16_t* h[MLP_NL];  // h[0]: [n_pad][64], h[1..4]: [n_pad][256]
+    float* z[MLP_NL - 1];
+    float* y64;
+    float* gyf;
+    uint16_t* gyb;
+    float* gzf[2];
+    uint16_t* gzb[2];
+    float* part;
+    float* colpart;
+    float* loss_i;
+    float* loss_part;
+    float* loss;
+};
+
+constexpr int COLSPLIT = 64;
+
+static int mlp_workspace(Handle& H, int64_t n_pad, bool train, MlpWork& W) {
+    int rc = SIRIUS_HIP_OK;
+    const size_t r = (size_t)n_pad;
+    W.h[0] = (uint16_t*)H.scratch("mlp_h0", r * MLP_X0 * 2, &rc);
+    const char* hn[4] = {"mlp_h1", "mlp_h2", "mlp_h3", "mlp_h4"};
+    const char* zn[4] = {"mlp_z0", "mlp_z1", "mlp_z2", "mlp_z3"};
+    for (int l = 1; l < MLP_NL; ++l)
+        W.h[l] = (uint16_t*)H.scratch(train ? hn[l - 1] : hn[(l - 1) & 1], r * MLP_H * 2, &rc);
+    for (int l = 0; l < MLP_NL - 1; ++l) W.z[l] = train ? (float*)H.scratch(zn[l], r * MLP_H * 4, &rc) : nullptr;
+    W.y64 = (float*)H.scratch("mlp_y64", r * MLP_N4 * 4, &rc);
+    if (train) {
+        W.gyf = (float*)H.scratch("mlp_gyf", r * MLP_N4 * 4, &rc);
+        W.gyb = (uint16_t*)H.scratch("mlp_gyb", r * MLP_N4 * 2, &rc);
+        W.gzf[0] = (float*)H.scratch("mlp_gzf0", r * MLP_H * 4, &rc);
+        W.gzf[1] = (float*)H.scratch("mlp_gzf1", r * MLP_H * 4, &rc);
+        W.gzb[0] = (uint16_t*)H.scratch("mlp_gzb0", r * MLP_H * 2, &rc);
+        W.gzb[1] = (uint16_t*)H.scratch("mlp_gzb1", r * MLP_H * 2, &rc);
+        W.part = (float*)H.scratch("mlp_part", (size_t)64 * MLP_H * MLP_H * 4, &rc);
+        W.colpart = (float*)H.scratch("mlp_colpart", (size_t)COLSPLIT * MLP_H * 4, &rc);
+        W.loss_i = (float*)H.scratch("mlp_loss_i", r * 4, &rc);
+        W.loss_part = (float*)H.scratch("mlp_loss_part", 256 * 4, &rc);
+        W.loss = (float*)H.scratch("mlp_loss", 256, &rc);
+    }
+    return rc;
+}
+
+static GemmArgs gemm(const uint16_t* A, int64_t sai, int64_t sak, const uint16_t* B, int64_t sbk, int64_t sbj, int M,
+                     int N, int K) {
+    GemmArgs g;
+    std::memset(&g, 0, sizeof(g));
+    g.A = A; g.sai = sai; g.sak = sak;
+    g.B = B; g.sbk = sbk; g.sbj = sbj;
+    g.M = M; g.N = N; g.K = K; g.kchunk = K;
+    return g;
+}
+
+// Forward of n_pad rows already in W.h[0]; keeps z (training) or only the last activations.
+static hipError_t mlp_forward_rows(MlpState& S, MlpWork& W, int64_t n_pad, hipStream_t st) {
+    hipError_t e;
+    for (int l = 0; l < MLP_NL - 1; ++l) {
+        const int64_t lda = l == 0 ? MLP_X0 : MLP_H;
+        GemmArgs g = gemm(W.h[l], lda, 1, S.w[l], 1, MLP_KP[l], (int)n_pad, MLP_H, MLP_KP[l]);
+        g.bias = S.b[l];
+        g.zout = W.z[l];
+        g.hout = W.h[l + 1];
+        if ((e = launch_gemm(EPI_HIDDEN, g, 1, st)) != hipSuccess) return e;
+    }
+    GemmArgs g = gemm(W.h[4], MLP_H, 1, S.w[4], 1, MLP_H, (int)n_pad, MLP_N4, MLP_H);
+    g.bias = S.b[4];
+    g.out = W.y64;
+    return launch_gemm(EPI_OUT, g, 1, st);
+}
+
+// Split of the sample dimension for the weight-gradient GEMMs
+static void split_k(int64_t n_pad, int* nsplit, int* kchunk) {
+    int s = (int)std::min<int64_t>(64, std::max<int64_t>(1, n_pad / 1024));
+    int64_t c = (n_pad + s - 1) / s;
+    c = (c + 31) / 32 * 32;
+    *kchunk = (int)c;
+    *nsplit = (int)((n_pad + c - 1) / c);
+}
+
+// Loss and gradient (into grad, public layout) of a batch; *loss_host gets the mean loss.
+static int mlp_loss_grad_dev(Handle& H, const float* x, const int32_t* cls, const float* dir, int n, float* grad,
+                             float* loss_host) {
+    MlpState& S = *H.mlp;
+    const int64_t n_pad = pad64(n);
+    MlpWork W;
+    int rc = mlp_workspace(H, n_pad, true, W);
+    if (rc) return rc;
+    hipStream_t st = H.stream;
+    hipError_t e;
+    if ((e = launch_mlp_input(x, n, n_pad, W.h[0], st)) != hipSuccess) return H.hip_fail(e, "mlp input");
+    if ((e = mlp_forward_rows(S, W, n_pad, st)) != hipSuccess) return H.hip_fail(e, "mlp forward");
+    if ((e = launch_mlp_loss(W.y64, cls, dir, n, n_pad, W.gyf, W.gyb, W.loss_i, st)) != hipSuccess)
+        return H.hip_fail(e, "mlp loss");
+    int nsplit, kchunk;
+    split_k(n_pad, &nsplit, &kchunk);
+    // layer 4: dW4 = gy^T h4, db4 = colsum(gy), gz3 = (gy W4) * gelu'(z3)
+    {
+        GemmArgs g = gemm(W.gyb, 1, MLP_N4, W.h[4], MLP_H, 1, MLP_N4, MLP_H, (int)n_pad);
+        g.kchunk = kchunk;
+        g.out = W.part;
+        if ((e = launch_gemm(EPI_PART, g, nsplit, st)) != hipSuccess) return H.hip_fail(e, "dW4");
+        if ((e = launch_reduce_partials(W.part, nsplit, MLP_N4, MLP_H, SIRIUS_MLP_OUT, MLP_H,
+                                        grad + mlp_param_offset_w(4), st)) != hipSuccess)
+            return H.hip_fail(e, "dW4 reduce");
+        if ((e = launch_colsum(W.gyf, n_pad, SIRIUS_MLP_OUT, MLP_N4, W.colpart, COLSPLIT, grad + mlp_param_offset_b(4),
+                               st)) != hipSuccess)
+            return H.hip_fail(e, "db4");
+        GemmArgs d = gemm(W.gyb, MLP_N4, 1, S.w[4], MLP_H, 1, (int)n_pad, MLP_H, MLP_N4);
+        d.zin = W.z[3];
+        d.gzf = W.gzf[0];
+        d.gzb = W.gzb[0];
+        if ((e = launch_gemm(EPI_BWD, d, 1, st)) != hipSuccess) return H.hip_fail(e, "dX4");
+    }
+    int cur = 0;
+    for (int l = MLP_NL - 2; l >= 0; --l) {
+        const int ncols = l == 0 ? MLP_X0 : MLP_H;  // width of the stored layer input rows
+        GemmArgs g = gemm(W.gzb[cur], 1, MLP_H, W.h[l], ncols, 1, MLP_H, ncols, (int)n_pad);
+        g.kchunk = kchunk;
+        g.out = W.part;
+        if ((e = launch_gemm(EPI_PART, g, nsplit, st)) != hipSuccess) return H.hip_fail(e, "dW");
+        if ((e = launch_reduce_partials(W.part, nsplit, MLP_H, ncols, MLP_H, MLP_IN[l], grad + mlp_param_offset_w(l),
+                                        st)) != hipSuccess)
+            return H.hip_fail(e, "dW reduce");
+        if ((e = launch_colsum(W.gzf[cur], n_pad, MLP_H, MLP_H, W.colpart, COLSPLIT, grad + mlp_param_offset_b(l),
+                               st)) != hipSuccess)
+            return H.hip_fail(e, "db");
+        if (l == 0) break;
+        GemmArgs d = gemm(W.gzb[cur], MLP_H, 1, S.w[l], MLP_H, 1, (int)n_pad, MLP_H, MLP_H);
+        d.zin = W.z[l - 1];
+        d.gzf = W.gzf[cur ^ 1];
+        d.gzb = W.gzb[cur ^ 1];
+        if ((e = launch_gemm(EPI_BWD, d, 1, st)) != hipSuccess) return H.hip_fail(e, "dX");
+        cur ^= 1;
+    }
+    if ((e = launch_sum(W.loss_i, n_pad, W.loss_part, W.loss, st)) != hipSuccess) return H.hip_fail(e, "loss sum");
+    float total = 0.f;
+    if ((e = hipMemcpyAsync(&total, W.loss, 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (e = hipStreamSynchronize(st)) != hipSuccess)
+        return H.hip_fail(e, "loss copy");
+    if (loss_host) *loss_host = total / (float)n;
+    return SIRIUS_HIP_OK;
+}
+
+// Forward pass of n samples x[n*5] -> y[n*6] (device), in chunks.
+int mlp_forward_dev(Handle& H, const float* x, float* y, int64_t n) {
+    MlpState& S = *H.mlp;
+    const int64_t CH = 1 << 18;
+    MlpWork W;
+    int rc = mlp_workspace(H, pad64(std::min(n, CH)), false, W);
+    if (rc) return rc;
+    hipError_t e;
+    for (int64_t s = 0; s < n; s += CH) {
+        int64_t m = std::min(CH, n - s), m_pad = pad64(m);
+        if ((e = launch_mlp_input(x + s * SIRIUS_MLP_IN, m, m_pad, W.h[0], H.stream)) != hipSuccess ||
+            (e = mlp_forward_rows(S, W, m_pad, H.stream)) != hipSuccess ||
+            (e = launch_mlp_extract(W.y64, m, y + s * SIRIUS_MLP_OUT, H.stream)) != hipSuccess)
+            return H.hip_fail(e, "mlp forward");
+    }
+    return SIRIUS_HIP_OK;
+}
 
 int surrogate_trace(Handle& H, Batch&, const SiriusHipTraceOpts&, DevOut&, SiriusHipStats*) {
-    return H.fail(SIRIUS_HIP_E_UNSUPPORTED, "surrogate not built yet");
+    return H.fail(SIRIUS_HIP_E_UNSUPPORTED, "surrogate tracing is not available yet");
+}
+
+// splitmix64 stream for the initial weights
+static uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
 }
 
 }  // namespace sirius
 
+using namespace sirius;
+
+#define MLP_GUARD_BEGIN try {
+#define MLP_GUARD_END(h)                                                                   \
+    }                                                                                      \
+    catch (const std::bad_alloc&) { return (h)->fail(SIRIUS_HIP_E_NOMEM, "host allocation"); } \
+    catch (...) { return (h)->fail(SIRIUS_HIP_E_DEVICE, "unexpected exception"); }
+
 extern "C" {
-size_t sirius_hip_mlp_param_count(void) { return 0; }
-int sirius_hip_mlp_init(SiriusHipHandle, uint64_t) { return SIRIUS_HIP_E_UNSUPPORTED; }
-int sirius_hip_mlp_get_params(SiriusHipHandle, float*, size_t) { return SIRIUS_HIP_E_UNSUPPORTED; }
-int sirius_hip_mlp_set_params(SiriusHipHandle, const float*, size_t) { return SIRIUS_HIP_E_UNSUPPORTED; }
-int sirius_hip_mlp_forward(SiriusHipHandle, const float*, float*, int) { return SIRIUS_HIP_E_UNSUPPORTED; }
-int sirius_hip_mlp_loss_grad(SiriusHipHandle, const float*, const int32_t*, const float*, int, float*, float*) {
-    return SIRIUS_HIP_E_UNSUPPORTED;
+
+size_t sirius_hip_mlp_param_count(void) { return mlp_param_count(); }
+
+int sirius_hip_mlp_init(SiriusHipHandle hh, uint64_t seed) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    MLP_GUARD_BEGIN
+    int rc = mlp_alloc(*H);
+    if (rc) return rc;
+    std::vector<float> p(mlp_param_count(), 0.f);
+    uint64_t s = seed;
+    for (int l = 0; l < MLP_NL; ++l) {
+        const double lim = std::sqrt(6.0 / MLP_IN[l]);
+        float* w = p.data() + mlp_param_offset_w(l);
+        for (int i = 0; i < MLP_OUTD[l] * MLP_IN[l]; ++i) {
+            double u = (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0);
+            w[i] = (float)((2.0 * u - 1.0) * lim);
+        }
+    }
+    hipError_t e;
+    if ((e = hipMemcpy(H->mlp->params, p.data(), p.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        return H->hip_fail(e, "params upload");
+    return mlp_reset_adam(*H);
+    MLP_GUARD_END(H)
 }
-int sirius_hip_mlp_train_step(SiriusHipHandle, const float*, const int32_t*, const float*, int, float, float*) {
-    return SIRIUS_HIP_E_UNSUPPORTED;
+
+int sirius_hip_mlp_get_params(SiriusHipHandle hh, float* host_params, size_t n) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    if (!H->mlp || !H->mlp->ready) return H->fail(SIRIUS_HIP_E_STATE, "network not initialised");
+    if (!host_params || n != mlp_param_count()) return H->fail(SIRIUS_HIP_E_INVALID, "parameter count");
+    hipError_t e = hipMemcpy(host_params, H->mlp->params, n * 4, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? SIRIUS_HIP_OK : H->hip_fail(e, "params download");
 }
+
+int sirius_hip_mlp_set_params(SiriusHipHandle hh, const float* host_params, size_t n) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    MLP_GUARD_BEGIN
+    if (!host_params || n != mlp_param_count()) return H->fail(SIRIUS_HIP_E_INVALID, "parameter count");
+    int rc = mlp_alloc(*H);
+    if (rc) return rc;
+    hipError_t e = hipMemcpy(H->mlp->params, host_params, n * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return H->hip_fail(e, "params upload");
+    return mlp_reset_adam(*H);
+    MLP_GUARD_END(H)
 }
+
+int sirius_hip_mlp_forward(SiriusHipHandle hh, const float* x, float* y, int n) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    MLP_GUARD_BEGIN
+    if (!H->mlp || !H->mlp->ready) return H->fail(SIRIUS_HIP_E_STATE, "network not initialised");
+    if (!x || !y || n <= 0) return H->fail(SIRIUS_HIP_E_INVALID, "x, y and n > 0 are required");
+    int rc = mlp_forward_dev(*H, x, y, n);
+    if (rc) return rc;
+    hipError_t e = hipStreamSynchronize(H->stream);
+    return e == hipSuccess ? SIRIUS_HIP_OK : H->hip_fail(e, "synchronize");
+    MLP_GUARD_END(H)
+}
+
+int sirius_hip_mlp_loss_grad(SiriusHipHandle hh, const float* x, const int32_t* cls, const float* dir, int n,
+                             float* grad, float* loss) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    MLP_GUARD_BEGIN
+    if (!H->mlp || !H->mlp->ready) return H->fail(SIRIUS_HIP_E_STATE, "network not initialised");
+    if (!x || !cls || !dir || !grad || n <= 0) return H->fail(SIRIUS_HIP_E_INVALID, "x, cls, dir, grad, n > 0");
+    return mlp_loss_grad_dev(*H, x, cls, dir, n, grad, loss);
+    MLP_GUARD_END(H)
+}
+
+int sirius_hip_mlp_train_step(SiriusHipHandle hh, const float* x, const int32_t* cls, const float* dir, int n,
+                              float lr, float* loss) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    MLP_GUARD_BEGIN
+    if (!H->mlp || !H->mlp->ready) return H->fail(SIRIUS_HIP_E_STATE, "network not initialised");
+    if (!x || !cls || !dir || n <= 0 || !(lr > 0)) return H->fail(SIRIUS_HIP_E_INVALID, "x, cls, dir, n > 0, lr > 0");
+    MlpState& S = *H->mlp;
+    int rc = mlp_loss_grad_dev(*H, x, cls, dir, n, S.grad, loss);
+    if (rc) return rc;
+    S.step++;
+    const float c1 = 1.f - std::pow(0.9f, (float)S.step), c2 = 1.f - std::pow(0.999f, (float)S.step);
+    hipError_t e;
+    if ((e = launch_adam(S.params, S.grad, S.adam_m, S.adam_v, (int64_t)mlp_param_count(), lr, c1, c2, H->stream)) !=
+            hipSuccess ||
+        (e = launch_pack(S.params, S.w, S.b, H->stream)) != hipSuccess)
+        return H->hip_fail(e, "adam");
+    return SIRIUS_HIP_OK;
+    MLP_GUARD_END(H)
+}
+
+}  // extern "C"

@@ -97,7 +97,8 @@ SIRIUS_HD __forceinline__ void ray_start(Ray<T, FAM>& R, const CamConst* cams, i
     R.lam = T(0);
     R.tau = T(0);
     R.vr_prev = T(-1);
-    R.attempts = R.rejected = R.landings = R.flags = 0;
+    R.attempts = R.rejected = R.landings = 0;
+    R.flags = tp.gate_flags ? (o.flags[id] & SIRIUS_FLAG_CRITICAL) : 0;
     if constexpr (EMIT == SIRIUS_EMISSION_VOLUMETRIC)
         for (int k = 0; k < tp.n_bins; ++k) o.intensity[(int64_t)k * o.n_rays + id] = 0.f;
 }
@@ -328,8 +329,10 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
             T rc = kerr_r(s, xc[1], xc[2], xc[3]);
             if constexpr (fp32) {
                 // near the inner edge the emission falls steeply to zero: an FP32 position error
-                // there is a large relative intensity error, so such rays go to FP64
-                if (tp.escalate && fabs(rc - T(c.r_in)) < T(tp.esc_edge * c.r_in)) return ST_ESCALATE;
+                // there is a large relative intensity error, so such rays go to FP64 (rays in the
+                // critical band, which the accuracy contract exempts, stay FP32)
+                if (tp.escalate && !(R.flags & SIRIUS_FLAG_CRITICAL) && fabs(rc - T(c.r_in)) < T(tp.esc_edge * c.r_in))
+                    return ST_ESCALATE;
             }
             if (rc >= T(c.r_in) && rc <= T(c.r_out)) ev = SIRIUS_RAY_DISK;
         }
@@ -374,6 +377,10 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
     for (int i = 0; i < 4; ++i) { R.x[i] = x1[i]; R.p[i] = p1[i]; }
     R.d = d1;
     if constexpr (fp32) {
+        // escalation of a warp ray entering the bubble wall (|r_s - R| < 3 / sigma, or inside):
+        // rays crossing the wall are deflected by up to ~2 rad and their outcome is sensitive
+        // far beyond what the FP32 path's tolerance controls
+        if (FAM == 2 && tp.escalate && r1 < T(c.wR + 3.0 / c.wsig)) return ST_ESCALATE;
         // escalation: a radial turning point inside the photon region
         if (FAM == 0 && tp.escalate) {
             T vr = x1[1] * d1.k[1] + x1[2] * d1.k[2] + x1[3] * d1.k[3];

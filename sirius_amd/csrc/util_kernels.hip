@@ -89,35 +89,37 @@ hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float
 }
 
 // Gate of the FP32 path: rays within `band` of the critical curve go to list B (traced in FP64
-// from the start, on a second stream), all others to list A (FP32 persistent tracer). A block
+// from the start, on a second stream), all others to list A (FP32 persistent tracer); rays
+// within `crit_band` get SIRIUS_FLAG_CRITICAL in flags[] (every ray's flag byte is written). A block
 // of 256 threads classifies a contiguous chunk of GATE_RAYS rays and reserves its slots with
 // one atomic per list (device-scope atomics on one address serialise, so one per wave is too
 // many at 1080p); within a chunk the lists keep ray order. counts[0] = |A|, counts[1] = |B|.
 constexpr int GATE_PASSES = 8;
 constexpr int GATE_RAYS = 256 * GATE_PASSES;
 
-__device__ __forceinline__ bool gate_in_band(const CamConst* __restrict__ cams, int n_cam,
-                                             const float* __restrict__ tabs, int64_t id, float band) {
+// |distance| of ray id from its camera's critical curve (M); +inf for other spacetimes
+__device__ __forceinline__ float gate_distance(const CamConst* __restrict__ cams, int n_cam,
+                                              const float* __restrict__ tabs, int64_t id) {
     int lo = 0, hi = n_cam - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
         if (cams[mid].offset <= id) lo = mid; else hi = mid - 1;
     }
     const CamConst& c = cams[lo];
-    if (c.family != 0) return false;
+    if (c.family != 0) return INFINITY;
     // FP32 is ample for band membership (alpha, beta ~ 10 M, band ~ 0.1 M)
     int64_t pix = id - c.offset;
     Spacetime<float> s = make_spacetime<float>(c);
     float x[4], p[4];
     init_state<float>(c, s, (int)(pix % c.width), (int)(pix / c.width), x, p);
-    float d = crit_distance<float>(tabs + (int64_t)lo * (SIRIUS_CC_TABLE + 1), s, x, p);
-    return fabsf(d) < band;
+    return fabsf(crit_distance<float>(tabs + (int64_t)lo * (SIRIUS_CC_TABLE + 1), s, x, p));
 }
 
 __global__ __launch_bounds__(256) void k_gate(const CamConst* __restrict__ cams, int n_cam,
                                               const float* __restrict__ tabs, int64_t n, float band,
-                                              int32_t* __restrict__ list_a, int32_t* __restrict__ list_b,
-                                              uint32_t* __restrict__ counts) {
+                                              float crit_band, int32_t* __restrict__ list_a,
+                                              int32_t* __restrict__ list_b, uint32_t* __restrict__ counts,
+                                              uint8_t* __restrict__ flags) {
     __shared__ uint32_t off_a[GATE_PASSES][4], off_b[GATE_PASSES][4];
     __shared__ uint32_t base[2];
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
@@ -128,7 +130,9 @@ __global__ __launch_bounds__(256) void k_gate(const CamConst* __restrict__ cams,
     for (int k = 0; k < GATE_PASSES; ++k) {
         int64_t id = chunk + k * 256 + threadIdx.x;
         bool valid = id < n;
-        bool to_b = valid && band > 0.f && gate_in_band(cams, n_cam, tabs, id, band);
+        float d = valid ? gate_distance(cams, n_cam, tabs, id) : INFINITY;
+        bool to_b = valid && d < band;
+        if (valid) flags[id] = d < crit_band ? (uint8_t)SIRIUS_FLAG_CRITICAL : (uint8_t)0;
         in_b |= (uint32_t)to_b << k;
         unsigned long long ma = __ballot(valid && !to_b), mb = __ballot(to_b);
         if (lane == 0) {
@@ -162,10 +166,10 @@ __global__ __launch_bounds__(256) void k_gate(const CamConst* __restrict__ cams,
     }
 }
 
-hipError_t launch_gate(const CamConst* cams, int n_cam, const float* tabs, int64_t n, double band, int32_t* list_a,
-                       int32_t* list_b, uint32_t* counts, hipStream_t st) {
+hipError_t launch_gate(const CamConst* cams, int n_cam, const float* tabs, int64_t n, double band, double crit_band,
+                       int32_t* list_a, int32_t* list_b, uint32_t* counts, uint8_t* flags, hipStream_t st) {
     hipLaunchKernelGGL(k_gate, dim3((unsigned)((n + GATE_RAYS - 1) / GATE_RAYS)), dim3(256), 0, st, cams, n_cam, tabs,
-                       n, (float)band, list_a, list_b, counts);
+                       n, (float)band, (float)crit_band, list_a, list_b, counts, flags);
     return hipGetLastError();
 }
 

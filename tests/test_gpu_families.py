@@ -1,0 +1,186 @@
+"""GPU parity of the remaining spacetimes, integrators and emission models against the FP64
+oracle, through the C-ABI.
+
+Bars: the plain FP64 kernels equal the oracle's classes and agree to 1e-6 (rad for escape
+directions, relative for intensities and optical depths); the FP32 path meets the trace
+contract's bar (1e-4 rad, 1e-3 relative intensity against the pixel's brightest bin, floored
+at 1e-2 of the median pixel) outside 0.5 M of the critical curve, where Kerr-Schild rays are
+exempt.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from sirius_amd import abi
+from workloads import random_spacetime_batch, warp_scene, wormhole_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _angle(u, v):
+    c = (u * v).sum(1) / np.linalg.norm(u, axis=1) / np.linalg.norm(v, axis=1)
+    return np.arccos(np.clip(c, -1, 1))
+
+
+def _intensity_rel(got, ref, sel):
+    Ir, Ig = ref.intensity[:, sel], got.intensity[:, sel]
+    if Ir.size == 0:
+        return 0.0
+    scale = np.median(Ir.max(0))
+    return float((np.abs(Ig - Ir).max(0) / np.maximum(Ir.max(0), 1e-2 * scale)).max())
+
+
+def _check(got, ref, mask, ang_bar, rel_bar, flags=True):
+    assert np.array_equal(got.cls[mask], ref.cls[mask])
+    if flags:
+        far = abi.FLAG_FAR_SIDE
+        assert np.array_equal(got.flags[mask] & far, ref.flags[mask] & far)
+    esc = mask & (ref.cls == abi.RAY_ESCAPED)
+    ang = _angle(got.direction()[esc], ref.direction()[esc]).max() if esc.any() else 0.0
+    rel = _intensity_rel(got, ref, mask & (ref.cls == abi.RAY_DISK))
+    assert ang < ang_bar and rel < rel_bar, (ang, rel)
+    return ang, rel
+
+
+@pytest.mark.parametrize("scene", ["wormhole", "warp"])
+def test_wormhole_and_warp_match_oracle(acc, scene):
+    cam, m = (wormhole_scene if scene == "wormhole" else warp_scene)(96)
+    o64 = O.default_opts(abi.PRECISION_FP64)
+    ref, _ = O.trace([cam], [m], o64)
+    everything = np.ones(ref.cls.shape, bool)
+    print(f"[case] {scene} 96x96 fp64", flush=True)
+    got, _ = acc.trace_batch([cam], [m], o64)
+    _check(got, ref, everything, 1e-6, 1e-6)
+    assert np.allclose(got.lam, ref.lam, rtol=1e-6)
+    print(f"[case] {scene} 96x96 fp32", flush=True)
+    got32, st = acc.trace_batch([cam], [m], O.default_opts(abi.PRECISION_FP32_ESCALATE))
+    _check(got32, ref, everything, 1e-4, 1e-3)
+    esc = (got32.flags & abi.FLAG_ESCALATED) != 0
+    if scene == "warp":
+        # rays entering the bubble wall are re-traced in FP64; the rest stay FP32
+        assert 0 < esc.sum() < 0.5 * esc.size and st["rays_escalated"] == esc.sum()
+    else:
+        assert not esc.any()
+    if scene == "wormhole":
+        far = (ref.flags & abi.FLAG_FAR_SIDE) != 0
+        assert 0 < far.sum() < far.size  # both sides of the throat are seen
+
+
+def test_symplectic_matches_oracle_and_reports_null_drift(acc):
+    """Fixed-step Tao + Yoshida-4 (h = 0.01 M) on Kerr a = 0.9: the device and the oracle run
+    the same fixed-step arithmetic, so classes, directions and null drift agree closely."""
+    m = abi.metric_ks(1.0, 0.9)
+    cam = abi.camera(30.0, 75.0, 30.0, 32, 18, disk=(0.0, 30.0))
+    o = O.default_opts(abi.PRECISION_FP64)
+    o.integrator = abi.INTEGRATOR_SYMPLECTIC
+    o.symplectic_h = 0.01
+    o.r_escape = 60.0
+    o.max_steps = 50000
+    ref, _ = O.trace([cam], [m], o)
+    print("[case] symplectic fp64 32x18", flush=True)
+    got, st = acc.trace_batch([cam], [m], o)
+    assert np.array_equal(got.cls, ref.cls)
+    esc = ref.cls == abi.RAY_ESCAPED
+    assert _angle(got.direction()[esc], ref.direction()[esc]).max() < 1e-6
+    dk = ref.cls == abi.RAY_DISK
+    assert np.allclose(got.intensity[:, dk], ref.intensity[:, dk], rtol=1e-6, atol=1e-12)
+    assert np.array_equal(got.steps, ref.steps)
+    # the null constraint drift is reported per ray, and matches the oracle's
+    live = esc | dk
+    assert np.allclose(got.null_drift[live], ref.null_drift[live], rtol=1e-3, atol=1e-12)
+    assert st["null_drift_max"] > 0 and np.median(got.null_drift[live]) < 1e-8
+    # the same rays by DP45: the symplectic result agrees to its step error
+    od = O.default_opts(abi.PRECISION_FP64)
+    od.r_escape = 60.0
+    dp, _ = acc.trace_batch([cam], [m], od)
+    both = esc & (dp.cls == abi.RAY_ESCAPED)
+    assert np.percentile(_angle(got.direction()[both], dp.direction()[both]), 90) < 1e-4
+
+
+@pytest.mark.parametrize("prec", [abi.PRECISION_FP64, abi.PRECISION_FP32_ESCALATE])
+def test_volumetric_emission_matches_oracle(acc, prec):
+    m = abi.metric_ks(1.0, 0.7, 0.2, 5e-5)
+    cam = abi.camera(40.0, 65.0, 30.0, 64, 48)
+    o = O.default_opts(prec)
+    o.emission = abi.EMISSION_VOLUMETRIC
+    o.n_bins = 8
+    ref, _ = O.trace([cam], [m], _fp64_like(o))
+    print(f"[case] volumetric prec {prec} 64x48", flush=True)
+    got, _ = acc.trace_batch([cam], [m], o)
+    lit = ref.intensity.max(0) > 0
+    assert lit.mean() > 0.3
+    if prec == abi.PRECISION_FP64:
+        assert np.array_equal(got.cls, ref.cls)
+        assert np.allclose(got.intensity, ref.intensity, rtol=1e-6, atol=1e-9 * ref.intensity.max())
+        assert np.allclose(got.redshift, ref.redshift, rtol=1e-6, atol=1e-9)  # optical depth
+    else:
+        crit = acc.critical_distance([cam], [m])
+        keep = np.abs(crit) > 0.5
+        assert np.array_equal(got.cls[keep], ref.cls[keep])
+        sel = keep & lit & (got.cls == ref.cls)
+        Ir, Ig = ref.intensity[:, sel], got.intensity[:, sel]
+        rel = np.abs(Ig - Ir).max(0) / np.maximum(Ir.max(0), 1e-2 * np.median(Ir.max(0)))
+        assert rel.max() < 1e-3, rel.max()
+
+
+def _fp64_like(o):
+    """FP64 oracle options with the emission settings of o."""
+    r = O.default_opts(abi.PRECISION_FP64)
+    for k in ("emission", "n_bins", "vol_j0", "vol_h_over_r", "vol_kappa", "vol_r_min", "vol_r_max", "vol_tau_max"):
+        setattr(r, k, getattr(o, k))
+    return r
+
+
+def test_random_spacetime_batch_full_size(acc):
+    """The 64-camera random Kerr-Newman-de Sitter batch (about 18.9 M rays, volumetric emission,
+    8 bins) in one FP32 launch: every ray is classified, and a random sample of 3000 rays
+    matches the FP64 oracle to the FP32 bar outside the critical band."""
+    cams, metrics = random_spacetime_batch()
+    o = O.default_opts(abi.PRECISION_FP32_ESCALATE)
+    o.emission = abi.EMISSION_VOLUMETRIC
+    o.n_bins = 8
+    n = sum(c.width * c.height for c in cams)
+    print(f"[case] random batch 64 cameras {n} rays fp32", flush=True)
+    st = acc.trace_resident(cams, metrics, o)
+    assert st["rays"] == n
+    got = acc.fetch_results()
+    assert np.all(got.cls <= abi.RAY_DISK) and np.isfinite(got.intensity).all() and (got.intensity >= 0).all()
+    rng = np.random.default_rng(0x51121005)
+    ids = np.sort(rng.choice(n, 3000, replace=False))
+    ref, _ = O.trace(cams, metrics, _fp64_like(o), ray_ids=ids)
+    crit = acc.critical_distance(cams, metrics)[ids]
+    sample = got.take(ids)
+    keep = np.abs(crit) > 0.5
+    assert np.array_equal(sample.cls[keep], ref.cls[keep])
+    sel = keep & (ref.intensity.max(0) > 0)
+    Ir, Ig = ref.intensity[:, sel], sample.intensity[:, sel]
+    rel = np.abs(Ig - Ir).max(0) / np.maximum(Ir.max(0), 1e-2 * np.median(Ir.max(0)))
+    assert rel.max() < 1e-3, rel.max()
+
+
+def test_random_spacetime_batch_fp64_matches_oracle(acc):
+    """The same batch at 1/32 of the image sides, every ray, plain FP64 against the oracle."""
+    cams, metrics = random_spacetime_batch(scale=32)
+    o = O.default_opts(abi.PRECISION_FP64)
+    o.emission = abi.EMISSION_VOLUMETRIC
+    o.n_bins = 8
+    ref, _ = O.trace(cams, metrics, o)
+    print(f"[case] random batch 64 cameras {ref.cls.size} rays fp64", flush=True)
+    got, _ = acc.trace_batch(cams, metrics, o)
+    assert np.array_equal(got.cls, ref.cls)
+    assert np.allclose(got.intensity, ref.intensity, rtol=1e-6, atol=1e-9 * ref.intensity.max())
+
+
+def test_autotuner_picks_a_launch_shape_that_keeps_results(acc):
+    m = abi.metric_ks(1.0, 0.9)
+    cam = abi.camera(50.0, 75.0, 20.0, 480, 270, disk=(0.0, 30.0))
+    o = O.default_opts(abi.PRECISION_FP32_ESCALATE)
+    o.escalate_margin = 0.0
+    base, _ = acc.trace_batch([cam], [m], o)
+    print("[case] autotune 480x270", flush=True)
+    best = acc.autotune([cam], [m], o)
+    assert best > 0
+    assert o.block_size in (64, 128, 256) and o.blocks_per_cu >= 1 and 1 <= o.refill_min <= 64
+    tuned, _ = acc.trace_batch([cam], [m], o)
+    for k in ("cls", "theta", "phi", "steps", "intensity"):
+        assert np.array_equal(getattr(tuned, k), getattr(base, k)), k

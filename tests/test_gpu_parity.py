@@ -98,7 +98,7 @@ def test_persistent_queue_equals_grid_launch(acc):
     m = abi.metric_ks(1.0, 0.9)
     cam = abi.camera(50.0, 75.0, 20.0, 320, 180, disk=(0.0, 30.0))
     o = O.default_opts(abi.PRECISION_FP32_ESCALATE)
-    o.escalate_margin = 0.0
+    o.escalate_margin = o.escalate_band = o.escalate_disk_edge = 0.0
     a, sa = acc.trace_batch([cam], [m], o)
     o.schedule = abi.SCHEDULE_GRID
     b, sb = acc.trace_batch([cam], [m], o)
@@ -114,15 +114,17 @@ def test_persistent_queue_equals_grid_launch(acc):
 
 
 def test_escalated_rays_are_bitwise_fp64(acc):
-    """Escalated rays (the gate's critical band and in-flight turning points inside the photon
-    region) are traced by the FP64 persistent kernel with the FP32 path's step policy: they equal
-    an all-FP64 run with that policy bit for bit."""
+    """Escalated rays (the gate's critical band, in-flight turning points inside the photon
+    region, disk crossings at the inner edge) are traced by the FP64 persistent kernel with the
+    FP32 path's step caps and the escalation tolerance: they equal an all-FP64 run with that
+    policy bit for bit."""
     m = abi.metric_ks(1.0, 0.9)
     cam = abi.camera(50.0, 75.0, 20.0, 480, 270, disk=(0.0, 30.0))
     o = O.default_opts(abi.PRECISION_FP32_ESCALATE)
     got, st = acc.trace_batch([cam], [m], o)
     o64 = O.default_opts(abi.PRECISION_FP32_ESCALATE)
     o64.precision = abi.PRECISION_FP64
+    o64.rtol, o64.atol = o.escalate_rtol, o.atol * o.escalate_rtol / o.rtol
     ref, _ = acc.trace_batch([cam], [m], o64)
     e = (got.flags & abi.FLAG_ESCALATED) != 0
     assert e.sum() > 10
