@@ -176,6 +176,10 @@ typedef struct SiriusHipStats {
     double flops_escalated;      /* algorithmic flops of the escalation kernel              */
     double null_drift_max;
     double null_drift_median;
+    double audit_ms;             /* HIP-event time of the surrogate audit (FP64 re-traces)   */
+    double audit_dir_err_max;    /* largest escape-direction error of audited answers (rad) */
+    uint64_t audit_class_mismatch; /* audited answers whose class the integrator disputes    */
+    double surrogate_flops;      /* algorithmic flops of the network inference               */
 } SiriusHipStats;
 
 typedef void* SiriusHipHandle;
@@ -263,6 +267,19 @@ int sirius_hip_mlp_loss_grad(SiriusHipHandle h, const float* x, const int32_t* c
  * receives the batch loss before the update. */
 int sirius_hip_mlp_train_step(SiriusHipHandle h, const float* x, const int32_t* cls,
                               const float* dir, int n, float lr, float* loss);
+/* Network inputs x[n*5] and initial Cartesian directions n0[n*3] of every ray of the
+ * cameras (device buffers; Kerr-Schild cameras). */
+int sirius_hip_surrogate_features(SiriusHipHandle h, const SiriusHipCamera* cams, int n_cam,
+                                  const SiriusHipMetric* metrics, int n_metric, float* x, float* n0);
+/* Trains the network on every ray of the cameras, labelled by the integrator with label_opts
+ * (FP64): inputs and targets are built on the device, then `epochs` passes of Adam over
+ * batches of `batch` rays in a seeded random order, at learning rate lr with a cosine decay
+ * to lr/20. *loss (host) receives the mean batch loss of the last epoch. Uses the current
+ * parameters (sirius_hip_mlp_init first for a fresh network). */
+int sirius_hip_surrogate_train(SiriusHipHandle h, const SiriusHipCamera* cams, int n_cam,
+                               const SiriusHipMetric* metrics, int n_metric,
+                               const SiriusHipTraceOpts* label_opts, int epochs, int batch,
+                               float lr, uint64_t seed, float* loss);
 
 #ifdef __cplusplus
 }

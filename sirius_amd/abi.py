@@ -83,6 +83,8 @@ class Stats(C.Structure):
         ("kernel_ms", C.c_double), ("escalate_ms", C.c_double), ("surrogate_ms", C.c_double),
         ("total_ms", C.c_double), ("flops", C.c_double), ("flops_escalated", C.c_double),
         ("null_drift_max", C.c_double), ("null_drift_median", C.c_double),
+        ("audit_ms", C.c_double), ("audit_dir_err_max", C.c_double), ("audit_class_mismatch", C.c_uint64),
+        ("surrogate_flops", C.c_double),
     ]
 
     def as_dict(self):

@@ -68,6 +68,11 @@ def lib():
                                            C.POINTER(C.c_float)]
     L.sirius_hip_mlp_train_step.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, i, C.c_float,
                                             C.POINTER(C.c_float)]
+    L.sirius_hip_surrogate_features.argtypes = [H, C.POINTER(abi.Camera), i, C.POINTER(abi.Metric), i,
+                                                C.c_void_p, C.c_void_p]
+    L.sirius_hip_surrogate_train.argtypes = [H, C.POINTER(abi.Camera), i, C.POINTER(abi.Metric), i,
+                                             C.POINTER(abi.TraceOpts), i, i, C.c_float, C.c_uint64,
+                                             C.POINTER(C.c_float)]
     _lib = L
     return L
 
@@ -248,6 +253,31 @@ class HipAccelerator:
         self._check(self._L.sirius_hip_mlp_train_step(self._h, self._dev(x, torch.float32), self._dev(cls, torch.int32),
                                                       self._dev(dirs, torch.float32), x.shape[0], C.c_float(lr),
                                                       C.byref(loss)))
+        return loss.value
+
+    def surrogate_features(self, cams, metrics):
+        """(x, n0) CUDA float32 tensors (n, 5) and (n, 3): network inputs and initial directions."""
+        import torch
+        cams, metrics = list(cams), list(metrics)
+        ca = (abi.Camera * len(cams))(*cams)
+        ma = (abi.Metric * len(metrics))(*metrics)
+        n = sum(c.width * c.height for c in cams)
+        x = torch.empty((n, 5), dtype=torch.float32, device="cuda")
+        n0 = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        self._check(self._L.sirius_hip_surrogate_features(self._h, ca, len(cams), ma, len(metrics), x.data_ptr(),
+                                                          n0.data_ptr()))
+        return x, n0
+
+    def surrogate_train(self, cams, metrics, label_opts, epochs, batch, lr, seed):
+        """Train the network on the cameras' rays (labels from the integrator); returns the last
+        epoch's mean batch loss."""
+        cams, metrics = list(cams), list(metrics)
+        ca = (abi.Camera * len(cams))(*cams)
+        ma = (abi.Metric * len(metrics))(*metrics)
+        loss = C.c_float()
+        self._check(self._L.sirius_hip_surrogate_train(self._h, ca, len(cams), ma, len(metrics), C.byref(label_opts),
+                                                       epochs, batch, C.c_float(lr), seed, C.byref(loss)))
         return loss.value
 
     def autotune(self, cams, metrics, opts):

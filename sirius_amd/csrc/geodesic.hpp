@@ -484,6 +484,15 @@ SIRIUS_HD __forceinline__ void mt_radial_tangential(const CamConst& c, const T n
     nt = sqrt(ns[1] * ns[1] + ns[2] * ns[2]);
 }
 
+// Unit initial direction of pixel (px, py) in the Cartesian frame (Kerr-Schild and warp cameras).
+template <class T>
+SIRIUS_HD __forceinline__ void camera_cart_dir(const CamConst& c, int px, int py, T n[3]) {
+    T ns[3];
+    camera_dir<T>(c, px, py, ns);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) n[i] = ns[0] * T(c.er[i]) + ns[1] * T(c.et[i]) + ns[2] * T(c.ep[i]);
+}
+
 template <class T>
 SIRIUS_HD __forceinline__ void init_state(const CamConst& c, const Spacetime<T>& s, int px, int py, T x[4], T p[4]) {
     T ns[3];
@@ -501,7 +510,7 @@ SIRIUS_HD __forceinline__ void init_state(const CamConst& c, const Spacetime<T>&
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             x[i + 1] = T(c.r_obs * c.er[i]);
-            k[i + 1] = ns[0] * T(c.er[i]) + ns[1] * T(c.et[i]) + ns[2] * T(c.ep[i]);
+            k[i + 1] = ns[0] * T(c.er[i]) + ns[1] * T(c.et[i]) + ns[2] * T(c.ep[i]);  // unit Cartesian direction
         }
     }
     metric_cov(s, x, g);
@@ -568,14 +577,25 @@ SIRIUS_HD __forceinline__ void disk_emission(const CamConst& c, const TraceParam
 // ------------------------------------------------------------------------------------------
 #define SIRIUS_CC_TABLE 512
 
+// Impact parameters of a ray on the observer's sky (Bardeen): alpha = -L_z / (E sin theta),
+// beta = p_theta / E, from the conserved E = -p_t and L_z = x p_y - y p_x and the polar
+// momentum at the ray's position.
 template <class T>
-SIRIUS_HD __forceinline__ T crit_distance(const float* tab, const Spacetime<T>& s, const T x[4], const T p[4]) {
+SIRIUS_HD __forceinline__ void sky_alpha_beta(const Spacetime<T>& s, const T x[4], const T p[4], T& alpha, T& beta) {
     T E = -p[0];
     T L = x[1] * p[2] - x[2] * p[1];
     T r = kerr_r(s, x[1], x[2], x[3]);
     T ct = x[3] / r, st = sqrt(fmax(T(1) - ct * ct, T(1e-30)));
     T pth = (ct / st) * (x[1] * p[1] + x[2] * p[2]) - r * st * p[3];
-    T da = -L / (E * st) - T(tab[0]), b = fabs(pth / E);
+    alpha = -L / (E * st);
+    beta = pth / E;
+}
+
+template <class T>
+SIRIUS_HD __forceinline__ T crit_distance(const float* tab, const Spacetime<T>& s, const T x[4], const T p[4]) {
+    T alpha, beta;
+    sky_alpha_beta(s, x, p, alpha, beta);
+    T da = alpha - T(tab[0]), b = fabs(beta);
     T psi = atan2(b, da), rho = sqrt(da * da + b * b);
     T u = psi / T(3.141592653589793) * T(SIRIUS_CC_TABLE - 1);
     int j = (int)u;

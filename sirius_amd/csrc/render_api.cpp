@@ -131,8 +131,7 @@ int sirius_hip_launch(SiriusHipHandle hh, const SiriusHipCamera* cam, const Siri
     if (rc) return rc;
     DevOut dout;
     if ((rc = bind_outputs(*H, nullptr, B.n, opts->n_bins, dout))) return rc;
-    if (opts->use_surrogate) rc = surrogate_trace(*H, B, *opts, dout, nullptr);
-    else rc = run_trace(*H, B, *opts, dout, nullptr, nullptr, 0, nullptr, false);
+    rc = run_trace(*H, B, *opts, dout, nullptr, nullptr, 0, nullptr, false);
     if (rc) return rc;
     hipError_t e = launch_shade(dout, B.n, c.width, c.height, opts->n_bins, H->bg_dev, H->bg_w, H->bg_h, H->fb_dev,
                                 H->stream);

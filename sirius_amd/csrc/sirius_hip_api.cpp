@@ -18,6 +18,7 @@
 
 #include "../../include/sirius_hip.h"
 #include "geodesic.hpp"
+#include "mlp.hpp"
 #include "sirius_internal.hpp"
 
 namespace sirius {
@@ -45,6 +46,9 @@ hipError_t launch_gate(const CamConst* cams, int n_cam, const float* tabs, int64
                        int32_t* list_a, int32_t* list_b, uint32_t* counts, uint8_t* flags, hipStream_t st);
 hipError_t launch_shade(const DevOut& o, int64_t n, int width, int height, int n_bins, const uint8_t* bg, int bgw,
                         int bgh, float* fb, hipStream_t st);
+hipError_t launch_sur_audit(const CamConst* cams, const TraceParams& tp, const DevOut& oa, const DevOut& o,
+                            const int32_t* list, const uint32_t* count, int64_t cap, unsigned int* acc,
+                            hipStream_t st);
 
 // ---------------------------------------------------------------------------------------
 // Host-side closed forms (FP64), shared with the camera constant setup.
@@ -214,7 +218,7 @@ void Handle::release() {
     ev0 = ev1 = ev2 = ev3 = ev4 = nullptr;
     delete last;
     last = nullptr;
-    for (hipEvent_t* ev : {&ev_gate, &ev_band0, &ev_band1}) {
+    for (hipEvent_t* ev : {&ev_gate, &ev_band0, &ev_band1, &ev_s0, &ev_s1, &ev_a0, &ev_a1}) {
         if (*ev) (void)hipEventDestroy(*ev);
         *ev = nullptr;
     }
@@ -253,7 +257,7 @@ int Handle::ensure_device() {
     total_mem = prop.totalGlobalMem;
     if ((e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "stream");
     if ((e = hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "stream");
-    for (hipEvent_t* ev : {&ev_gate, &ev_band0, &ev_band1})
+    for (hipEvent_t* ev : {&ev_gate, &ev_band0, &ev_band1, &ev_s0, &ev_s1, &ev_a0, &ev_a1})
         if ((e = hipEventCreate(ev)) != hipSuccess) return hip_fail(e, "event");
     if ((e = hipEventCreate(&ev0)) != hipSuccess) return hip_fail(e, "event");
     if ((e = hipEventCreate(&ev1)) != hipSuccess) return hip_fail(e, "event");
@@ -305,6 +309,11 @@ static int validate_opts(Handle& H, const SiriusHipTraceOpts& o) {
         return H.fail(SIRIUS_HIP_E_INVALID, "block_size must be 0 or a multiple of 64 in [64, 1024]");
     if (o.integrator == SIRIUS_INTEGRATOR_SYMPLECTIC && !(o.symplectic_h > 0))
         return H.fail(SIRIUS_HIP_E_INVALID, "symplectic_h must be positive");
+    if (o.use_surrogate && (o.precision != SIRIUS_PRECISION_FP32_ESCALATE || o.schedule != SIRIUS_SCHEDULE_PERSISTENT ||
+                            o.integrator != SIRIUS_INTEGRATOR_DP45))
+        return H.fail(SIRIUS_HIP_E_UNSUPPORTED, "the surrogate runs on the FP32 persistent DP45 path");
+    if (o.use_surrogate && !(o.audit_fraction >= 0 && o.audit_fraction <= 1))
+        return H.fail(SIRIUS_HIP_E_INVALID, "audit_fraction must be in [0, 1]");
     return SIRIUS_HIP_OK;
 }
 
@@ -384,6 +393,26 @@ int bind_outputs(Handle& H, const SiriusHipRayOutput* out, int64_t n, int n_bins
     o.intensity = (float*)get("o_int", out ? out->intensity : nullptr, 4 * n * std::max(n_bins, 1));
     // the final state is always kept: the finish kernel derives every output from it
     o.state = (double*)get("o_state", out ? out->state : nullptr, 64 * n);
+    return rc;
+}
+
+// Library-owned result arrays of the surrogate audit (separate from the batch's outputs).
+static int bind_audit_outputs(Handle& H, int64_t n, int n_bins, DevOut& o) {
+    int rc = SIRIUS_HIP_OK;
+    o.n_rays = n;
+    o.cls = (uint8_t*)H.scratch("a_cls", n, &rc);
+    o.flags = (uint8_t*)H.scratch("a_flags", n, &rc);
+    o.theta = (float*)H.scratch("a_theta", 4 * n, &rc);
+    o.phi = (float*)H.scratch("a_phi", 4 * n, &rc);
+    o.lam = (float*)H.scratch("a_lam", 4 * n, &rc);
+    o.disk_r = (float*)H.scratch("a_disk_r", 4 * n, &rc);
+    o.redshift = (float*)H.scratch("a_redshift", 4 * n, &rc);
+    o.null_drift = (float*)H.scratch("a_null", 4 * n, &rc);
+    o.steps = (uint32_t*)H.scratch("a_steps", 4 * n, &rc);
+    o.rejected = (uint32_t*)H.scratch("a_rej", 4 * n, &rc);
+    o.landings = (uint32_t*)H.scratch("a_land", 4 * n, &rc);
+    o.intensity = (float*)H.scratch("a_int", 4 * n * std::max(n_bins, 1), &rc);
+    o.state = (double*)H.scratch("a_state", 64 * n, &rc);
     return rc;
 }
 
@@ -470,11 +499,30 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
         const uint32_t* count_a = list_count_dev;
         int64_t cap_a = n_work;
         int32_t* list_b = nullptr;
-        bool gate = !list_dev && B.family == SIRIUS_METRIC_KERR_SCHILD && B.tp.escalate &&
+        // the surrogate answers before the integrator (Kerr-Schild, thin disk or no emission)
+        const bool sur = o.use_surrogate != 0;
+        if (sur && (list_dev || B.family != SIRIUS_METRIC_KERR_SCHILD || B.tp.emission == SIRIUS_EMISSION_VOLUMETRIC))
+            return H.fail(SIRIUS_HIP_E_UNSUPPORTED, "the surrogate serves whole Kerr-Schild batches without volumetric emission");
+        if (sur && (!H.mlp || !H.mlp->ready))
+            return H.fail(SIRIUS_HIP_E_STATE, "use_surrogate needs a network (sirius_hip_mlp_init / set_params / surrogate_train)");
+        bool gate = !sur && !list_dev && B.family == SIRIUS_METRIC_KERR_SCHILD && B.tp.escalate &&
                     (o.escalate_band > 0 || o.critical_band > 0);
-        bool band = gate && o.escalate_band > 0;
+        bool band = (gate || sur) && o.escalate_band > 0;
         TraceParams t32 = B.tp;
-        t32.gate_flags = gate ? 1 : 0;
+        t32.gate_flags = (gate || sur) ? 1 : 0;
+        int32_t* list_audit = nullptr;
+        if (sur) {
+            int32_t* la = (int32_t*)H.scratch("gate_a", 4 * (size_t)B.n, &rc);
+            list_b = (int32_t*)H.scratch("gate_b", 4 * (size_t)B.n, &rc);
+            list_audit = (int32_t*)H.scratch("sur_audit", 4 * (size_t)B.n, &rc);
+            if (rc) return rc;
+            if (time_it) (void)hipEventRecord(H.ev_s0, H.stream);
+            if ((rc = surrogate_stage(H, B, o, dout, la, list_b, list_audit, work + 8))) return rc;
+            if (time_it) (void)hipEventRecord(H.ev_s1, H.stream);
+            list_a = la;
+            count_a = work + 8;
+            cap_a = B.n;
+        }
         if (gate) {
             float* tabs = upload_critical_tables(H, B, &rc);
             int32_t* la = (int32_t*)H.scratch("gate_a", 4 * (size_t)B.n, &rc);
@@ -530,6 +578,30 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
         }
         if (band) (void)hipStreamWaitEvent(H.stream, H.ev_band1, 0);
         H.last_band = band;
+        H.last_audit = sur && o.audit_fraction > 0;
+        if (H.last_audit) {
+            // audit after the batch (timed apart): FP64 re-traces of the audited answers
+            if (time_it) (void)hipEventRecord(H.ev_a0, H.stream);
+            DevOut oa;
+            if ((rc = bind_audit_outputs(H, B.n, o.n_bins, oa))) return rc;
+            unsigned int* aacc = (unsigned int*)H.scratch("audit_acc", 64, &rc);
+            if (rc) return rc;
+            if ((e = launch_fill(aacc, 2, 0u, H.stream)) != hipSuccess) return H.hip_fail(e, "fill");
+            unsigned long long* aatt = (unsigned long long*)H.scratch("audit_att", 8, &rc);
+            if (rc) return rc;
+            TraceParams ta = t64;
+            ta.gate_flags = 0;
+            int bpc64 = persistent_blocks_per_cu(true, B.family, B.tp.emission, B.tp.n_cam > 1, 64);
+            if ((e = launch_trace_persistent(true, B.family, B.tp.emission, B.tp.n_cam > 1, B.cams_dev, ta, oa,
+                                             list_audit, work + 10, B.n, work + 11, work + 5, nullptr, aatt,
+                                             std::max(1, bpc64 * H.cu_count), 64, 1, SIRIUS_FLAG_AUDITED,
+                                             H.stream)) != hipSuccess)
+                return H.hip_fail(e, "audit launch");
+            if ((e = launch_sur_audit(B.cams_dev, ta, oa, dout, list_audit, work + 10, B.n, aacc, H.stream)) !=
+                hipSuccess)
+                return H.hip_fail(e, "audit compare");
+            if (time_it) (void)hipEventRecord(H.ev_a1, H.stream);
+        }
     }
     if (time_it) {
         if (!persistent) (void)hipEventRecord(H.ev1, H.stream);
@@ -572,6 +644,20 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
         }
         stats->steps_bulk += bulk;
         stats->flops += (double)bulk * flops_per_step(B.family);
+        if (H.last_audit) {
+            unsigned int aa[2] = {0, 0};
+            if ((e = hipMemcpy(aa, H.bufs["audit_acc"].ptr, sizeof(aa), hipMemcpyDeviceToHost)) != hipSuccess)
+                return H.hip_fail(e, "audit copy");
+            float err;
+            std::memcpy(&err, &aa[0], 4);
+            stats->audit_dir_err_max = std::max(stats->audit_dir_err_max, (double)err);
+            stats->audit_class_mismatch += aa[1];
+        }
+        if (o.use_surrogate) {
+            stats->rays_surrogate += h[5];
+            stats->rays_fallback += (uint64_t)wk[8] + wk[9];
+            stats->surrogate_flops += (double)B.n * 398848.0;
+        }
         if (time_it) {
             float ms = 0, ms2 = 0, ms3 = 0;
             (void)hipEventElapsedTime(&ms, H.ev0, H.ev1);
@@ -579,6 +665,15 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
             if (H.last_band) (void)hipEventElapsedTime(&ms3, H.ev_band0, H.ev_band1);
             stats->kernel_ms += ms;
             stats->escalate_ms += std::max(ms2, ms3);
+            if (o.use_surrogate) {
+                float ms4 = 0, ms5 = 0;
+                (void)hipEventElapsedTime(&ms4, H.ev_s0, H.ev_s1);
+                stats->surrogate_ms += ms4;
+                if (H.last_audit) {
+                    (void)hipEventElapsedTime(&ms5, H.ev_a0, H.ev_a1);
+                    stats->audit_ms += ms5;
+                }
+            }
         }
     }
     return SIRIUS_HIP_OK;
@@ -747,11 +842,7 @@ int sirius_hip_trace_batch(SiriusHipHandle hh, const SiriusHipCamera* cams, int 
     H->last_bins = opts->n_bins;
     if (stats) std::memset(stats, 0, sizeof(*stats));
     (void)hipEventRecord(H->ev2, H->stream);
-    if (opts->use_surrogate) {
-        rc = surrogate_trace(*H, B, *opts, dout, stats);
-    } else {
-        rc = run_trace(*H, B, *opts, dout, nullptr, nullptr, 0, stats, true);
-    }
+    rc = run_trace(*H, B, *opts, dout, nullptr, nullptr, 0, stats, true);
     if (rc) return rc;
     (void)hipEventRecord(H->ev3, H->stream);
     hipError_t e = hipEventSynchronize(H->ev3);

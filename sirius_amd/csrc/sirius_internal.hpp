@@ -29,6 +29,7 @@ struct Handle {
     hipStream_t stream2 = nullptr;  // second stream: the FP64 band runs beside the FP32 bulk
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr, ev4 = nullptr;
     hipEvent_t ev_gate = nullptr, ev_band0 = nullptr, ev_band1 = nullptr;
+    hipEvent_t ev_s0 = nullptr, ev_s1 = nullptr, ev_a0 = nullptr, ev_a1 = nullptr;  // surrogate, audit
     int cu_count = 0;
     int max_threads = 0;
     size_t total_mem = 0;
@@ -47,6 +48,7 @@ struct Handle {
     DevOut* last = nullptr;
     int last_bins = 0;
     bool last_band = false;  // the last FP32 trace ran the FP64 band on the second stream
+    bool last_audit = false; // the last FP32 trace audited surrogate answers
     // surrogate network
     MlpState* mlp = nullptr;
 
@@ -82,6 +84,10 @@ float* upload_critical_tables(Handle& H, Batch& B, int* rc);
 
 // surrogate (mlp_kernels.hip / surrogate.cpp)
 void mlp_release(Handle& H);
-int surrogate_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, SiriusHipStats* stats);
+// Network stage of the FP32 path: inputs, inference, per-ray decision. Fills list_a (FP32
+// integrator), list_b (FP64 band) and list_audit; counts[0..2] are their lengths.
+int surrogate_stage(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, int32_t* list_a,
+                    int32_t* list_b, int32_t* list_audit, uint32_t* counts);
+int mlp_forward_dev(Handle& H, const float* x, float* y, int64_t n);
 
 }  // namespace sirius

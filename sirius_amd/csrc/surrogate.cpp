@@ -230,8 +230,36 @@ int mlp_forward_dev(Handle& H, const float* x, float* y, int64_t n) {
     return SIRIUS_HIP_OK;
 }
 
-int surrogate_trace(Handle& H, Batch&, const SiriusHipTraceOpts&, DevOut&, SiriusHipStats*) {
-    return H.fail(SIRIUS_HIP_E_UNSUPPORTED, "surrogate tracing is not available yet");
+hipError_t launch_sur_features(const CamConst* cams, int n_cam, const float* tabs, int64_t n, double crit_band,
+                               float* x, float* n0, float* dist, uint8_t* flags, hipStream_t st);
+hipError_t launch_sur_targets(const DevOut& o, const float* n0, int64_t n, int32_t* cls, float* dir, hipStream_t st);
+hipError_t launch_sur_apply(const float* y, const float* n0, const float* dist, int64_t n, double esc_band,
+                            double sur_band, double pmin, double audit_fraction, uint64_t audit_seed, const DevOut& o,
+                            int n_bins, int32_t* list_a, int32_t* list_b, int32_t* list_audit, uint32_t* counts,
+                            hipStream_t st);
+hipError_t launch_sur_gather(const int32_t* perm, int64_t start, int64_t m, const float* x, const int32_t* cls,
+                             const float* dir, float* xb, int32_t* cb, float* db, hipStream_t st);
+
+int surrogate_stage(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, int32_t* list_a,
+                    int32_t* list_b, int32_t* list_audit, uint32_t* counts) {
+    int rc = SIRIUS_HIP_OK;
+    float* tabs = upload_critical_tables(H, B, &rc);
+    const int64_t n = B.n;
+    float* x = (float*)H.scratch("sur_x", (size_t)n * 5 * 4, &rc);
+    float* n0 = (float*)H.scratch("sur_n0", (size_t)n * 3 * 4, &rc);
+    float* dist = (float*)H.scratch("sur_dist", (size_t)n * 4, &rc);
+    float* y = (float*)H.scratch("sur_y", (size_t)n * SIRIUS_MLP_OUT * 4, &rc);
+    if (rc) return rc;
+    hipError_t e;
+    if ((e = launch_sur_features(B.cams_dev, (int)B.cc.size(), tabs, n, std::max(o.critical_band, 0.0), x, n0, dist,
+                                 dout.flags, H.stream)) != hipSuccess)
+        return H.hip_fail(e, "surrogate inputs");
+    if ((rc = mlp_forward_dev(H, x, y, n))) return rc;
+    if ((e = launch_sur_apply(y, n0, dist, n, std::max(o.escalate_band, 0.0), o.surrogate_band, o.surrogate_pmin,
+                              o.audit_fraction, o.audit_seed, dout, o.n_bins, list_a, list_b, list_audit, counts,
+                              H.stream)) != hipSuccess)
+        return H.hip_fail(e, "surrogate decision");
+    return SIRIUS_HIP_OK;
 }
 
 // splitmix64 stream for the initial weights
@@ -325,6 +353,103 @@ int sirius_hip_mlp_loss_grad(SiriusHipHandle hh, const float* x, const int32_t* 
     MLP_GUARD_END(H)
 }
 
+static int mlp_adam_step(Handle& H, const float* x, const int32_t* cls, const float* dir, int n, float lr,
+                         float* loss) {
+    MlpState& S = *H.mlp;
+    int rc = mlp_loss_grad_dev(H, x, cls, dir, n, S.grad, loss);
+    if (rc) return rc;
+    S.step++;
+    const float c1 = 1.f - std::pow(0.9f, (float)S.step), c2 = 1.f - std::pow(0.999f, (float)S.step);
+    hipError_t e;
+    if ((e = launch_adam(S.params, S.grad, S.adam_m, S.adam_v, (int64_t)mlp_param_count(), lr, c1, c2, H.stream)) !=
+            hipSuccess ||
+        (e = launch_pack(S.params, S.w, S.b, H.stream)) != hipSuccess)
+        return H.hip_fail(e, "adam");
+    return SIRIUS_HIP_OK;
+}
+
+int sirius_hip_surrogate_features(SiriusHipHandle hh, const SiriusHipCamera* cams, int n_cam,
+                                  const SiriusHipMetric* metrics, int n_metric, float* x, float* n0) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    MLP_GUARD_BEGIN
+    if (!x || !n0) return H->fail(SIRIUS_HIP_E_INVALID, "x and n0 are required");
+    int rc = H->ensure_device();
+    if (rc) return rc;
+    SiriusHipTraceOpts o;
+    sirius_hip_default_opts(&o, SIRIUS_PRECISION_FP64);
+    Batch B;
+    if ((rc = prepare_batch(*H, cams, n_cam, metrics, n_metric, o, B))) return rc;
+    if (B.family != SIRIUS_METRIC_KERR_SCHILD) return H->fail(SIRIUS_HIP_E_UNSUPPORTED, "Kerr-Schild cameras only");
+    hipError_t e = launch_sur_features(B.cams_dev, (int)B.cc.size(), nullptr, B.n, 0.0, x, n0, nullptr, nullptr,
+                                       H->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(H->stream);
+    return e == hipSuccess ? SIRIUS_HIP_OK : H->hip_fail(e, "surrogate inputs");
+    MLP_GUARD_END(H)
+}
+
+int sirius_hip_surrogate_train(SiriusHipHandle hh, const SiriusHipCamera* cams, int n_cam,
+                               const SiriusHipMetric* metrics, int n_metric, const SiriusHipTraceOpts* label_opts,
+                               int epochs, int batch, float lr, uint64_t seed, float* loss) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    MLP_GUARD_BEGIN
+    if (!H->mlp || !H->mlp->ready) return H->fail(SIRIUS_HIP_E_STATE, "network not initialised");
+    if (!label_opts || epochs <= 0 || batch <= 0 || !(lr > 0))
+        return H->fail(SIRIUS_HIP_E_INVALID, "label_opts, epochs > 0, batch > 0, lr > 0");
+    if (label_opts->use_surrogate) return H->fail(SIRIUS_HIP_E_INVALID, "labels come from the integrator");
+    int rc;
+    Batch B;
+    if ((rc = prepare_batch(*H, cams, n_cam, metrics, n_metric, *label_opts, B))) return rc;
+    if (B.family != SIRIUS_METRIC_KERR_SCHILD) return H->fail(SIRIUS_HIP_E_UNSUPPORTED, "Kerr-Schild cameras only");
+    DevOut dout;
+    if ((rc = bind_outputs(*H, nullptr, B.n, label_opts->n_bins, dout))) return rc;
+    if ((rc = run_trace(*H, B, *label_opts, dout, nullptr, nullptr, 0, nullptr, false))) return rc;
+    const int64_t n = B.n;
+    float* x = (float*)H->scratch("tr_x", (size_t)n * 5 * 4, &rc);
+    float* n0 = (float*)H->scratch("tr_n0", (size_t)n * 3 * 4, &rc);
+    int32_t* cls = (int32_t*)H->scratch("tr_cls", (size_t)n * 4, &rc);
+    float* dir = (float*)H->scratch("tr_dir", (size_t)n * 3 * 4, &rc);
+    int32_t* perm = (int32_t*)H->scratch("tr_perm", (size_t)n * 4, &rc);
+    const int64_t m = std::min<int64_t>(batch, n);
+    float* xb = (float*)H->scratch("tr_xb", (size_t)m * 5 * 4, &rc);
+    int32_t* cb = (int32_t*)H->scratch("tr_cb", (size_t)m * 4, &rc);
+    float* db = (float*)H->scratch("tr_db", (size_t)m * 3 * 4, &rc);
+    if (rc) return rc;
+    hipError_t e;
+    if ((e = launch_sur_features(B.cams_dev, (int)B.cc.size(), nullptr, n, 0.0, x, n0, nullptr, nullptr, H->stream)) !=
+            hipSuccess ||
+        (e = launch_sur_targets(dout, n0, n, cls, dir, H->stream)) != hipSuccess)
+        return H->hip_fail(e, "training set");
+    std::vector<int32_t> p((size_t)n);
+    uint64_t s = seed;
+    const int64_t steps_per_epoch = n / m;
+    const int64_t total = steps_per_epoch * epochs;
+    int64_t step = 0;
+    double last = 0;
+    for (int ep = 0; ep < epochs; ++ep) {
+        for (int64_t i = 0; i < n; ++i) p[(size_t)i] = (int32_t)i;
+        for (int64_t i = n - 1; i > 0; --i) std::swap(p[(size_t)i], p[(size_t)(splitmix(s) % (uint64_t)(i + 1))]);
+        if ((e = hipMemcpyAsync(perm, p.data(), (size_t)n * 4, hipMemcpyHostToDevice, H->stream)) != hipSuccess)
+            return H->hip_fail(e, "permutation upload");
+        double sum = 0;
+        for (int64_t b = 0; b < steps_per_epoch; ++b, ++step) {
+            if ((e = launch_sur_gather(perm, b * m, m, x, cls, dir, xb, cb, db, H->stream)) != hipSuccess)
+                return H->hip_fail(e, "gather");
+            const double f = 0.5 * (1.0 + std::cos(3.141592653589793 * (double)step / (double)std::max<int64_t>(total, 1)));
+            const float lr_t = (float)(lr * (0.05 + 0.95 * f));
+            float l = 0.f;
+            if ((rc = mlp_adam_step(*H, xb, cb, db, (int)m, lr_t, &l))) return rc;
+            sum += l;
+        }
+        last = sum / (double)std::max<int64_t>(steps_per_epoch, 1);
+    }
+    if ((e = hipStreamSynchronize(H->stream)) != hipSuccess) return H->hip_fail(e, "synchronize");
+    if (loss) *loss = (float)last;
+    return SIRIUS_HIP_OK;
+    MLP_GUARD_END(H)
+}
+
 int sirius_hip_mlp_train_step(SiriusHipHandle hh, const float* x, const int32_t* cls, const float* dir, int n,
                               float lr, float* loss) {
     if (!hh) return SIRIUS_HIP_E_INVALID;
@@ -332,17 +457,7 @@ int sirius_hip_mlp_train_step(SiriusHipHandle hh, const float* x, const int32_t*
     MLP_GUARD_BEGIN
     if (!H->mlp || !H->mlp->ready) return H->fail(SIRIUS_HIP_E_STATE, "network not initialised");
     if (!x || !cls || !dir || n <= 0 || !(lr > 0)) return H->fail(SIRIUS_HIP_E_INVALID, "x, cls, dir, n > 0, lr > 0");
-    MlpState& S = *H->mlp;
-    int rc = mlp_loss_grad_dev(*H, x, cls, dir, n, S.grad, loss);
-    if (rc) return rc;
-    S.step++;
-    const float c1 = 1.f - std::pow(0.9f, (float)S.step), c2 = 1.f - std::pow(0.999f, (float)S.step);
-    hipError_t e;
-    if ((e = launch_adam(S.params, S.grad, S.adam_m, S.adam_v, (int64_t)mlp_param_count(), lr, c1, c2, H->stream)) !=
-            hipSuccess ||
-        (e = launch_pack(S.params, S.w, S.b, H->stream)) != hipSuccess)
-        return H->hip_fail(e, "adam");
-    return SIRIUS_HIP_OK;
+    return mlp_adam_step(*H, x, cls, dir, n, lr, loss);
     MLP_GUARD_END(H)
 }
 

@@ -515,6 +515,31 @@ __global__ __launch_bounds__(256) void k_finish(const CamConst* __restrict__ cam
     ray_outputs<FAM, EMIT>(cams[cam], tp, o, id);
 }
 
+// Surrogate audit: outputs of the FP64 re-traces of audited rays (in oa), compared with the
+// surrogate's answers in o. acc[0]: largest escape-direction error (float bits), acc[1]:
+// class mismatches.
+__global__ void k_sur_audit(const CamConst* __restrict__ cams, TraceParams tp, DevOut oa, DevOut o,
+                            const int32_t* __restrict__ list, const uint32_t* __restrict__ count, int64_t cap,
+                            unsigned int* __restrict__ acc) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t lim = min((int64_t)*count, cap);
+    if (i >= lim) return;
+    const int64_t id = list[i];
+    const int cam = tp.n_cam == 1 ? 0 : find_camera(cams, tp.n_cam, id);
+    ray_outputs<0, SIRIUS_EMISSION_THIN_DISK>(cams[cam], tp, oa, id);
+    const int ca = oa.cls[id], cs = o.cls[id];
+    if (ca != cs) {
+        atomicAdd(&acc[1], 1u);
+        return;
+    }
+    if (ca == SIRIUS_RAY_ESCAPED) {
+        float ta = oa.theta[id], pa = oa.phi[id], ts = o.theta[id], ps = o.phi[id];
+        float dot = sinf(ta) * sinf(ts) * cosf(pa - ps) + cosf(ta) * cosf(ts);
+        float err = acosf(fminf(1.f, fmaxf(-1.f, dot)));
+        atomicMax(&acc[0], __float_as_uint(err));
+    }
+}
+
 // Fill a small device array (graph-safe replacement for a memset).
 __global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -711,6 +736,14 @@ hipError_t launch_symplectic(bool fp64, int family, const CamConst* cams, const 
             default: hipLaunchKernelGGL((k_symplectic_grid<float, 0>), g, b, 0, st, cams, tp, o); break;
         }
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_sur_audit(const CamConst* cams, const TraceParams& tp, const DevOut& oa, const DevOut& o,
+                            const int32_t* list, const uint32_t* count, int64_t cap, unsigned int* acc,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(k_sur_audit, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, cams, tp, oa, o, list,
+                       count, cap, acc);
     return hipGetLastError();
 }
 
