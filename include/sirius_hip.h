@@ -109,9 +109,11 @@ typedef struct SiriusHipTraceOpts {
     int32_t n_bins;          /* spectral bins, log-spaced in h nu / k T_max                 */
     int32_t use_surrogate;   /* 1: MLP far-field surrogate with integrator fallback         */
     double x_min, x_max;     /* bin range in h nu / k T_max                                 */
-    double escalate_margin;  /* FP32 path: escalate inside r < escalate_margin * r_ph(retro)*/
+    double escalate_margin;  /* FP32 path: escalate at a radial turning point inside
+                              * r < escalate_margin * r_ph(retro); 0 = off (default)          */
     double escalate_band;    /* FP32 path: rays within this distance (M) of the critical
-                              * curve are traced in FP64 from the start (Kerr-Schild only)   */
+                              * curve are traced in FP64 from the start on a second stream
+                              * (Kerr-Schild only); 0 = off (default)                          */
     double surrogate_band;   /* fallback band around the critical curve (M)                 */
     double surrogate_pmin;   /* fallback when max class probability is below this           */
     double audit_fraction;   /* fraction of surrogate rays re-traced in FP64                */
@@ -129,7 +131,9 @@ typedef struct SiriusHipTraceOpts {
                               * this fraction of r_in (|r/r_in - 1| < edge) is re-traced in FP64,
                               * where the emission's steep edge magnifies FP32 position error */
     double escalate_rtol;    /* relative tolerance of escalated FP64 re-traces (atol scales with
-                              * it); they keep the FP32 path's step caps. 0: the path's rtol    */
+                              * it); they keep the FP32 path's step caps. 0 (default): the
+                              * path's rtol, except 1e-8 for the warp bubble, whose wall
+                              * crossings the FP32 tolerance does not control                  */
     double critical_band;    /* FP32 path, Kerr-Schild: rays within this distance (M) of the
                               * critical curve are flagged SIRIUS_FLAG_CRITICAL; the accuracy
                               * contract exempts them and the inner-edge escalation skips them */

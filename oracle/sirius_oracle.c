@@ -313,8 +313,8 @@ void oracle_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->n_bins = 16;
     o->x_min = 0.1;
     o->x_max = 10.0;
-    o->escalate_margin = 1.0;
-    o->escalate_band = 0.25;
+    o->escalate_margin = 0.0;
+    o->escalate_band = 0.0;
     o->surrogate_band = 0.5;
     o->surrogate_pmin = 0.99;
     o->audit_fraction = 0.01;
@@ -326,7 +326,7 @@ void oracle_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->vol_r_max = 50.0;
     o->vol_tau_max = 10.0;
     o->escalate_disk_edge = 0.02;
-    o->escalate_rtol = 1e-8;
+    o->escalate_rtol = 0.0;
     o->critical_band = 0.5;
     if (precision == SIRIUS_PRECISION_FP32_ESCALATE) {
         o->rtol = 1e-6;

@@ -209,12 +209,15 @@ class HipAccelerator:
                                                          out.ctypes.data, 0))
         return out
 
-    # ---- surrogate network (device buffers are torch CUDA tensors: device-memory plumbing) ----
+    # ---- surrogate network (x, y, cls, dirs, grad: device buffers, given as integer pointers) ----
     def mlp_init(self, seed):
         self._check(self._L.sirius_hip_mlp_init(self._h, seed))
 
+    def mlp_param_count(self):
+        return int(self._L.sirius_hip_mlp_param_count())
+
     def mlp_get_params(self):
-        p = np.zeros(self._L.sirius_hip_mlp_param_count(), np.float32)
+        p = np.zeros(self.mlp_param_count(), np.float32)
         self._check(self._L.sirius_hip_mlp_get_params(self._h, p.ctypes.data, p.size))
         return p
 
@@ -222,52 +225,26 @@ class HipAccelerator:
         p = np.ascontiguousarray(p, np.float32)
         self._check(self._L.sirius_hip_mlp_set_params(self._h, p.ctypes.data, p.size))
 
-    @staticmethod
-    def _dev(t, dtype):
-        import torch
-        assert t.is_cuda and t.dtype == dtype and t.is_contiguous()
-        return t.data_ptr()
+    def mlp_forward(self, x_dev, y_dev, n):
+        self._check(self._L.sirius_hip_mlp_forward(self._h, x_dev, y_dev, n))
 
-    def mlp_forward(self, x):
-        """x: CUDA float32 tensor (n, 5) -> CUDA float32 tensor (n, 6)."""
-        import torch
-        y = torch.empty((x.shape[0], 6), dtype=torch.float32, device=x.device)
-        torch.cuda.synchronize()
-        self._check(self._L.sirius_hip_mlp_forward(self._h, self._dev(x, torch.float32), y.data_ptr(), x.shape[0]))
-        return y
-
-    def mlp_loss_grad(self, x, cls, dirs):
-        import torch
-        g = torch.empty(self._L.sirius_hip_mlp_param_count(), dtype=torch.float32, device=x.device)
+    def mlp_loss_grad(self, x_dev, cls_dev, dirs_dev, n, grad_dev):
         loss = C.c_float()
-        torch.cuda.synchronize()
-        self._check(self._L.sirius_hip_mlp_loss_grad(self._h, self._dev(x, torch.float32), self._dev(cls, torch.int32),
-                                                     self._dev(dirs, torch.float32), x.shape[0], g.data_ptr(),
-                                                     C.byref(loss)))
-        return loss.value, g
+        self._check(self._L.sirius_hip_mlp_loss_grad(self._h, x_dev, cls_dev, dirs_dev, n, grad_dev, C.byref(loss)))
+        return loss.value
 
-    def mlp_train_step(self, x, cls, dirs, lr):
-        import torch
+    def mlp_train_step(self, x_dev, cls_dev, dirs_dev, n, lr):
         loss = C.c_float()
-        torch.cuda.synchronize()
-        self._check(self._L.sirius_hip_mlp_train_step(self._h, self._dev(x, torch.float32), self._dev(cls, torch.int32),
-                                                      self._dev(dirs, torch.float32), x.shape[0], C.c_float(lr),
+        self._check(self._L.sirius_hip_mlp_train_step(self._h, x_dev, cls_dev, dirs_dev, n, C.c_float(lr),
                                                       C.byref(loss)))
         return loss.value
 
-    def surrogate_features(self, cams, metrics):
-        """(x, n0) CUDA float32 tensors (n, 5) and (n, 3): network inputs and initial directions."""
-        import torch
+    def surrogate_features(self, cams, metrics, x_dev, n0_dev):
+        """Network inputs x[n*5] and initial directions n0[n*3] of every ray (device buffers)."""
         cams, metrics = list(cams), list(metrics)
         ca = (abi.Camera * len(cams))(*cams)
         ma = (abi.Metric * len(metrics))(*metrics)
-        n = sum(c.width * c.height for c in cams)
-        x = torch.empty((n, 5), dtype=torch.float32, device="cuda")
-        n0 = torch.empty((n, 3), dtype=torch.float32, device="cuda")
-        torch.cuda.synchronize()
-        self._check(self._L.sirius_hip_surrogate_features(self._h, ca, len(cams), ma, len(metrics), x.data_ptr(),
-                                                          n0.data_ptr()))
-        return x, n0
+        self._check(self._L.sirius_hip_surrogate_features(self._h, ca, len(cams), ma, len(metrics), x_dev, n0_dev))
 
     def surrogate_train(self, cams, metrics, label_opts, epochs, batch, lr, seed):
         """Train the network on the cameras' rays (labels from the integrator); returns the last

@@ -85,6 +85,7 @@ struct DevOut {
     uint32_t* landings;
     float* intensity;
     double* state;  // optional, 8 per ray
+    double* vol_acc;  // volumetric emission: per-bin FP64 accumulators (bin-major), or null
     int64_t n_rays;
 };
 

@@ -130,7 +130,7 @@ int sirius_hip_launch(SiriusHipHandle hh, const SiriusHipCamera* cam, const Siri
     int rc = prepare_batch(*H, &c, 1, metric, 1, *opts, B);
     if (rc) return rc;
     DevOut dout;
-    if ((rc = bind_outputs(*H, nullptr, B.n, opts->n_bins, dout))) return rc;
+    if ((rc = bind_outputs(*H, nullptr, B.n, opts->n_bins, dout, opts->emission == SIRIUS_EMISSION_VOLUMETRIC))) return rc;
     rc = run_trace(*H, B, *opts, dout, nullptr, nullptr, 0, nullptr, false);
     if (rc) return rc;
     hipError_t e = launch_shade(dout, B.n, c.width, c.height, opts->n_bins, H->bg_dev, H->bg_w, H->bg_h, H->fb_dev,
@@ -227,7 +227,7 @@ int sirius_hip_autotune(SiriusHipHandle hh, const SiriusHipCamera* cams, int n_c
         return H->hip_fail(e, "upload");
     if ((e = hipMemcpy(cnt, &nn, 4, hipMemcpyHostToDevice)) != hipSuccess) return H->hip_fail(e, "upload");
     DevOut dout;
-    if ((rc = bind_outputs(*H, nullptr, B.n, opts->n_bins, dout))) return rc;
+    if ((rc = bind_outputs(*H, nullptr, B.n, opts->n_bins, dout, opts->emission == SIRIUS_EMISSION_VOLUMETRIC))) return rc;
     double best = 1e300;
     SiriusHipTraceOpts bo = *opts;
     const int blocks[3] = {64, 128, 256};

@@ -372,7 +372,7 @@ int prepare_batch(Handle& H, const SiriusHipCamera* cams, int n_cam, const Siriu
 }
 
 // Device output arrays: the caller's (on_device) or library scratch.
-int bind_outputs(Handle& H, const SiriusHipRayOutput* out, int64_t n, int n_bins, DevOut& o) {
+int bind_outputs(Handle& H, const SiriusHipRayOutput* out, int64_t n, int n_bins, DevOut& o, bool volumetric) {
     int rc = SIRIUS_HIP_OK;
     auto get = [&](const char* name, void* user, size_t bytes) -> void* {
         if (out && out->on_device && user) return user;
@@ -393,6 +393,8 @@ int bind_outputs(Handle& H, const SiriusHipRayOutput* out, int64_t n, int n_bins
     o.intensity = (float*)get("o_int", out ? out->intensity : nullptr, 4 * n * std::max(n_bins, 1));
     // the final state is always kept: the finish kernel derives every output from it
     o.state = (double*)get("o_state", out ? out->state : nullptr, 64 * n);
+    // volumetric emission accumulates in FP64 (the finish kernel rounds it into intensity)
+    o.vol_acc = volumetric ? (double*)H.scratch("o_vacc", 8 * n * std::max(n_bins, 1), &rc) : nullptr;
     return rc;
 }
 
@@ -413,6 +415,7 @@ static int bind_audit_outputs(Handle& H, int64_t n, int n_bins, DevOut& o) {
     o.landings = (uint32_t*)H.scratch("a_land", 4 * n, &rc);
     o.intensity = (float*)H.scratch("a_int", 4 * n * std::max(n_bins, 1), &rc);
     o.state = (double*)H.scratch("a_state", 64 * n, &rc);
+    o.vol_acc = nullptr;  // the audit serves thin-disk batches only
     return rc;
 }
 
@@ -539,9 +542,11 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
         // escalated rays: FP64 arithmetic, the path's step caps, the escalation tolerance
         TraceParams t64 = t32;
         t64.escalate = 0;
-        if (o.escalate_rtol > 0) {
-            t64.rtol = o.escalate_rtol;
-            t64.atol = o.atol * (o.escalate_rtol / o.rtol);
+        const double ertol = o.escalate_rtol > 0 ? o.escalate_rtol
+                                                 : (B.family == SIRIUS_METRIC_ALCUBIERRE ? std::min(o.rtol, 1e-8) : 0.0);
+        if (ertol > 0) {
+            t64.rtol = ertol;
+            t64.atol = o.atol * (ertol / o.rtol);
         }
         if (band) {
             // FP64 band on the second stream, small blocks so the long rays spread over the chip.
@@ -802,8 +807,8 @@ void sirius_hip_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->n_bins = 16;
     o->x_min = 0.1;
     o->x_max = 10.0;
-    o->escalate_margin = 1.0;
-    o->escalate_band = 0.25;
+    o->escalate_margin = 0.0;
+    o->escalate_band = 0.0;
     o->surrogate_band = 0.5;
     o->surrogate_pmin = 0.99;
     o->audit_fraction = 0.01;
@@ -815,7 +820,7 @@ void sirius_hip_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->vol_r_max = 50.0;
     o->vol_tau_max = 10.0;
     o->escalate_disk_edge = 0.02;
-    o->escalate_rtol = 1e-8;
+    o->escalate_rtol = 0.0;
     o->critical_band = 0.5;
     if (precision == SIRIUS_PRECISION_FP32_ESCALATE) {
         o->rtol = 1e-6;
@@ -836,7 +841,7 @@ int sirius_hip_trace_batch(SiriusHipHandle hh, const SiriusHipCamera* cams, int 
     Batch B;
     if ((rc = prepare_batch(*H, cams, n_cam, metrics, n_metric, *opts, B))) return rc;
     DevOut dout;
-    if ((rc = bind_outputs(*H, out, B.n, opts->n_bins, dout))) return rc;
+    if ((rc = bind_outputs(*H, out, B.n, opts->n_bins, dout, opts->emission == SIRIUS_EMISSION_VOLUMETRIC))) return rc;
     if (!H->last) H->last = new DevOut;
     *H->last = dout;
     H->last_bins = opts->n_bins;

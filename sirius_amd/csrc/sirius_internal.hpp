@@ -74,7 +74,7 @@ bool make_cam_const(const SiriusHipCamera& cam, const SiriusHipMetric& m, int64_
                     std::string& err);
 int prepare_batch(Handle& H, const SiriusHipCamera* cams, int n_cam, const SiriusHipMetric* metrics, int n_metric,
                   const SiriusHipTraceOpts& o, Batch& B);
-int bind_outputs(Handle& H, const SiriusHipRayOutput* out, int64_t n, int n_bins, DevOut& o);
+int bind_outputs(Handle& H, const SiriusHipRayOutput* out, int64_t n, int n_bins, DevOut& o, bool volumetric);
 int copy_outputs_to_host(Handle& H, const SiriusHipRayOutput* out, const DevOut& o, int n_bins);
 int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, const int32_t* list_dev,
               const uint32_t* list_count_dev, int64_t list_cap, SiriusHipStats* stats, bool time_it);

@@ -301,7 +301,8 @@ int sirius_hip_mlp_init(SiriusHipHandle hh, uint64_t seed) {
         }
     }
     hipError_t e;
-    if ((e = hipMemcpy(H->mlp->params, p.data(), p.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+    if ((e = hipStreamSynchronize(H->stream)) != hipSuccess ||
+        (e = hipMemcpy(H->mlp->params, p.data(), p.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
         return H->hip_fail(e, "params upload");
     return mlp_reset_adam(*H);
     MLP_GUARD_END(H)
@@ -312,7 +313,8 @@ int sirius_hip_mlp_get_params(SiriusHipHandle hh, float* host_params, size_t n) 
     Handle* H = static_cast<Handle*>(hh);
     if (!H->mlp || !H->mlp->ready) return H->fail(SIRIUS_HIP_E_STATE, "network not initialised");
     if (!host_params || n != mlp_param_count()) return H->fail(SIRIUS_HIP_E_INVALID, "parameter count");
-    hipError_t e = hipMemcpy(host_params, H->mlp->params, n * 4, hipMemcpyDeviceToHost);
+    hipError_t e = hipStreamSynchronize(H->stream);  // training steps run asynchronously on the stream
+    if (e == hipSuccess) e = hipMemcpy(host_params, H->mlp->params, n * 4, hipMemcpyDeviceToHost);
     return e == hipSuccess ? SIRIUS_HIP_OK : H->hip_fail(e, "params download");
 }
 
@@ -323,7 +325,8 @@ int sirius_hip_mlp_set_params(SiriusHipHandle hh, const float* host_params, size
     if (!host_params || n != mlp_param_count()) return H->fail(SIRIUS_HIP_E_INVALID, "parameter count");
     int rc = mlp_alloc(*H);
     if (rc) return rc;
-    hipError_t e = hipMemcpy(H->mlp->params, host_params, n * 4, hipMemcpyHostToDevice);
+    hipError_t e = hipStreamSynchronize(H->stream);
+    if (e == hipSuccess) e = hipMemcpy(H->mlp->params, host_params, n * 4, hipMemcpyHostToDevice);
     if (e != hipSuccess) return H->hip_fail(e, "params upload");
     return mlp_reset_adam(*H);
     MLP_GUARD_END(H)
@@ -403,7 +406,7 @@ int sirius_hip_surrogate_train(SiriusHipHandle hh, const SiriusHipCamera* cams, 
     if ((rc = prepare_batch(*H, cams, n_cam, metrics, n_metric, *label_opts, B))) return rc;
     if (B.family != SIRIUS_METRIC_KERR_SCHILD) return H->fail(SIRIUS_HIP_E_UNSUPPORTED, "Kerr-Schild cameras only");
     DevOut dout;
-    if ((rc = bind_outputs(*H, nullptr, B.n, label_opts->n_bins, dout))) return rc;
+    if ((rc = bind_outputs(*H, nullptr, B.n, label_opts->n_bins, dout, label_opts->emission == SIRIUS_EMISSION_VOLUMETRIC))) return rc;
     if ((rc = run_trace(*H, B, *label_opts, dout, nullptr, nullptr, 0, nullptr, false))) return rc;
     const int64_t n = B.n;
     float* x = (float*)H->scratch("tr_x", (size_t)n * 5 * 4, &rc);

@@ -100,7 +100,7 @@ SIRIUS_HD __forceinline__ void ray_start(Ray<T, FAM>& R, const CamConst* cams, i
     R.attempts = R.rejected = R.landings = 0;
     R.flags = tp.gate_flags ? (o.flags[id] & SIRIUS_FLAG_CRITICAL) : 0;
     if constexpr (EMIT == SIRIUS_EMISSION_VOLUMETRIC)
-        for (int k = 0; k < tp.n_bins; ++k) o.intensity[(int64_t)k * o.n_rays + id] = 0.f;
+        for (int k = 0; k < tp.n_bins; ++k) o.vol_acc[(int64_t)k * o.n_rays + id] = 0.0;
 }
 
 // A finished ray leaves its class, counters and final state (x^mu, p_mu in FP64) in the
@@ -166,7 +166,10 @@ SIRIUS_HD __forceinline__ void ray_outputs(const CamConst& c, const TraceParams&
         } else {
             for (int k = 0; k < tp.n_bins; ++k) o.intensity[(int64_t)k * o.n_rays + id] = 0.f;
         }
-    } else if constexpr (EMIT != SIRIUS_EMISSION_VOLUMETRIC) {
+    } else if constexpr (EMIT == SIRIUS_EMISSION_VOLUMETRIC) {
+        for (int k = 0; k < tp.n_bins; ++k)
+            o.intensity[(int64_t)k * o.n_rays + id] = (float)o.vol_acc[(int64_t)k * o.n_rays + id];
+    } else {
         for (int k = 0; k < tp.n_bins; ++k) o.intensity[(int64_t)k * o.n_rays + id] = 0.f;
     }
     o.theta[id] = th;
@@ -228,7 +231,7 @@ SIRIUS_HD __forceinline__ void vol_accumulate(Ray<T, FAM>& R, const CamConst& c,
         for (int k = 0; k < tp.n_bins; ++k) {
             double xk = tp.x_min * exp(lr * (k + 0.5) / tp.n_bins);
             double xe = xk / gf;
-            o.intensity[(int64_t)k * o.n_rays + R.id] += (float)(w * xe * xe * xe / expm1(xe / tv));
+            o.vol_acc[(int64_t)k * o.n_rays + R.id] += w * xe * xe * xe / expm1(xe / tv);
         }
         R.tau += T(dtau);
     }

@@ -1,5 +1,5 @@
 """GPU parity of the surrogate network (BF16 MFMA forward, backward, Adam) against the network
-oracle, through the C-ABI. Device buffers are torch CUDA tensors (memory plumbing only).
+oracle, through the C-ABI (device buffers from hipMalloc, tests/devmem.py).
 
 Bars: with the oracle rounding GEMM operands to BF16 where the device does, outputs agree to
 2e-3 of the output scale and every layer's gradient to 2e-2 in relative L2 norm (a BF16
@@ -9,6 +9,7 @@ ulp, 2^-8); against the unrounded FP64 network the outputs agree to the BF16 lev
 import numpy as np
 import pytest
 
+from devmem import DeviceArray
 from oracle import oracle_py as O
 
 pytestmark = pytest.mark.gpu
@@ -44,7 +45,7 @@ def _batch(n, seed):
 
 
 def test_param_layout_round_trip(acc):
-    assert acc._L.sirius_hip_mlp_param_count() == O.mlp_param_count()
+    assert acc.mlp_param_count() == O.mlp_param_count()
     p = _params(1)
     acc.mlp_set_params(p)
     assert np.array_equal(acc.mlp_get_params(), p)
@@ -62,12 +63,13 @@ def test_param_layout_round_trip(acc):
 
 @pytest.mark.parametrize("n", [1, 63, 1000])
 def test_forward_matches_oracle(acc, n):
-    import torch
     p = _params(2)
     acc.mlp_set_params(p)
     x, _, _ = _batch(n, 3)
     print(f"[case] mlp forward n={n}", flush=True)
-    y = acc.mlp_forward(torch.from_numpy(x).cuda()).cpu().numpy()
+    xd, yd = DeviceArray(x), DeviceArray(shape=(n, 6), dtype=np.float32)
+    acc.mlp_forward(xd.data_ptr(), yd.data_ptr(), n)
+    y = yd.get()
     ref = O.mlp_forward(p, x, round_bf16=True)
     exact = O.mlp_forward(p, x, round_bf16=False)
     scale = np.abs(exact).max()
@@ -76,13 +78,14 @@ def test_forward_matches_oracle(acc, n):
 
 
 def test_gradient_matches_oracle(acc):
-    import torch
     p = _params(4)
     acc.mlp_set_params(p)
     x, cls, d = _batch(777, 5)
     print("[case] mlp loss_grad n=777", flush=True)
-    loss, g = acc.mlp_loss_grad(torch.from_numpy(x).cuda(), torch.from_numpy(cls).cuda(), torch.from_numpy(d).cuda())
-    g = g.cpu().numpy().astype(np.float64)
+    xd, cd, dd = DeviceArray(x), DeviceArray(cls), DeviceArray(d)
+    gd = DeviceArray(shape=(len(p),), dtype=np.float32)
+    loss = acc.mlp_loss_grad(xd.data_ptr(), cd.data_ptr(), dd.data_ptr(), len(x), gd.data_ptr())
+    g = gd.get().astype(np.float64)
     ref_loss, ref = O.mlp_loss_grad(p, x, cls, d, round_bf16=True)
     assert abs(loss - ref_loss) < 1e-3 * abs(ref_loss)
     for name, sl in _slices():
@@ -93,18 +96,18 @@ def test_gradient_matches_oracle(acc):
 def test_training_reduces_loss_and_matches_oracle_step(acc):
     """Adam on the GPU fits a smooth synthetic target; the first step's update equals the
     oracle gradient's Adam step (lr * sign at step 1, up to BF16 gradient error)."""
-    import torch
     rng = np.random.default_rng(8)
     n = 4096
     x = rng.uniform(-1, 1, (n, 5)).astype(np.float32)
     cls = np.where(x[:, 3] ** 2 + x[:, 4] ** 2 < 0.2, 1, np.where(x[:, 4] > 0.5, 2, 0)).astype(np.int32)
     d = np.stack([0.3 * np.sin(2 * x[:, 3]), 0.2 * x[:, 4] * x[:, 0], 0.1 * np.cos(x[:, 1] + x[:, 3])], 1).astype(np.float32)
-    xt, ct, dt = (torch.from_numpy(v).cuda() for v in (x, cls, d))
+    xd, cd, dd = DeviceArray(x), DeviceArray(cls), DeviceArray(d)
+    args = (xd.data_ptr(), cd.data_ptr(), dd.data_ptr(), n)
     acc.mlp_init(99)
     p0 = acc.mlp_get_params()
     lr = 1e-3
     print("[case] mlp train 4096", flush=True)
-    l0 = acc.mlp_train_step(xt, ct, dt, lr)
+    l0 = acc.mlp_train_step(*args, lr)
     acc.synchronise()
     p1 = acc.mlp_get_params()
     _, g = O.mlp_loss_grad(p0, x, cls, d, round_bf16=True)
@@ -113,5 +116,5 @@ def test_training_reduces_loss_and_matches_oracle_step(acc):
     assert np.mean(np.abs(step[big] - lr * np.sign(g[big])) < 1e-2 * lr) > 0.99
     losses = [l0]
     for _ in range(300):
-        losses.append(acc.mlp_train_step(xt, ct, dt, lr))
+        losses.append(acc.mlp_train_step(*args, lr))
     assert losses[-1] < 0.1 * losses[0], (losses[0], losses[-1])

@@ -121,6 +121,7 @@ def test_escalated_rays_are_bitwise_fp64(acc):
     m = abi.metric_ks(1.0, 0.9)
     cam = abi.camera(50.0, 75.0, 20.0, 480, 270, disk=(0.0, 30.0))
     o = O.default_opts(abi.PRECISION_FP32_ESCALATE)
+    o.escalate_band, o.escalate_margin, o.escalate_rtol = 0.25, 1.0, 1e-8  # every criterion on
     got, st = acc.trace_batch([cam], [m], o)
     o64 = O.default_opts(abi.PRECISION_FP32_ESCALATE)
     o64.precision = abi.PRECISION_FP64
