@@ -10,6 +10,10 @@ One step = one full frame traced on the GPU with every result left in device mem
                  FP32 vector peak
   cpu_baseline   the reference's own CPU integrator (oracle/_ref, compiled from its sources)
                  on every pixel of the frame, on this host's cores (about 15 s)
+  surrogate      ablation: the FP32 path with the MLP surrogate answering far-field rays, after
+                 a one-time training on 2^20 integrator-labelled rays of 256 random cameras
+                 (training excluded from the frame time, reported apart), with its accuracy on
+                 the rays it answered
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -74,6 +78,47 @@ def accuracy(fast, ref, crit, band=0.5):
     return out
 
 
+def surrogate_ablation(acc, cam, metric, ref, steps, warmup):
+    """Train the network once, then time the frame with the surrogate on and measure the
+    answered rays against the FP64 frame."""
+    from oracle import oracle_py as O
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from workloads import MT19937_64
+    from sirius_amd import abi
+    from sirius_amd.accelerator import default_opts
+    g = MT19937_64(0x51121006)
+    cams, metrics = [], []
+    for i in range(256):
+        a, incl, r_obs = g.uniform(0.0, 0.99), g.uniform(5.0, 89.0), g.uniform(20.0, 100.0)
+        metrics.append(abi.metric_ks(1.0, a))
+        cams.append(abi.camera(r_obs, incl, 20.0, 64, 64, disk=(0.0, 30.0), metric_index=i))
+    lab = O.default_opts(abi.PRECISION_FP64)
+    lab.far_slope = 0.25
+    acc.mlp_init(0x51121006)
+    t0 = time.perf_counter()
+    loss = acc.surrogate_train(cams, metrics, lab, 12, 8192, 2e-3, 0x51121006)
+    train_s = time.perf_counter() - t0
+    o = default_opts(abi.PRECISION_FP32_ESCALATE)
+    o.use_surrogate = 1
+    ms, st = time_frames(acc, cam, metric, o, steps, warmup)
+    got = acc.fetch_results()
+    sur = (got.flags & abi.FLAG_SURROGATE) != 0
+    out = {"ms_per_step": ms * 1e3, "value": W * H / ms / 1e6, "answered_fraction": float(sur.mean()),
+           "train_s": train_s, "train_rays": 256 * 64 * 64, "train_final_loss": loss,
+           "surrogate_ms": float(np.mean([x["surrogate_ms"] for x in st])),
+           "audit_fraction": o.audit_fraction, "audit_ms": float(np.mean([x["audit_ms"] for x in st])),
+           "audit_dir_err_max_rad": st[-1]["audit_dir_err_max"]}
+    if sur.any():
+        out["class_agreement_answered"] = float((got.cls[sur] == ref.cls[sur]).mean())
+        esc = sur & (got.cls == 0) & (ref.cls == 0)
+        if esc.any():
+            u, v = got.direction()[esc], ref.direction()[esc]
+            ang = np.arccos(np.clip((u * v).sum(1) / np.linalg.norm(u, axis=1) / np.linalg.norm(v, axis=1), -1, 1))
+            out["escape_dir_err_median_rad"] = float(np.median(ang))
+            out["escape_dir_err_max_rad"] = float(ang.max())
+    return out
+
+
 def cpu_baseline(n_threads):
     """The reference's own integrator (oracle/_ref) when present, else the FP64 port, on every
     pixel of the frame."""
@@ -107,6 +152,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this backend runs on one GPU; --gpus {args.gpus} is not supported "
@@ -148,13 +194,14 @@ def main():
         "config": {"workload": "kerr_a0.9_1080p_dp45_fp32_escalate", "width": W, "height": H, "a": 0.9,
                    "r_obs": 50.0, "inclination_deg": 75.0, "fov_deg": 20.0, "disk": "ISCO..30M",
                    "n_bins": 16, "rtol": o32.rtol, "far_slope": o32.far_slope,
-                   "escalate_margin": o32.escalate_margin, "surrogate": False},
+                   "escalate_band": o32.escalate_band, "escalate_margin": o32.escalate_margin,
+                   "escalate_disk_edge": o32.escalate_disk_edge, "critical_band": o32.critical_band,
+                   "surrogate": False},
         "roofline": {"bound": "valu", "kernel": "k_trace_persistent<float,0>", "achieved": achieved,
                      "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_VALU_PEAK_TFLOPS,
                      "traffic": None, "flops_per_launch": s["flops"], "avg_launch_ms": kern_ms},
         "geodesic_steps_per_s": s["steps_bulk"] / (kern_ms * 1e-3),
         "rays_escalated": s["rays_escalated"],
-        "escalate_band": o32.escalate_band,
         "escalate_ms": float(np.mean([x["escalate_ms"] for x in st32])),
         "fp64_plain": {"value": n_rays / ms64 / 1e6, "unit": "Mrays/s", "ms_per_step": ms64 * 1e3,
                        "kernel_ms": k64, "steps_per_ray": s64["steps_bulk"] / n_rays,
@@ -166,6 +213,8 @@ def main():
     if not args.no_accuracy:
         crit = acc.critical_distance([cam], [metric])
         result["accuracy_vs_fp64"] = accuracy(fast, ref, crit)
+    if not args.no_surrogate and ref is not None:
+        result["surrogate"] = surrogate_ablation(acc, cam, metric, ref, args.steps, args.warmup)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(min(16, os.cpu_count() or 1))
     acc.close()

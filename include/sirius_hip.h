@@ -48,6 +48,11 @@ enum {
 /* ---- enums ---------------------------------------------------------------------------- */
 enum { SIRIUS_METRIC_KERR_SCHILD = 0, SIRIUS_METRIC_MORRIS_THORNE = 1, SIRIUS_METRIC_ALCUBIERRE = 2 };
 enum { SIRIUS_INTEGRATOR_DP45 = 0, SIRIUS_INTEGRATOR_SYMPLECTIC = 1 };
+/* FP32_ESCALATE: FP32 stepping with FP64 null-constraint re-normalisation; a ray is re-traced
+ * in FP64 from its pixel when it meets an escalation criterion: Kerr-Schild - the options
+ * escalate_band, escalate_margin, escalate_disk_edge; Morris-Thorne - impact parameter within
+ * 10 % of the throat radius; Alcubierre - entering the bubble wall (r_s < R + 3 / sigma).
+ * With any criterion on (the default) these family criteria apply too. */
 enum { SIRIUS_PRECISION_FP64 = 0, SIRIUS_PRECISION_FP32_ESCALATE = 1 };
 enum { SIRIUS_EMISSION_NONE = 0, SIRIUS_EMISSION_THIN_DISK = 1, SIRIUS_EMISSION_VOLUMETRIC = 2 };
 enum { SIRIUS_SCHEDULE_GRID = 0, SIRIUS_SCHEDULE_PERSISTENT = 1 };
@@ -100,7 +105,8 @@ typedef struct SiriusHipTraceOpts {
     int32_t schedule;        /* SIRIUS_SCHEDULE_*                                           */
     double rtol, atol;       /* DP45 tolerances (IntegratorConfig, PHGD001A.h:48-58)        */
     double h_init, h_min, h_max;
-    double far_slope;        /* FP32 path: h_max(r) = max(h_max, far_slope * r); 0 = off    */
+    double far_slope;        /* FP32 path: h_max(r) = max(h_max, far_slope * r); 0 = off.
+                              * When on, a ray's first step is max(h_init, h_max(r_start))    */
     double symplectic_h;     /* fixed step of the symplectic integrator                     */
     int32_t symplectic_order;/* 2 or 4 (Yoshida composition)                                */
     int32_t max_steps;       /* step attempts per ray                                       */
@@ -124,8 +130,9 @@ typedef struct SiriusHipTraceOpts {
     int32_t reserved0;
     /* volumetric emission (emitter frame): j_nu = rho(x) x^3 / (e^{x / tau_v(r)} - 1),
      * rho = j0 r^-2 exp(-z^2 / (2 (h_over_r r)^2)) W(r), tau_v = (6 / r)^{3/4},
-     * alpha = kappa rho; W is a C1 smoothstep window, 0 at vol_r_min and vol_r_max and 1 on
-     * [1.1 vol_r_min, 0.9 vol_r_max] */
+     * alpha = kappa rho; W is a C1 smoothstep window, 0 at r_lo and vol_r_max and 1 on
+     * [1.1 r_lo, 0.9 vol_r_max], r_lo = max(vol_r_min, the camera's disk r_in, by default the
+     * ISCO); emitters are on Keplerian (Kerr-Newman) circles */
     double vol_j0, vol_h_over_r, vol_kappa, vol_r_min, vol_r_max, vol_tau_max;
     double escalate_disk_edge; /* FP32 path, thin disk: a disk crossing whose radius lies within
                               * this fraction of r_in (|r/r_in - 1| < edge) is re-traced in FP64,

@@ -765,7 +765,9 @@ static double vol_window(double r, double r_min, double r_max) {
 /* Volumetric transfer over one accepted step, 3-node Gauss-Legendre on the Hermite
  * interpolant. Invariant form (I/nu^3, j/nu^2, nu alpha): with g = nu_obs/nu_emit,
  *   dtau = kappa rho nu_e dlam,  dI_k = rho S(x_k / g) (nu_o^3 / nu_e^2) e^{-tau} dlam,
- * rho = j0 r^-2 exp(-z^2/(2 sigma^2)) W(r) (W: the shell window above),
+ * rho = j0 r^-2 exp(-z^2/(2 sigma^2)) W(r) (W: the shell window above, from max(r_min, r_in) to
+ * r_max: inside the ISCO (r_in by default) a Keplerian emitter is unstable and, nearer the
+ * photon orbit, ultra-relativistic, so the emitting shell starts there),
  * S(x) = x^3/(e^{x/tau_v} - 1), tau_v = (6/r)^{3/4},
  * emitter on a Keplerian (Kerr-Newman) circle at r_c = sqrt(x^2 + y^2 - a^2). This is the
  * build's definition of the volumetric case; the reference's gray straight-chord march
@@ -782,10 +784,11 @@ static void vol_accumulate(const cam_ctx* cx, const SiriusHipTraceOpts* o, const
         const double* x = y8;
         const double* p = y8 + 4;
         double r = oracle_kerr_radius(m, x[1], x[2], x[3]);
-        if (r < o->vol_r_min || r > o->vol_r_max) continue;
+        const double r_lo = cx->r_in > o->vol_r_min ? cx->r_in : o->vol_r_min; /* no emitters inside the ISCO */
+        if (r < r_lo || r > o->vol_r_max) continue;
         double sig = o->vol_h_over_r * r;
         double rho = o->vol_j0 / (r * r) * exp(-x[3] * x[3] / (2 * sig * sig)) *
-                     vol_window(r, o->vol_r_min, o->vol_r_max);
+                     vol_window(r, r_lo, o->vol_r_max);
         double M = m->M, a = m->a, Q2 = m->Q * m->Q;
         double rc = sqrt(fmax(x[1] * x[1] + x[2] * x[2] - a * a, 1e-12));
         double sq = sqrt(fmax(M * rc - Q2, 0.0));
@@ -828,6 +831,12 @@ static void trace_dp45(const cam_ctx* cx, const SiriusHipTraceOpts* o, const st8
     der8 d0, d1;
     eval_der(m, &s0, &d0);
     double h = o->h_init, lam = 0, err4[4];
+    /* with a far-field cap the first step starts at the cap (the FP32 path's policy); the
+     * reference's h0 = 0.01 would spend ~10 doubling steps reaching it from the observer */
+    if (o->far_slope > 0) {
+        double cap = step_cap(m, o, s0.y);
+        if (cap > h) h = cap;
+    }
     double tau = 0;
     memset(R, 0, sizeof(*R));
     R->cls = SIRIUS_RAY_MAX_STEPS;

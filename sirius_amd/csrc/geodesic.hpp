@@ -61,6 +61,10 @@ struct TraceParams {
     int32_t max_steps, n_bins, emission, sym_order;
     int32_t n_cam, escalate, gate_flags, pad1;  // gate_flags: the gate wrote per-ray flags
     int64_t n_rays;
+    // persistent queue: rays listed here are taken before the kernel's own list (the rays
+    // nearest the critical curve, the longest, start first); null for none
+    const int32_t* first_list;
+    const uint32_t* first_count;
 };
 
 // Finite test usable on host and device: x - x is NaN exactly when x is NaN or infinite.

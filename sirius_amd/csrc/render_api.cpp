@@ -164,6 +164,7 @@ int sirius_hip_upload_background(SiriusHipHandle hh, const unsigned char* rgba, 
     if (!rgba || width <= 0 || height <= 0) return H->fail(SIRIUS_HIP_E_INVALID, "background image");
     int rc = H->ensure_device();
     if (rc) return rc;
+    (void)hipStreamSynchronize(H->stream);  // a shading launch may still read the old image
     if (H->bg_dev) (void)hipFree(H->bg_dev);
     H->bg_dev = nullptr;
     size_t bytes = (size_t)width * height * 4;
@@ -193,8 +194,13 @@ int sirius_hip_critical_distance(SiriusHipHandle hh, const SiriusHipCamera* cams
     if (rc) return rc;
     if ((rc = critical_distance_dev(*H, B, d))) return rc;
     if (!on_device) {
-        hipError_t e = hipMemcpy(out, d, 4 * (size_t)B.n, hipMemcpyDeviceToHost);
+        // the kernel ran on the library's non-blocking stream: copy on that stream, then wait
+        hipError_t e = hipMemcpyAsync(out, d, 4 * (size_t)B.n, hipMemcpyDeviceToHost, H->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(H->stream);
         if (e != hipSuccess) return H->hip_fail(e, "copy");
+    } else {
+        hipError_t e = hipStreamSynchronize(H->stream);
+        if (e != hipSuccess) return H->hip_fail(e, "synchronize");
     }
     return SIRIUS_HIP_OK;
     GUARD_END(H)
@@ -223,6 +229,7 @@ int sirius_hip_autotune(SiriusHipHandle hh, const SiriusHipCamera* cams, int n_c
     if (rc) return rc;
     uint32_t nn = (uint32_t)ids.size();
     hipError_t e;
+    if ((e = hipStreamSynchronize(H->stream)) != hipSuccess) return H->hip_fail(e, "synchronize");
     if ((e = hipMemcpy(list, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice)) != hipSuccess)
         return H->hip_fail(e, "upload");
     if ((e = hipMemcpy(cnt, &nn, 4, hipMemcpyHostToDevice)) != hipSuccess) return H->hip_fail(e, "upload");

@@ -43,7 +43,8 @@ hipError_t launch_reduce_stats(const DevOut& o, int64_t n, unsigned long long* a
 hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float* tabs, int64_t n, float* out,
                                     hipStream_t st);
 hipError_t launch_gate(const CamConst* cams, int n_cam, const float* tabs, int64_t n, double band, double crit_band,
-                       int32_t* list_a, int32_t* list_b, uint32_t* counts, uint8_t* flags, hipStream_t st);
+                       double long_band, int32_t* list_a, int32_t* list_b, int32_t* list_l, uint32_t* counts,
+                       uint8_t* flags, hipStream_t st);
 hipError_t launch_shade(const DevOut& o, int64_t n, int width, int height, int n_bins, const uint8_t* bg, int bgw,
                         int bgh, float* fb, hipStream_t st);
 hipError_t launch_sur_audit(const CamConst* cams, const TraceParams& tp, const DevOut& oa, const DevOut& o,
@@ -351,6 +352,8 @@ int prepare_batch(Handle& H, const SiriusHipCamera* cams, int n_cam, const Siriu
     tp.vol_r_min = o.vol_r_min; tp.vol_r_max = o.vol_r_max; tp.vol_tau_max = o.vol_tau_max;
     tp.esc_edge = o.escalate_disk_edge;
     tp.gate_flags = 0;
+    tp.first_list = nullptr;
+    tp.first_count = nullptr;
     tp.max_steps = o.max_steps;
     tp.n_bins = o.n_bins;
     tp.emission = B.family == SIRIUS_METRIC_KERR_SCHILD ? o.emission : SIRIUS_EMISSION_NONE;
@@ -459,11 +462,12 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
     bool persistent = o.schedule == SIRIUS_SCHEDULE_PERSISTENT && o.integrator == SIRIUS_INTEGRATOR_DP45;
     // work[0] bulk queue head, work[1] escalation count, work[2..3] bulk attempts (u64),
     // work[4] escalation queue head, work[5] unused, work[6..7] escalation attempts (u64),
-    // work[8..9] gate counts (A, B), work[12] band queue head, work[13] unused,
-    // work[14..15] band attempts (u64)
-    uint32_t* work = (uint32_t*)H.scratch("work", 64, &rc);
+    // work[8..10] surrogate list counts (A, B, audit), work[11] audit queue head,
+    // work[12] band queue head, work[13] unused, work[14..15] band attempts (u64),
+    // work[16..18] gate counts (A, B, L)
+    uint32_t* work = (uint32_t*)H.scratch("work", 128, &rc);
     if (rc) return rc;
-    if ((e = launch_fill(work, 16, 0u, H.stream)) != hipSuccess) return H.hip_fail(e, "fill");
+    if ((e = launch_fill(work, 32, 0u, H.stream)) != hipSuccess) return H.hip_fail(e, "fill");
     H.last_band = false;
     if (time_it) (void)hipEventRecord(H.ev0, H.stream);
     if (o.integrator == SIRIUS_INTEGRATOR_SYMPLECTIC) {
@@ -530,18 +534,26 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
             float* tabs = upload_critical_tables(H, B, &rc);
             int32_t* la = (int32_t*)H.scratch("gate_a", 4 * (size_t)B.n, &rc);
             list_b = (int32_t*)H.scratch("gate_b", 4 * (size_t)B.n, &rc);
+            int32_t* ll = (int32_t*)H.scratch("gate_l", 4 * (size_t)B.n, &rc);
             if (rc) return rc;
-            uint32_t* counts = work + 8;
+            uint32_t* counts = work + 16;
+            // rays within 1 M of the critical curve are queued first (longest first)
+            const double long_band = 1.0;
             if ((e = launch_gate(B.cams_dev, (int)B.cc.size(), tabs, B.n, std::max(o.escalate_band, 0.0),
-                                 std::max(o.critical_band, 0.0), la, list_b, counts, dout.flags, H.stream)) != hipSuccess)
+                                 std::max(o.critical_band, 0.0), long_band, la, list_b, ll, counts, dout.flags,
+                                 H.stream)) != hipSuccess)
                 return H.hip_fail(e, "gate launch");
             list_a = la;
             count_a = counts;
             cap_a = B.n;
+            t32.first_list = ll;
+            t32.first_count = counts + 2;
         }
         // escalated rays: FP64 arithmetic, the path's step caps, the escalation tolerance
         TraceParams t64 = t32;
         t64.escalate = 0;
+        t64.first_list = nullptr;  // the FP64 kernels trace only their own lists
+        t64.first_count = nullptr;
         const double ertol = o.escalate_rtol > 0 ? o.escalate_rtol
                                                  : (B.family == SIRIUS_METRIC_ALCUBIERRE ? std::min(o.rtol, 1e-8) : 0.0);
         if (ertol > 0) {
@@ -557,7 +569,7 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
             int bpc64 = persistent_blocks_per_cu(true, B.family, B.tp.emission, B.tp.n_cam > 1, 64);
             (void)hipEventRecord(H.ev_band0, H.stream2);
             if ((e = launch_trace_persistent(true, B.family, B.tp.emission, B.tp.n_cam > 1, B.cams_dev, t64, dout, list_b,
-                                             work + 9, B.n, work + 12, work + 13, nullptr,
+                                             sur ? work + 9 : work + 17, B.n, work + 12, work + 13, nullptr,
                                              (unsigned long long*)(work + 14), std::max(1, bpc64 * H.cu_count), 64, 1,
                                              SIRIUS_FLAG_ESCALATED, H.stream2)) != hipSuccess)
                 return H.hip_fail(e, "band launch");
@@ -622,7 +634,7 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
         unsigned long long h[8] = {0};
         if ((e = hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, H.stream)) != hipSuccess)
             return H.hip_fail(e, "stats copy");
-        uint32_t wk[16] = {0};
+        uint32_t wk[32] = {0};
         if (persistent) {
             if ((e = hipMemcpyAsync(wk, H.bufs["work"].ptr, sizeof(wk), hipMemcpyDeviceToHost, H.stream)) != hipSuccess)
                 return H.hip_fail(e, "stats copy");

@@ -94,6 +94,13 @@ SIRIUS_HD __forceinline__ void ray_start(Ray<T, FAM>& R, const CamConst* cams, i
     init_state<T>(c, s, (int)(pix % c.width), (int)(pix / c.width), R.x, R.p);
     eval<T, FAM>(s, R.x, R.p, R.d);
     R.h = T(tp.h_init);
+    if (tp.far_slope > 0) {
+        // the first step starts at the far-field cap (with h0 = 0.01 a ray would spend ~10
+        // doubling steps reaching it from the observer)
+        T cap = event_radius(s, R.x) * T(tp.far_slope);
+        if (cap < T(tp.h_max)) cap = T(tp.h_max);
+        if (cap > R.h) R.h = cap;
+    }
     R.lam = T(0);
     R.tau = T(0);
     R.vr_prev = T(-1);
@@ -202,10 +209,11 @@ SIRIUS_HD __forceinline__ void vol_accumulate(Ray<T, FAM>& R, const CamConst& c,
         T xq[4];
         hermite(R.x, R.d.k, x1, d1.k, R.h, gn[q], xq);
         T r = kerr_r(s, xq[1], xq[2], xq[3]);
-        if (r < T(tp.vol_r_min) || r > T(tp.vol_r_max)) continue;
+        const T r_lo = T(c.r_in > tp.vol_r_min ? c.r_in : tp.vol_r_min);  // no emitters inside the ISCO
+        if (r < r_lo || r > T(tp.vol_r_max)) continue;
         T sig = T(tp.vol_h_over_r) * r;
         T rho = T(tp.vol_j0) / (r * r) * exp(-xq[3] * xq[3] / (T(2) * sig * sig)) *
-                vol_window(r, T(tp.vol_r_min), T(tp.vol_r_max));
+                vol_window(r, r_lo, T(tp.vol_r_max));
         // emitter velocity and frequencies in FP64; p by cubic Hermite with dp/dlam
         double th = (double)gn[q], t2 = th * th, t3 = t2 * th, hh = (double)R.h;
         double h00 = 2 * t3 - 3 * t2 + 1, h10 = t3 - 2 * t2 + th, h01 = 3 * t2 - 2 * t3, h11 = t3 - t2;
@@ -384,6 +392,10 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
         // rays crossing the wall are deflected by up to ~2 rad and their outcome is sensitive
         // far beyond what the FP32 path's tolerance controls
         if (FAM == 2 && tp.escalate && r1 < T(c.wR + 3.0 / c.wsig)) return ST_ESCALATE;
+        // escalation of a wormhole ray whose impact parameter L = p_psi lies within 10 % of the
+        // throat radius: it winds about the throat's unstable photon orbit, where FP32 round-off
+        // grows exponentially with each turn
+        if (FAM == 1 && tp.escalate && fabs(R.p[2] - T(c.b0)) < T(0.1 * c.b0)) return ST_ESCALATE;
         // escalation: a radial turning point inside the photon region
         if (FAM == 0 && tp.escalate) {
             T vr = x1[1] * d1.k[1] + x1[2] * d1.k[2] + x1[3] * d1.k[3];
@@ -451,7 +463,7 @@ __global__ __launch_bounds__(256) void k_trace_grid(const CamConst* __restrict__
 // (count in *esc_count); each lane adds the attempts of every ray it traced to
 // *attempts_out when the wave retires.
 template <class T, int FAM, int EMIT, bool MULTI>
-__global__ __launch_bounds__(256) void k_trace_persistent(const CamConst* __restrict__ cams, TraceParams tp, DevOut o,
+__device__ __forceinline__ void persistent_body(const CamConst* __restrict__ cams, TraceParams tp, DevOut o,
                                                           const int32_t* __restrict__ ray_list,
                                                           const uint32_t* __restrict__ list_count, int64_t n,
                                                           uint32_t* __restrict__ head, uint32_t* __restrict__ esc_count,
@@ -459,8 +471,11 @@ __global__ __launch_bounds__(256) void k_trace_persistent(const CamConst* __rest
                                                           unsigned long long* __restrict__ attempts_out,
                                                           int refill_min, uint32_t flag_or) {
     const int lane = __lane_id();
-    int64_t total = list_count ? (int64_t)*list_count : n;
-    if (total > n) total = n;
+    int64_t main_total = list_count ? (int64_t)*list_count : n;
+    if (main_total > n) main_total = n;
+    int64_t first_total = tp.first_count ? (int64_t)*tp.first_count : 0;
+    if (first_total > n) first_total = n;
+    const int64_t total = main_total + first_total;
     Ray<T, FAM> R;
     Spacetime<T> s;
     if (!MULTI) s = make_spacetime<T>(cams[0]);
@@ -481,7 +496,8 @@ __global__ __launch_bounds__(256) void k_trace_persistent(const CamConst* __rest
                 unsigned long long below = lane ? (idle & ((1ull << lane) - 1ull)) : 0ull;
                 int64_t slot = (int64_t)base + __popcll(below);
                 if (slot < total) {
-                    int64_t id = ray_list ? (int64_t)ray_list[slot] : slot;
+                    int64_t id = slot < first_total ? (int64_t)tp.first_list[slot]
+                                                    : (ray_list ? (int64_t)ray_list[slot - first_total] : slot - first_total);
                     int cam = MULTI ? find_camera(cams, tp.n_cam, id) : 0;
                     ray_start<T, FAM, EMIT>(R, cams, cam, id, tp, o);
                     R.flags |= flag_or;
@@ -506,6 +522,36 @@ __global__ __launch_bounds__(256) void k_trace_persistent(const CamConst* __rest
     }
     for (int off = 32; off > 0; off >>= 1) my_attempts += __shfl_xor(my_attempts, off);
     if (lane == 0) atomicAdd(attempts_out, my_attempts);
+}
+
+
+#define SIRIUS_PERSISTENT_PARAMS                                                                           \
+    const CamConst *__restrict__ cams, TraceParams tp, DevOut o, const int32_t *__restrict__ ray_list,     \
+        const uint32_t *__restrict__ list_count, int64_t n, uint32_t *__restrict__ head,                   \
+        uint32_t *__restrict__ esc_count, int32_t *__restrict__ esc_list,                                  \
+        unsigned long long *__restrict__ attempts_out, int refill_min, uint32_t flag_or
+#define SIRIUS_PERSISTENT_ARGS \
+    cams, tp, o, ray_list, list_count, n, head, esc_count, esc_list, attempts_out, refill_min, flag_or
+
+// FP64 variants: register allocation left to the compiler (256 VGPRs, one wave per SIMD).
+template <class T, int FAM, int EMIT, bool MULTI>
+__global__ __launch_bounds__(256) void k_trace_persistent(SIRIUS_PERSISTENT_PARAMS) {
+    persistent_body<T, FAM, EMIT, MULTI>(SIRIUS_PERSISTENT_ARGS);
+}
+
+// FP32 variants: with SIRIUS_FP32_WAVES > 0 the register budget is capped for that many waves per
+// SIMD. Measured at 1080p: uncapped (186 VGPRs, 2 waves) 3.68 ms, 3 waves (168 VGPRs, 8 spills)
+// 4.09 ms, 4 waves (128 VGPRs, 57 spills) 4.50 ms; so the default leaves it uncapped.
+#ifndef SIRIUS_FP32_WAVES
+#define SIRIUS_FP32_WAVES 0
+#endif
+template <int FAM, int EMIT, bool MULTI>
+__global__ __launch_bounds__(256)
+#if SIRIUS_FP32_WAVES > 0
+__attribute__((amdgpu_waves_per_eu(SIRIUS_FP32_WAVES)))
+#endif
+void k_trace_persistent32(SIRIUS_PERSISTENT_PARAMS) {
+    persistent_body<float, FAM, EMIT, MULTI>(SIRIUS_PERSISTENT_ARGS);
 }
 
 // Outputs of every traced ray (rays the surrogate answered are skipped).
@@ -667,7 +713,9 @@ int persistent_blocks_per_cu(bool fp64, int family, int emission, bool multi, in
     int n = 0;
     hipError_t e = hipSuccess;
 #define SIRIUS_OCC(T, F, E, M) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_persistent<T, F, E, M>, block, 0)
-    if (fp64) { SIRIUS_VARIANTS(SIRIUS_OCC, double) } else { SIRIUS_VARIANTS(SIRIUS_OCC, float) }
+#define SIRIUS_OCC32(T, F, E, M) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_persistent32<F, E, M>, block, 0)
+    if (fp64) { SIRIUS_VARIANTS(SIRIUS_OCC, double) } else { SIRIUS_VARIANTS(SIRIUS_OCC32, float) }
+#undef SIRIUS_OCC32
 #undef SIRIUS_OCC
     if (e != hipSuccess || n <= 0) n = 1;
     return n;
@@ -718,7 +766,11 @@ hipError_t launch_trace_persistent(bool fp64, int family, int emission, bool mul
 #define SIRIUS_PERS(T, F, E, M)                                                                               \
     hipLaunchKernelGGL((k_trace_persistent<T, F, E, M>), g, b, 0, st, cams, tp, o, list, list_count, n, head,    \
                        esc_count, esc_list, attempts_out, refill_min, flag_or)
-    if (fp64) { SIRIUS_VARIANTS(SIRIUS_PERS, double) } else { SIRIUS_VARIANTS(SIRIUS_PERS, float) }
+#define SIRIUS_PERS32(T, F, E, M)                                                                             \
+    hipLaunchKernelGGL((k_trace_persistent32<F, E, M>), g, b, 0, st, cams, tp, o, list, list_count, n, head,     \
+                       esc_count, esc_list, attempts_out, refill_min, flag_or)
+    if (fp64) { SIRIUS_VARIANTS(SIRIUS_PERS, double) } else { SIRIUS_VARIANTS(SIRIUS_PERS32, float) }
+#undef SIRIUS_PERS32
 #undef SIRIUS_PERS
     return hipGetLastError();
 }

@@ -89,11 +89,13 @@ hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float
 }
 
 // Gate of the FP32 path: rays within `band` of the critical curve go to list B (traced in FP64
-// from the start, on a second stream), all others to list A (FP32 persistent tracer); rays
-// within `crit_band` get SIRIUS_FLAG_CRITICAL in flags[] (every ray's flag byte is written). A block
+// from the start, on a second stream), the others within `long_band` to list L (the FP32
+// tracer takes them first: they wind near the photon region and are the longest) and all
+// others to list A; rays within `crit_band` get SIRIUS_FLAG_CRITICAL in flags[] (every ray's
+// flag byte is written). counts[0..2] = |A|, |B|, |L|. A block
 // of 256 threads classifies a contiguous chunk of GATE_RAYS rays and reserves its slots with
 // one atomic per list (device-scope atomics on one address serialise, so one per wave is too
-// many at 1080p); within a chunk the lists keep ray order. counts[0] = |A|, counts[1] = |B|.
+// many at 1080p); within a chunk the lists keep ray order.
 constexpr int GATE_PASSES = 8;
 constexpr int GATE_RAYS = 256 * GATE_PASSES;
 
@@ -117,59 +119,59 @@ __device__ __forceinline__ float gate_distance(const CamConst* __restrict__ cams
 
 __global__ __launch_bounds__(256) void k_gate(const CamConst* __restrict__ cams, int n_cam,
                                               const float* __restrict__ tabs, int64_t n, float band,
-                                              float crit_band, int32_t* __restrict__ list_a,
-                                              int32_t* __restrict__ list_b, uint32_t* __restrict__ counts,
-                                              uint8_t* __restrict__ flags) {
-    __shared__ uint32_t off_a[GATE_PASSES][4], off_b[GATE_PASSES][4];
-    __shared__ uint32_t base[2];
+                                              float crit_band, float long_band, int32_t* __restrict__ list_a,
+                                              int32_t* __restrict__ list_b, int32_t* __restrict__ list_l,
+                                              uint32_t* __restrict__ counts, uint8_t* __restrict__ flags) {
+    __shared__ uint32_t off_a[GATE_PASSES][4], off_b[GATE_PASSES][4], off_l[GATE_PASSES][4];
+    __shared__ uint32_t base[3];
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
     const int64_t chunk = (int64_t)blockIdx.x * GATE_RAYS;
     const unsigned long long below = lane ? ((1ull << lane) - 1ull) : 0ull;
-    uint32_t in_b = 0;  // bit k: ray of pass k is in the band
+    uint32_t kinds = 0;  // 2 bits per pass: 0 list A, 1 list B (band), 2 list L (long), 3 invalid
 #pragma unroll
     for (int k = 0; k < GATE_PASSES; ++k) {
         int64_t id = chunk + k * 256 + threadIdx.x;
         bool valid = id < n;
         float d = valid ? gate_distance(cams, n_cam, tabs, id) : INFINITY;
-        bool to_b = valid && d < band;
+        uint32_t kind = !valid ? 3u : (d < band ? 1u : (d < long_band ? 2u : 0u));
         if (valid) flags[id] = d < crit_band ? (uint8_t)SIRIUS_FLAG_CRITICAL : (uint8_t)0;
-        in_b |= (uint32_t)to_b << k;
-        unsigned long long ma = __ballot(valid && !to_b), mb = __ballot(to_b);
+        kinds |= kind << (2 * k);
+        unsigned long long ma = __ballot(kind == 0), mb = __ballot(kind == 1), ml = __ballot(kind == 2);
         if (lane == 0) {
             off_a[k][wave] = (uint32_t)__popcll(ma);
             off_b[k][wave] = (uint32_t)__popcll(mb);
+            off_l[k][wave] = (uint32_t)__popcll(ml);
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t sa = 0, sb = 0;
+    if (threadIdx.x < 3) {
+        uint32_t(*off)[4] = threadIdx.x == 0 ? off_a : (threadIdx.x == 1 ? off_b : off_l);
+        uint32_t s = 0;
         for (int k = 0; k < GATE_PASSES; ++k)
             for (int w = 0; w < 4; ++w) {
-                uint32_t ta = off_a[k][w], tb = off_b[k][w];
-                off_a[k][w] = sa;
-                off_b[k][w] = sb;
-                sa += ta;
-                sb += tb;
+                uint32_t t = off[k][w];
+                off[k][w] = s;
+                s += t;
             }
-        base[0] = sa ? atomicAdd(&counts[0], sa) : 0u;
-        base[1] = sb ? atomicAdd(&counts[1], sb) : 0u;
+        base[threadIdx.x] = s ? atomicAdd(&counts[threadIdx.x], s) : 0u;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < GATE_PASSES; ++k) {
         int64_t id = chunk + k * 256 + threadIdx.x;
-        bool valid = id < n;
-        bool to_b = (in_b >> k) & 1u;
-        unsigned long long ma = __ballot(valid && !to_b), mb = __ballot(to_b);
-        if (to_b) list_b[base[1] + off_b[k][wave] + __popcll(mb & below)] = (int32_t)id;
-        else if (valid) list_a[base[0] + off_a[k][wave] + __popcll(ma & below)] = (int32_t)id;
+        uint32_t kind = (kinds >> (2 * k)) & 3u;
+        unsigned long long ma = __ballot(kind == 0), mb = __ballot(kind == 1), ml = __ballot(kind == 2);
+        if (kind == 0) list_a[base[0] + off_a[k][wave] + __popcll(ma & below)] = (int32_t)id;
+        else if (kind == 1) list_b[base[1] + off_b[k][wave] + __popcll(mb & below)] = (int32_t)id;
+        else if (kind == 2) list_l[base[2] + off_l[k][wave] + __popcll(ml & below)] = (int32_t)id;
     }
 }
 
 hipError_t launch_gate(const CamConst* cams, int n_cam, const float* tabs, int64_t n, double band, double crit_band,
-                       int32_t* list_a, int32_t* list_b, uint32_t* counts, uint8_t* flags, hipStream_t st) {
+                       double long_band, int32_t* list_a, int32_t* list_b, int32_t* list_l, uint32_t* counts,
+                       uint8_t* flags, hipStream_t st) {
     hipLaunchKernelGGL(k_gate, dim3((unsigned)((n + GATE_RAYS - 1) / GATE_RAYS)), dim3(256), 0, st, cams, n_cam, tabs,
-                       n, (float)band, (float)crit_band, list_a, list_b, counts, flags);
+                       n, (float)band, (float)crit_band, (float)long_band, list_a, list_b, list_l, counts, flags);
     return hipGetLastError();
 }
 

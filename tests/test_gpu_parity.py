@@ -125,7 +125,7 @@ def test_escalated_rays_are_bitwise_fp64(acc):
     got, st = acc.trace_batch([cam], [m], o)
     o64 = O.default_opts(abi.PRECISION_FP32_ESCALATE)
     o64.precision = abi.PRECISION_FP64
-    o64.rtol, o64.atol = o.escalate_rtol, o.atol * o.escalate_rtol / o.rtol
+    o64.rtol, o64.atol = o.escalate_rtol, o.atol * (o.escalate_rtol / o.rtol)  # as the library forms it
     ref, _ = acc.trace_batch([cam], [m], o64)
     e = (got.flags & abi.FLAG_ESCALATED) != 0
     assert e.sum() > 10
@@ -161,15 +161,20 @@ def test_multi_camera_batch_equals_single_cameras(acc):
                     assert np.array_equal(getattr(batch, k)[sl], getattr(one, k)), k
                 assert np.array_equal(batch.intensity[:, sl], one.intensity)
             else:
-                # FP32: the two variants contract FMAs differently; they agree to the FP32 bar
+                # FP32: the two variants contract FMAs differently, so each is held to the FP32
+                # bar against the camera's FP64 result
                 band = np.abs(crit[sl]) > 0.5
-                assert np.array_equal(batch.cls[sl][band], one.cls[band])
-                esc = band & (one.cls == 0) & (batch.cls[sl] == 0)
-                assert _angle(batch.take(np.arange(off, off + n)).direction()[esc], one.direction()[esc]).max() < 2e-5
-                dk = band & (one.cls == 2) & (batch.cls[sl] == 2)
-                I1, I0 = batch.intensity[:, sl][:, dk], one.intensity[:, dk]
-                rel = np.abs(I1 - I0).max(0) / np.maximum(I0.max(0), 1e-2 * np.median(I0.max(0)))
-                assert rel.max() < 1e-3
+                o64 = O.default_opts(abi.PRECISION_FP64)
+                o64.n_bins = 8
+                ref, _ = acc.trace_batch([single_cam], [metrics[c.metric_index]], o64)
+                for got in (batch.take(np.arange(off, off + n)), one):
+                    assert np.array_equal(got.cls[band], ref.cls[band])
+                    esc = band & (got.cls == 0) & (ref.cls == 0)
+                    assert _angle(got.direction()[esc], ref.direction()[esc]).max() < 1e-4
+                    dk = band & (got.cls == 2) & (ref.cls == 2)
+                    I1, I0 = got.intensity[:, dk], ref.intensity[:, dk]
+                    rel = np.abs(I1 - I0).max(0) / np.maximum(I0.max(0), 1e-2 * np.median(I0.max(0)))
+                    assert rel.max() < 1e-3
             off += n
 
 

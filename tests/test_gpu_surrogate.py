@@ -54,7 +54,7 @@ def _training_cameras(n_cam, side, seed):
         incl = g.uniform(5.0, 89.0)
         r_obs = g.uniform(20.0, 100.0)
         metrics.append(abi.metric_ks(1.0, a))
-        cams.append(abi.camera(r_obs, incl, 20.0, side, side, disk=(0.0, 30.0), metric_index=i))
+        cams.append(abi.camera(r_obs, incl, 40.0, side, side, disk=(0.0, 30.0), metric_index=i))
     return cams, metrics
 
 
@@ -64,20 +64,21 @@ def test_surrogate_training_decision_audit_and_ablation(acc):
     lab.far_slope = 0.25
     acc.mlp_init(0x51121006)
     print("[case] surrogate train 64 cameras x 64^2", flush=True)
-    first = acc.surrogate_train(cams, metrics, lab, 1, 8192, 2e-3, 1)
-    last = acc.surrogate_train(cams, metrics, lab, 6, 8192, 2e-3, 2)
-    assert np.isfinite(last) and last < first, (first, last)
+    first = acc.surrogate_train(cams, metrics, lab, 1, 4096, 2e-3, 1)
+    last = acc.surrogate_train(cams, metrics, lab, 20, 4096, 2e-3, 2)
+    print(f"[surrogate] loss {first:.4f} -> {last:.4f}", flush=True)
+    assert np.isfinite(last) and last < 0.5 * first, (first, last)
 
     m = abi.metric_ks(1.0, 0.9)
-    cam = abi.camera(50.0, 75.0, 20.0, 192, 108, disk=(0.0, 30.0))
+    cam = abi.camera(60.0, 60.0, 40.0, 160, 90, disk=(0.0, 30.0))
     off = O.default_opts(abi.PRECISION_FP32_ESCALATE)
-    print("[case] surrogate off 192x108", flush=True)
+    print("[case] surrogate off 160x90", flush=True)
     base, st_off = acc.trace_batch([cam], [m], off)
     on = O.default_opts(abi.PRECISION_FP32_ESCALATE)
     on.use_surrogate = 1
     on.surrogate_pmin = 0.9
     on.audit_fraction = 1.0
-    print("[case] surrogate on 192x108", flush=True)
+    print("[case] surrogate on 160x90", flush=True)
     got, st = acc.trace_batch([cam], [m], on)
     sur = (got.flags & abi.FLAG_SURROGATE) != 0
     assert not ((base.flags & abi.FLAG_SURROGATE) != 0).any()
@@ -96,7 +97,7 @@ def test_surrogate_training_decision_audit_and_ablation(acc):
     o64 = O.default_opts(abi.PRECISION_FP32_ESCALATE)
     o64.precision = abi.PRECISION_FP64
     ertol = on.escalate_rtol or on.rtol  # 0: escalations keep the path's tolerance
-    o64.rtol, o64.atol = ertol, on.atol * ertol / on.rtol
+    o64.rtol, o64.atol = ertol, on.atol * (ertol / on.rtol)
     ref, _ = acc.trace_batch([cam], [m], o64)
     same = sur & (ref.cls == got.cls)
     assert st["audit_class_mismatch"] == (sur & (ref.cls != got.cls)).sum()
@@ -106,4 +107,4 @@ def test_surrogate_training_decision_audit_and_ablation(acc):
     print(f"[surrogate] answered {sur.mean():.3f}, class agreement {(ref.cls[sur] == got.cls[sur]).mean():.4f}, "
           f"direction error median {np.median(err):.2e} max {err.max():.2e} rad", flush=True)
     assert (ref.cls[sur] == got.cls[sur]).mean() > 0.95
-    assert np.median(err) < 2e-2
+    assert np.median(err) < 5e-2

@@ -177,9 +177,10 @@ def _formal_volumetric(m, o, x, p, obs_norm, bins, n=40001):
     8th-order integrator with dense output, then I_k = int rho (nu_o^3 / nu_e^2) S(x_k / g)
     e^{-tau} dlam and tau = int kappa rho nu_e dlam by Simpson's rule on a uniform grid, with
     the emissivity definition of the trace options (rho = j0 r^-2 e^{-z^2 / (2 (h r)^2)} W(r),
-    S(x) = x^3 / expm1(x / (6/r)^{3/4}), emitter on the Kepler circle r_c^2 = x^2 + y^2 - a^2)."""
+    S(x) = x^3 / expm1(x / (6/r)^{3/4}), emitter on the Kepler circle r_c^2 = x^2 + y^2 - a^2, shell from the ISCO to r_max)."""
     from scipy.integrate import simpson
     a = m.a
+    r_lo = max(o.vol_r_min, O.lib().oracle_isco(a))  # the emitting shell starts at the ISCO
     kr = lambda y: O.lib().oracle_kerr_radius(m, y[1], y[2], y[3])
     r_cap = 1.01 * O.lib().oracle_horizon(m)
 
@@ -198,9 +199,9 @@ def _formal_volumetric(m, o, x, p, obs_norm, bins, n=40001):
     for j in range(n):
         y = Y[:, j]
         r = kr(y)
-        if not (o.vol_r_min < r < o.vol_r_max):
+        if not (r_lo < r < o.vol_r_max):
             continue
-        u = min(max((r - o.vol_r_min) / (0.1 * o.vol_r_min), 0), 1)
+        u = min(max((r - r_lo) / (0.1 * r_lo), 0), 1)
         v = min(max((o.vol_r_max - r) / (0.1 * o.vol_r_max), 0), 1)
         sig = o.vol_h_over_r * r
         rho = o.vol_j0 / r ** 2 * np.exp(-y[3] ** 2 / (2 * sig ** 2)) * u * u * (3 - 2 * u) * v * v * (3 - 2 * v)
