@@ -144,6 +144,10 @@ typedef struct SiriusHipTraceOpts {
     double critical_band;    /* FP32 path, Kerr-Schild: rays within this distance (M) of the
                               * critical curve are flagged SIRIUS_FLAG_CRITICAL; the accuracy
                               * contract exempts them and the inner-edge escalation skips them */
+    double step_radial_exp;  /* DP45 step proposal after an accepted step that moved inward
+                              * (r1 < r0): h_next = h fac(err) (r1 / r0)^exp, anticipating the
+                              * local error's growth toward the hole. 0 = off (the reference's
+                              * controller, the FP64 default); FP32 default 2              */
 } SiriusHipTraceOpts;
 
 /* Structure-of-arrays result buffers. Every array holds n_rays entries, except intensity

@@ -328,10 +328,12 @@ void oracle_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->escalate_disk_edge = 0.02;
     o->escalate_rtol = 0.0;
     o->critical_band = 0.5;
+    o->step_radial_exp = 0.0;
     if (precision == SIRIUS_PRECISION_FP32_ESCALATE) {
         o->rtol = 1e-6;
         o->atol = 1e-8;
         o->far_slope = 0.25;
+        o->step_radial_exp = 2.0;
     }
 }
 
@@ -934,7 +936,17 @@ static void trace_dp45(const cam_ctx* cx, const SiriusHipTraceOpts* o, const st8
         }
         /* ---- accept ---- */
         lam += h;
-        h = next_step_cap(h, e, o->h_min, step_cap(m, o, s1.y));
+        {
+            /* inward steps scale the proposal by (r1 / r0)^step_radial_exp (0: the reference's
+             * controller, computeOptimalStep, PHGD001A.cpp:471-483) */
+            double hp = h;
+            if (o->step_radial_exp > 0) {
+                double r0 = fam == SIRIUS_METRIC_MORRIS_THORNE ? fabs(s0.y[1])
+                            : oracle_kerr_radius(m, s0.y[1], s0.y[2], s0.y[3]);
+                if (r1 < r0) hp *= pow(r1 / r0, o->step_radial_exp);
+            }
+            h = next_step_cap(hp, e, o->h_min, step_cap(m, o, s1.y));
+        }
         s0 = s1;
         d0 = d1;
         /* null re-normalisation when |g k k| > 1e-6 (PHGD001A.cpp:609-638) */

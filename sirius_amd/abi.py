@@ -61,6 +61,7 @@ class TraceOpts(C.Structure):
         ("vol_j0", C.c_double), ("vol_h_over_r", C.c_double), ("vol_kappa", C.c_double),
         ("vol_r_min", C.c_double), ("vol_r_max", C.c_double), ("vol_tau_max", C.c_double),
         ("escalate_disk_edge", C.c_double), ("escalate_rtol", C.c_double), ("critical_band", C.c_double),
+        ("step_radial_exp", C.c_double),
     ]
 
 

@@ -58,6 +58,7 @@ struct TraceParams {
     double sym_h;
     double vol_j0, vol_h_over_r, vol_kappa, vol_r_min, vol_r_max, vol_tau_max;
     double esc_edge;
+    double step_rexp;  // inward step-proposal exponent (0: the reference's controller)
     int32_t max_steps, n_bins, emission, sym_order;
     int32_t n_cam, escalate, gate_flags, pad1;  // gate_flags: the gate wrote per-ray flags
     int64_t n_rays;
@@ -327,9 +328,12 @@ SIRIUS_HD __forceinline__ void eval(const Spacetime<T>& s, const T x[4], const T
 // ------------------------------------------------------------------------------------------
 // TIME = false skips the coordinate time t (x[0]), which no equation depends on: the FP32
 // path integrates the 6 components (x, y, z, p_x, p_y, p_z) and measures its error on x, y, z.
+// dense (optional, 6 values): the step's dense-output term r5 for x, y, z, p_x, p_y, p_z, with
+// which dense_eval interpolates anywhere inside the step without new evaluations.
 template <class T, int FAM, bool TIME = true>
 SIRIUS_HD __forceinline__ void dp45_step(const Spacetime<T>& s, const T x0[4], const T p0[4], const Der<T>& d1,
-                                 T h, T x1[4], T p1[4], Der<T>& d7, T err[4], bool want_err) {
+                                 T h, T x1[4], T p1[4], Der<T>& d7, T err[4], bool want_err,
+                                 T* dense = nullptr) {
     constexpr double a21 = 1.0 / 5.0;
     constexpr double a31 = 3.0 / 40.0, a32 = 9.0 / 40.0;
     constexpr double a41 = 44.0 / 45.0, a42 = -56.0 / 15.0, a43 = 32.0 / 9.0;
@@ -393,6 +397,39 @@ SIRIUS_HD __forceinline__ void dp45_step(const Spacetime<T>& s, const T x0[4], c
             err[i] = h * (T(e1) * d1.k[i] + T(e3) * d3.k[i] + T(e4) * d4.k[i] + T(e5) * d5.k[i] + T(e6) * d6.k[i] +
                           T(e7) * d7.k[i]);
     }
+    if (dense) {
+        // Dormand-Prince continuous extension of order 4 (Hairer, Norsett & Wanner, Solving
+        // Ordinary Differential Equations I, sec. II.6; their DOPRI5 code's dense output)
+        constexpr double q1 = -12715105075.0 / 11282082432.0, q3 = 87487479700.0 / 32700410799.0,
+                         q4 = -10690763975.0 / 1880347072.0, q5 = 701980252875.0 / 199316789632.0,
+                         q6 = -1453857185.0 / 822651844.0, q7 = 69997945.0 / 29380423.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            dense[i] = h * (T(q1) * d1.k[i + 1] + T(q3) * d3.k[i + 1] + T(q4) * d4.k[i + 1] + T(q5) * d5.k[i + 1] +
+                            T(q6) * d6.k[i + 1] + T(q7) * d7.k[i + 1]);
+            dense[3 + i] = h * (T(q1) * d1.f[i] + T(q3) * d3.f[i] + T(q4) * d4.f[i] + T(q5) * d5.f[i] +
+                                T(q6) * d6.f[i] + T(q7) * d7.f[i]);
+        }
+    }
+}
+
+// One component of the dense output at th in [0, 1] from its values y0, y1 at the step's ends,
+// the end slopes k0, k1, the step length h and the step's r5 term:
+//   y(th) = y0 + th (d + (1 - th) (b + th (c + (1 - th) r5))),
+//   d = y1 - y0, b = hk0 - d, c = d - hk1 - b.
+// It matches y and its slope at both ends; with r5 = 0 it is the cubic Hermite interpolant.
+// Every multiply-add is an explicit fma, so the rounding does not depend on how the compiler
+// contracts (and the same ray gives the same bits in every kernel that traces it).
+template <class T>
+SIRIUS_HD __forceinline__ T dense_eval(T y0, T y1, T h, T k0, T k1, T r5, T th) {
+    T d = y1 - y0;
+    T b = fma(h, k0, -d);
+    T c = d - fma(h, k1, b);
+    T t1 = T(1) - th;
+    T u = fma(t1, r5, c);
+    u = fma(th, u, b);
+    u = fma(t1, u, d);
+    return fma(th, u, y0);
 }
 
 // computeOptimalStep (PHGD001A.cpp:471-483): h * clamp(0.9 err^-1/5, 0.1, 2), clamped.
@@ -438,6 +475,38 @@ SIRIUS_HD __forceinline__ T locate_event(const Spacetime<T>& s, int kind, T targ
         T mid = T(0.5) * (lo + hi);
         hermite(x0, k0, x1, k1, h, mid, xm);
         bool neg = event_value(s, kind, xm, target) < T(0);
+        if (neg == neg_lo) lo = mid; else hi = mid;
+    }
+    return T(0.5) * (lo + hi);
+}
+
+// Position on the dense output of a step (x^0 is carried unchanged: the FP32 path does not
+// integrate t).
+template <class T>
+SIRIUS_HD __forceinline__ void dense_pos(const T x0[4], const T k0[4], const T x1[4], const T k1[4], const T dn[6],
+                                         T h, T th, T out[4]) {
+    out[0] = x0[0];
+#pragma unroll
+    for (int i = 1; i < 4; ++i) out[i] = dense_eval(x0[i], x1[i], h, k0[i], k1[i], dn[i - 1], th);
+}
+
+// Event location on the dense output: bisection of the event function over th in [0, 1]
+// (the plane z = 0 needs only the z component).
+template <class T>
+SIRIUS_HD __forceinline__ T locate_event_dense(const Spacetime<T>& s, int kind, T target, const T x0[4],
+                                               const T k0[4], const T x1[4], const T k1[4], const T dn[6], T h) {
+    constexpr int iters = sizeof(T) == 8 ? 52 : 24;
+    T lo = T(0), hi = T(1), xm[4];
+    bool neg_lo = event_value(s, kind, x0, target) < T(0);
+    for (int it = 0; it < iters; ++it) {
+        T mid = T(0.5) * (lo + hi);
+        T v;
+        if (kind == EV_Z) v = dense_eval(x0[3], x1[3], h, k0[3], k1[3], dn[2], mid);
+        else {
+            dense_pos(x0, k0, x1, k1, dn, h, mid, xm);
+            v = event_value(s, kind, xm, target);
+        }
+        bool neg = v < T(0);
         if (neg == neg_lo) lo = mid; else hi = mid;
     }
     return T(0.5) * (lo + hi);

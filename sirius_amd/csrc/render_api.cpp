@@ -80,17 +80,32 @@ float* upload_critical_tables(Handle& H, Batch& B, int* rc) {
     int n_cam = (int)B.cc.size();
     // the host copy lives in the handle, so the asynchronous upload needs no synchronisation
     // (the stream is synchronised before this buffer is rewritten by the next batch)
-    std::vector<float>& tabs = H.crit_host;
-    tabs.assign((size_t)n_cam * (SIRIUS_CC_TABLE + 1), 0.f);
-    for (int i = 0; i < n_cam; ++i)
-        if (B.cc[i].family == 0) critical_table(B.cc[i], tabs.data() + (size_t)i * (SIRIUS_CC_TABLE + 1));
+    static ConstCache<7, std::vector<float>> table_cache;
+    const size_t per = SIRIUS_CC_TABLE + 1;
+    std::vector<float> tabs((size_t)n_cam * per, 0.f);
+    for (int i = 0; i < n_cam; ++i) {
+        const CamConst& c = B.cc[i];
+        if (c.family != 0) continue;
+        const double key[7] = {c.M, c.a, c.Q2, c.r_obs, c.er[0], c.er[1], c.er[2]};
+        std::vector<float> t = table_cache.get(key, [&] {
+            std::vector<float> v(per);
+            critical_table(c, v.data());
+            return v;
+        });
+        std::memcpy(tabs.data() + (size_t)i * per, t.data(), per * 4);
+    }
     float* d = (float*)H.scratch("crit_tabs", tabs.size() * 4, rc);
     if (*rc) return nullptr;
-    hipError_t e = hipMemcpyAsync(d, tabs.data(), tabs.size() * 4, hipMemcpyHostToDevice, H.stream);
+    // batches end synchronised, so the device still holds the last upload: skip an identical one
+    if (H.crit_dev_held == d && H.crit_host == tabs) return d;
+    H.crit_host.swap(tabs);
+    H.crit_dev_held = nullptr;
+    hipError_t e = hipMemcpyAsync(d, H.crit_host.data(), H.crit_host.size() * 4, hipMemcpyHostToDevice, H.stream);
     if (e != hipSuccess) {
         *rc = H.hip_fail(e, "upload critical tables");
         return nullptr;
     }
+    H.crit_dev_held = d;
     return d;
 }
 

@@ -27,9 +27,6 @@ namespace sirius {
 hipError_t launch_fill(uint32_t* p, int n, uint32_t v, hipStream_t st);
 hipError_t launch_finish(int family, int emission, const CamConst* cams, const TraceParams& tp, const DevOut& o,
                          hipStream_t st);
-hipError_t launch_trace_grid(bool fp64, int family, int emission, bool multi, const CamConst* cams, const TraceParams& tp,
-                             const DevOut& o, const int32_t* list, int64_t n, const uint32_t* count_dev,
-                             uint32_t flag_or, int block, hipStream_t st);
 hipError_t launch_trace_persistent(bool fp64, int family, int emission, bool multi, const CamConst* cams,
                                    const TraceParams& tp,
                                    const DevOut& o, const int32_t* list, const uint32_t* list_count, int64_t n,
@@ -149,6 +146,9 @@ bool make_cam_const(const SiriusHipCamera& cam, const SiriusHipMetric& m, int64_
         if (!(m.throat_b0 > 0) || cam.r_obs <= m.throat_b0) { err = "wormhole needs 0 < b0 < r_obs"; return false; }
         c.l_obs = std::sqrt(cam.r_obs * cam.r_obs - m.throat_b0 * m.throat_b0);
     }
+    for (double v : {m.M, m.a, m.Q, m.Lambda, m.throat_b0, m.warp_v, m.warp_R, m.warp_sigma, cam.r_obs, cam.theta_obs,
+                     cam.phi_obs, cam.disk_r_in, cam.disk_r_out})
+        if (!std::isfinite(v)) { err = "camera and metric parameters must be finite"; return false; }
     if (m.family == SIRIUS_METRIC_KERR_SCHILD) {
         double d = m.M * m.M - m.a * m.a - m.Q * m.Q;
         if (!(m.M > 0) || d < 0) { err = "Kerr-Schild family needs M > 0 and M^2 >= a^2 + Q^2"; return false; }
@@ -169,7 +169,9 @@ bool make_cam_const(const SiriusHipCamera& cam, const SiriusHipMetric& m, int64_
             double xi = c.pt_x[i], xj = c.pt_x[(i + 1) % 3], xk = c.pt_x[(i + 2) % 3];
             c.pt_c[i] = 3.0 * (xi - a) * (xi - a) / (xi * (xi - xj) * (xi - xk));
         }
-        c.fmax = tmax_f_over_r(c);
+        static ConstCache<5, double> fmax_cache;
+        const double key[5] = {c.M, c.a, c.Q2, c.r_in, c.r_out};
+        c.fmax = fmax_cache.get(key, [&] { return tmax_f_over_r(c); });
         c.r_cap = rplus;  // multiplied by horizon_factor in trace_batch
     }
     double x[4] = {0, c.r_obs * c.er[0], c.r_obs * c.er[1], c.r_obs * c.er[2]};
@@ -206,6 +208,7 @@ Handle::~Handle() { release(); }
 void Handle::release() {
     for (auto& kv : bufs) if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     bufs.clear();
+    cam_dev_held = crit_dev_held = nullptr;
     if (fb_dev) (void)hipFree(fb_dev);
     fb_dev = nullptr;
     if (bg_dev) (void)hipFree(bg_dev);
@@ -272,6 +275,9 @@ int Handle::ensure_device() {
 void* Handle::scratch(const char* name, size_t bytes, int* rc) {
     Buf& b = bufs[name];
     if (b.bytes >= bytes && b.ptr) return b.ptr;
+    // a replaced buffer no longer holds an upload (a new allocation may reuse its address)
+    if (b.ptr == cam_dev_held) cam_dev_held = nullptr;
+    if (b.ptr == crit_dev_held) crit_dev_held = nullptr;
     if (b.ptr) (void)hipFree(b.ptr);
     b.ptr = nullptr;
     b.bytes = 0;
@@ -296,6 +302,8 @@ static int validate_opts(Handle& H, const SiriusHipTraceOpts& o) {
     if (!(o.h_init > 0) || !(o.h_min > 0) || !(o.h_max >= o.h_min))
         return H.fail(SIRIUS_HIP_E_INVALID, "need 0 < h_min <= h_max and h_init > 0");
     if (o.max_steps <= 0) return H.fail(SIRIUS_HIP_E_INVALID, "max_steps must be positive");
+    if (!(o.step_radial_exp >= 0 && o.step_radial_exp <= 8))
+        return H.fail(SIRIUS_HIP_E_INVALID, "step_radial_exp must be in [0, 8]");
     if (!(o.r_escape > 0) || !(o.horizon_factor >= 1.0))
         return H.fail(SIRIUS_HIP_E_INVALID, "need r_escape > 0 and horizon_factor >= 1");
     if (!(o.x_min > 0 && o.x_max > o.x_min)) return H.fail(SIRIUS_HIP_E_INVALID, "need 0 < x_min < x_max");
@@ -351,6 +359,7 @@ int prepare_batch(Handle& H, const SiriusHipCamera* cams, int n_cam, const Siriu
     tp.vol_j0 = o.vol_j0; tp.vol_h_over_r = o.vol_h_over_r; tp.vol_kappa = o.vol_kappa;
     tp.vol_r_min = o.vol_r_min; tp.vol_r_max = o.vol_r_max; tp.vol_tau_max = o.vol_tau_max;
     tp.esc_edge = o.escalate_disk_edge;
+    tp.step_rexp = o.step_radial_exp;
     tp.gate_flags = 0;
     tp.first_list = nullptr;
     tp.first_count = nullptr;
@@ -368,9 +377,16 @@ int prepare_batch(Handle& H, const SiriusHipCamera* cams, int n_cam, const Siriu
     void* d = H.scratch("cams", sizeof(CamConst) * n_cam, &rc);
     if (!d) return rc;
     B.cams_dev = (CamConst*)d;
+    // every batch ends with a synchronisation, so the device copy of the last batch's cameras is
+    // intact: an identical batch (a repeated frame) needs no upload
+    if (H.cam_dev_held == d && H.cam_host.size() == B.cc.size() &&
+        std::memcmp(H.cam_host.data(), B.cc.data(), sizeof(CamConst) * n_cam) == 0)
+        return SIRIUS_HIP_OK;
     H.cam_host = B.cc;  // outlives the asynchronous copy
+    H.cam_dev_held = nullptr;
     hipError_t e = hipMemcpyAsync(d, H.cam_host.data(), sizeof(CamConst) * n_cam, hipMemcpyHostToDevice, H.stream);
     if (e != hipSuccess) return H.hip_fail(e, "upload cameras");
+    H.cam_dev_held = d;
     return SIRIUS_HIP_OK;
 }
 
@@ -475,11 +491,17 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
         if ((e = launch_symplectic(fp64, B.family, B.cams_dev, B.tp, dout, block, H.stream)) != hipSuccess)
             return H.hip_fail(e, "symplectic launch");
     } else if (!persistent) {
-        // the grid schedule has no escalation queue: FP32 lanes run to the end in FP32
+        // Grid schedule: one lane per ray, run to completion, no escalation queue (FP32 lanes
+        // run to the end in FP32). It is the tracer kernel with a static assignment: a grid of
+        // one wave per 64 rays, each wave taking one batch of 64 consecutive list slots when all
+        // its lanes are idle, so both schedules run the same compiled code, bit for bit.
         TraceParams tp = B.tp;
         tp.escalate = 0;
-        if ((e = launch_trace_grid(fp64, B.family, B.tp.emission, B.tp.n_cam > 1, B.cams_dev, tp, dout, list_dev, n_work, list_count_dev, 0,
-                                   block, H.stream)) != hipSuccess)
+        int grid = (int)std::max<int64_t>(1, (n_work + block - 1) / block);
+        if ((e = launch_trace_persistent(fp64, B.family, B.tp.emission, B.tp.n_cam > 1, B.cams_dev, tp, dout, list_dev,
+                                         list_count_dev, n_work, work, work + 1, nullptr,
+                                         (unsigned long long*)(work + 2), grid, block, 64, 0, H.stream)) !=
+            hipSuccess)
             return H.hip_fail(e, "grid launch");
     } else if (fp64) {
         TraceParams tp = B.tp;
@@ -834,10 +856,12 @@ void sirius_hip_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->escalate_disk_edge = 0.02;
     o->escalate_rtol = 0.0;
     o->critical_band = 0.5;
+    o->step_radial_exp = 0.0;
     if (precision == SIRIUS_PRECISION_FP32_ESCALATE) {
         o->rtol = 1e-6;
         o->atol = 1e-8;
         o->far_slope = 0.25;
+        o->step_radial_exp = 2.0;
     }
 }
 

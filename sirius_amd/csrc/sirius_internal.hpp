@@ -3,7 +3,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
+#include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -20,6 +23,34 @@ struct Buf {
 };
 
 struct MlpState;
+
+// Process-wide memo of host-side constants that depend only on a few doubles (the disk's flux
+// maximum, a camera's critical-curve table): keyed by the exact bits of those doubles, bounded
+// (cleared when it reaches max_entries), safe to share between handles and threads.
+template <int K, class V>
+class ConstCache {
+  public:
+    template <class Make>
+    V get(const double (&key)[K], Make make) {
+        std::array<uint64_t, K> k;
+        std::memcpy(k.data(), key, sizeof(k));
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            auto it = map_.find(k);
+            if (it != map_.end()) return it->second;
+        }
+        V v = make();
+        std::lock_guard<std::mutex> g(mu_);
+        if (map_.size() >= max_entries) map_.clear();
+        map_.emplace(k, v);
+        return v;
+    }
+    static constexpr size_t max_entries = 4096;
+
+  private:
+    std::mutex mu_;
+    std::map<std::array<uint64_t, K>, V> map_;
+};
 
 // One handle = one device (device 0), one stream, cached device scratch.
 struct Handle {
@@ -38,6 +69,9 @@ struct Handle {
     std::map<std::string, Buf> bufs;
     std::vector<float> crit_host;       // host copies that async uploads read from
     std::vector<CamConst> cam_host;
+    // device buffers that already hold crit_host / cam_host (an identical batch skips the upload)
+    const void* crit_dev_held = nullptr;
+    const void* cam_dev_held = nullptr;
     // framebuffer (IAccelerator::initialise / getFrameBuffer)
     float* fb_dev = nullptr;
     std::vector<float> fb_host;

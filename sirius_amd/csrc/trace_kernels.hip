@@ -3,13 +3,14 @@
 // Role replaced: the OptiX raygen mega-kernel's integrate/terminate/shade loop
 // (src/Sirius.Render/Acceleration/OptiX/RDOP002A.cu:5132-6106), with the physics contract of
 // the CPU tracer (src/Sirius.Render/Integration/GTRC001A.cpp:139-592) instead of the
-// OptiX kernel's approximations. Three launch shapes:
-//   k_trace_grid        one lane per ray, run to completion (plain FP64 path, escalations)
-//   k_trace_persistent  a device-filling grid of waves that pull rays from an atomic work
-//                       counter and refill idle lanes as rays terminate (FP32 bulk path);
-//                       rays whose periapsis falls inside the photon region are pushed to an
-//                       escalation queue and re-traced by k_trace_queue in FP64
-//   k_symplectic_grid   fixed-step Tao/Yoshida integrator
+// OptiX kernel's approximations. Kernels:
+//   k_trace_persistent    DP45 tracer (FP64): waves pull batches of rays from an atomic work
+//   k_trace_persistent32  counter and refill idle lanes as rays terminate (FP32 variant with its
+//                         own register budget). FP32 rays that meet an escalation criterion are
+//                         pushed to a queue and re-traced in FP64. The grid schedule is the same
+//                         kernel with one wave per 64 rays and no refills.
+//   k_finish              per-ray outputs (directions, emission, null drift) in FP64
+//   k_symplectic_grid     fixed-step Tao/Yoshida integrator
 #include <hip/hip_runtime.h>
 
 #include "geodesic.hpp"
@@ -278,6 +279,20 @@ SIRIUS_HD __forceinline__ T land(Ray<T, FAM>& R, const Spacetime<T>& s, bool pla
     return th;
 }
 
+// FP32 landing: the state at fraction th of the step from the step's dense output (x, y, z and
+// p_x, p_y, p_z; t and p_t are not stepped in FP32). Landings end the ray.
+template <class T, int FAM>
+SIRIUS_HD __forceinline__ void land_dense(Ray<T, FAM>& R, const T x1[4], const T p1[4], const Der<T>& d1,
+                                          const T dn[6], T th) {
+    T xl[4], pl[4];
+    dense_pos(R.x, R.d.k, x1, d1.k, dn, R.h, th, xl);
+    pl[0] = R.p[0];
+#pragma unroll
+    for (int i = 1; i < 4; ++i) pl[i] = dense_eval(R.p[i], p1[i], R.h, R.d.f[i - 1], d1.f[i - 1], dn[2 + i], th);
+    R.lam = fma(th, R.h, R.lam);
+    for (int i = 0; i < 4; ++i) { R.x[i] = xl[i]; R.p[i] = pl[i]; }
+}
+
 // One DP45 attempt with error control, events and acceptance. Returns ST_RUN, ST_DONE (the
 // ray ended: R.cls, R.rdisk and the final state are set; the caller writes it out with
 // ray_finish) or ST_ESCALATE (an FP32 lane hands the ray to the FP64 queue). Every event is
@@ -289,8 +304,10 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
     constexpr bool TIME = !fp32;
     constexpr bool vol = (FAM == 0) && EMIT == SIRIUS_EMISSION_VOLUMETRIC;
     T hmax = T(tp.h_max);
+    T r0 = T(0);
+    if (tp.far_slope > 0 || tp.step_rexp > 0) r0 = event_radius(s, R.x);
     if (tp.far_slope > 0) {
-        T rr = event_radius(s, R.x) * T(tp.far_slope);
+        T rr = r0 * T(tp.far_slope);
         hmax = rr > hmax ? rr : hmax;
     }
     if constexpr (vol) {
@@ -299,7 +316,9 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
     }
     T x1[4], p1[4], err[4];
     Der<T> d1;
-    dp45_step<T, FAM, TIME>(s, R.x, R.p, R.d, R.h, x1, p1, d1, err, true);
+    // FP32: the step's dense output locates and lands events (no further evaluations)
+    T dn[6];
+    dp45_step<T, FAM, TIME>(s, R.x, R.p, R.d, R.h, x1, p1, d1, err, true, fp32 ? dn : nullptr);
     R.attempts++;
     T e = T(0);
 #pragma unroll
@@ -334,9 +353,14 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
     T th_disk = T(1);
     if constexpr (FAM == 0 && EMIT == SIRIUS_EMISSION_THIN_DISK) {
         if (R.x[3] * x1[3] < T(0)) {
-            th_disk = locate_event(s, EV_Z, T(0), R.x, R.d.k, x1, d1.k, R.h);
             T xc[4];
-            hermite(R.x, R.d.k, x1, d1.k, R.h, th_disk, xc);
+            if constexpr (fp32) {
+                th_disk = locate_event_dense(s, EV_Z, T(0), R.x, R.d.k, x1, d1.k, dn, R.h);
+                dense_pos(R.x, R.d.k, x1, d1.k, dn, R.h, th_disk, xc);
+            } else {
+                th_disk = locate_event(s, EV_Z, T(0), R.x, R.d.k, x1, d1.k, R.h);
+                hermite(R.x, R.d.k, x1, d1.k, R.h, th_disk, xc);
+            }
             T rc = kerr_r(s, xc[1], xc[2], xc[3]);
             if constexpr (fp32) {
                 // near the inner edge the emission falls steeply to zero: an FP32 position error
@@ -366,8 +390,13 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
     if (ev >= 0) {
         bool plane = (ev == SIRIUS_RAY_DISK) && EMIT == SIRIUS_EMISSION_THIN_DISK;
         if (plane || locate) {
-            T th = plane ? th_disk : locate_event(s, kind, target, R.x, R.d.k, x1, d1.k, R.h);
-            land<T, FAM>(R, s, plane, th);
+            if constexpr (fp32) {
+                T th = plane ? th_disk : locate_event_dense(s, kind, target, R.x, R.d.k, x1, d1.k, dn, R.h);
+                land_dense<T, FAM>(R, x1, p1, d1, dn, th);
+            } else {
+                T th = plane ? th_disk : locate_event(s, kind, target, R.x, R.d.k, x1, d1.k, R.h);
+                land<T, FAM>(R, s, plane, th);
+            }
         } else {
             R.lam += R.h;
             for (int i = 0; i < 4; ++i) { R.x[i] = x1[i]; R.p[i] = p1[i]; }
@@ -384,7 +413,11 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
         T rr = r1 * T(tp.far_slope);
         hmax2 = rr > hmax2 ? rr : hmax2;
     }
-    R.h = next_step(R.h, e, T(tp.h_min), hmax2);
+    T hprop = R.h;
+    // an inward step predicts the next step's error growth: the local error of a step of a
+    // given length rises steeply as r falls, so the proposal scales with (r1 / r0)^exp
+    if (tp.step_rexp > 0 && r1 < r0) hprop *= pow(r1 / r0, T(tp.step_rexp));
+    R.h = next_step(hprop, e, T(tp.h_min), hmax2);
     for (int i = 0; i < 4; ++i) { R.x[i] = x1[i]; R.p[i] = p1[i]; }
     R.d = d1;
     if constexpr (fp32) {
@@ -433,29 +466,6 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
 // ------------------------------------------------------------------------------------------
 // Kernels
 // ------------------------------------------------------------------------------------------
-
-// One lane per ray, run to completion. ray_list (optional) maps lane -> ray; count_dev
-// (optional) holds the list length on the device.
-// MULTI = false: one camera per launch, so the camera and spacetime constants are
-// wave-uniform (scalar registers); MULTI = true: every lane finds its own camera.
-template <class T, int FAM, int EMIT, bool MULTI>
-__global__ __launch_bounds__(256) void k_trace_grid(const CamConst* __restrict__ cams, TraceParams tp, DevOut o,
-                                                    const int32_t* __restrict__ ray_list, int64_t n,
-                                                    const uint32_t* __restrict__ count_dev, uint32_t flag_or) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t lim = count_dev ? (int64_t)*count_dev : n;
-    if (lim > n) lim = n;
-    if (i >= lim) return;
-    int64_t id = ray_list ? (int64_t)ray_list[i] : i;
-    int cam = MULTI ? find_camera(cams, tp.n_cam, id) : 0;
-    Ray<T, FAM> R;
-    ray_start<T, FAM, EMIT>(R, cams, cam, id, tp, o);
-    R.flags |= flag_or;
-    Spacetime<T> s = make_spacetime<T>(cams[cam]);
-    while (ray_attempt<T, FAM, EMIT, MULTI>(R, cams, s, tp, o) == ST_RUN) {
-    }
-    ray_store<T, FAM, EMIT>(R, o);
-}
 
 // Persistent tracer. Each wave pulls batches of work-list slots from *head (one atomic per
 // refill) and refills its idle lanes when at least refill_min of them are idle; slot -> ray
@@ -741,18 +751,6 @@ hipError_t launch_finish(int family, int emission, const CamConst* cams, const T
 
 hipError_t launch_fill(uint32_t* p, int n, uint32_t v, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
-    return hipGetLastError();
-}
-
-hipError_t launch_trace_grid(bool fp64, int family, int emission, bool multi, const CamConst* cams, const TraceParams& tp,
-                             const DevOut& o, const int32_t* list, int64_t n, const uint32_t* count_dev,
-                             uint32_t flag_or, int block, hipStream_t st) {
-    if (n <= 0) return hipSuccess;
-    dim3 g((unsigned)((n + block - 1) / block)), b(block);
-#define SIRIUS_GRID(T, F, E, M) \
-    hipLaunchKernelGGL((k_trace_grid<T, F, E, M>), g, b, 0, st, cams, tp, o, list, n, count_dev, flag_or)
-    if (fp64) { SIRIUS_VARIANTS(SIRIUS_GRID, double) } else { SIRIUS_VARIANTS(SIRIUS_GRID, float) }
-#undef SIRIUS_GRID
     return hipGetLastError();
 }
 
