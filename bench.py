@@ -189,7 +189,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (f64 escalation and null-constraint re-normalisation)",
+        "dtype": "f32 (compensated state; f64 null-constraint re-normalisation and escalation)",
         "data": "synthetic",
         "config": {"workload": "kerr_a0.9_1080p_dp45_fp32_escalate", "width": W, "height": H, "a": 0.9,
                    "r_obs": 50.0, "inclination_deg": 75.0, "fov_deg": 20.0, "disk": "ISCO..30M",
@@ -213,6 +213,15 @@ def main():
     if not args.no_accuracy:
         crit = acc.critical_distance([cam], [metric])
         result["accuracy_vs_fp64"] = accuracy(fast, ref, crit)
+        # the same frame with the inner-edge escalation on: crossings within 2 % of r_in are
+        # re-traced in FP64 after the bulk (the cost of that path, and what it buys)
+        oe = default_opts(abi.PRECISION_FP32_ESCALATE)
+        oe.escalate_disk_edge = 0.02
+        mse, ste = time_frames(acc, cam, metric, oe, args.steps, args.warmup)
+        result["with_edge_escalation"] = {
+            "value": n_rays / mse / 1e6, "ms_per_step": mse * 1e3, "rays_escalated": ste[-1]["rays_escalated"],
+            "escalate_ms": float(np.mean([x["escalate_ms"] for x in ste])),
+            "accuracy_vs_fp64": accuracy(acc.fetch_results(), ref, crit)}
     if not args.no_surrogate and ref is not None:
         result["surrogate"] = surrogate_ablation(acc, cam, metric, ref, args.steps, args.warmup)
     if not args.no_cpu_baseline:

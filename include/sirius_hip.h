@@ -48,11 +48,12 @@ enum {
 /* ---- enums ---------------------------------------------------------------------------- */
 enum { SIRIUS_METRIC_KERR_SCHILD = 0, SIRIUS_METRIC_MORRIS_THORNE = 1, SIRIUS_METRIC_ALCUBIERRE = 2 };
 enum { SIRIUS_INTEGRATOR_DP45 = 0, SIRIUS_INTEGRATOR_SYMPLECTIC = 1 };
-/* FP32_ESCALATE: FP32 stepping with FP64 null-constraint re-normalisation; a ray is re-traced
- * in FP64 from its pixel when it meets an escalation criterion: Kerr-Schild - the options
- * escalate_band, escalate_margin, escalate_disk_edge; Morris-Thorne - impact parameter within
- * 10 % of the throat radius; Alcubierre - entering the bubble wall (r_s < R + 3 / sigma).
- * With any criterion on (the default) these family criteria apply too. */
+/* FP32_ESCALATE: FP32 stepping on a compensated state (FP64 initial state kept as high + low
+ * FP32 parts, error-free accumulation of every step) with FP64 null-constraint
+ * re-normalisation; a ray is re-traced in FP64 from its pixel when it meets an escalation
+ * criterion: Kerr-Schild - the options escalate_band, escalate_margin, escalate_disk_edge (all
+ * off by default); Morris-Thorne - impact parameter within 10 % of the throat radius;
+ * Alcubierre - entering the bubble wall (r_s < R + 3 / sigma) (both always on). */
 enum { SIRIUS_PRECISION_FP64 = 0, SIRIUS_PRECISION_FP32_ESCALATE = 1 };
 enum { SIRIUS_EMISSION_NONE = 0, SIRIUS_EMISSION_THIN_DISK = 1, SIRIUS_EMISSION_VOLUMETRIC = 2 };
 enum { SIRIUS_SCHEDULE_GRID = 0, SIRIUS_SCHEDULE_PERSISTENT = 1 };
@@ -136,7 +137,9 @@ typedef struct SiriusHipTraceOpts {
     double vol_j0, vol_h_over_r, vol_kappa, vol_r_min, vol_r_max, vol_tau_max;
     double escalate_disk_edge; /* FP32 path, thin disk: a disk crossing whose radius lies within
                               * this fraction of r_in (|r/r_in - 1| < edge) is re-traced in FP64,
-                              * where the emission's steep edge magnifies FP32 position error */
+                              * where the emission's steep edge magnifies position error; 0 =
+                              * off (default: the compensated FP32 state meets the intensity
+                              * tolerance there)                                             */
     double escalate_rtol;    /* relative tolerance of escalated FP64 re-traces (atol scales with
                               * it); they keep the FP32 path's step caps. 0 (default): the
                               * path's rtol, except 1e-8 for the warp bubble, whose wall

@@ -325,7 +325,7 @@ void oracle_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->vol_r_min = 2.0;
     o->vol_r_max = 50.0;
     o->vol_tau_max = 10.0;
-    o->escalate_disk_edge = 0.02;
+    o->escalate_disk_edge = 0.0;
     o->escalate_rtol = 0.0;
     o->critical_band = 0.5;
     o->step_radial_exp = 0.0;

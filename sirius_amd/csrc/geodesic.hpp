@@ -323,6 +323,24 @@ SIRIUS_HD __forceinline__ void eval(const Spacetime<T>& s, const T x[4], const T
     d.pk = rhs<T, FAM>(s, x, p, d.k, d.f);
 }
 
+// Stage point y0 + inc, and the step update y0 + inc: with a compensated state (lo non-null) the
+// low part lo[j] joins the increment, and the update's sum is error-free (Knuth's two-sum: hi + lo1[j] equals y0 + lo[j] + inc
+// exactly, inc as rounded).
+template <class T>
+SIRIUS_HD __forceinline__ T stage_pt(T y0, T inc, const T* lo, int j) {
+    return lo ? y0 + (lo[j] + inc) : y0 + inc;
+}
+
+template <class T>
+SIRIUS_HD __forceinline__ T sum_lo(T y0, T inc, const T* lo, T* lo1, int j) {
+    if (!lo) return y0 + inc;
+    T t = lo[j] + inc;
+    T hi = y0 + t;
+    T bb = hi - y0;
+    lo1[j] = (y0 - (hi - bb)) + (t - bb);
+    return hi;
+}
+
 // ------------------------------------------------------------------------------------------
 // Dormand-Prince 5(4), PHGD001A.cpp:16-79. p_t is a constant of motion and is not stepped.
 // ------------------------------------------------------------------------------------------
@@ -330,10 +348,14 @@ SIRIUS_HD __forceinline__ void eval(const Spacetime<T>& s, const T x[4], const T
 // path integrates the 6 components (x, y, z, p_x, p_y, p_z) and measures its error on x, y, z.
 // dense (optional, 6 values): the step's dense-output term r5 for x, y, z, p_x, p_y, p_z, with
 // which dense_eval interpolates anywhere inside the step without new evaluations.
+// lo (optional, 8 values: x^mu then p_mu): low parts of a compensated state, the state being
+// (x0 + lo[0..3], p0 + lo[4..7]). They enter every stage point, and the step's increment is
+// accumulated with an error-free sum whose low parts go to lo1. The FP32 path keeps its state
+// this way, so rounding of the position and momentum does not accumulate along the ray.
 template <class T, int FAM, bool TIME = true>
 SIRIUS_HD __forceinline__ void dp45_step(const Spacetime<T>& s, const T x0[4], const T p0[4], const Der<T>& d1,
                                  T h, T x1[4], T p1[4], Der<T>& d7, T err[4], bool want_err,
-                                 T* dense = nullptr) {
+                                 T* dense = nullptr, const T* lo = nullptr, T* lo1 = nullptr) {
     constexpr double a21 = 1.0 / 5.0;
     constexpr double a31 = 3.0 / 40.0, a32 = 9.0 / 40.0;
     constexpr double a41 = 44.0 / 45.0, a42 = -56.0 / 15.0, a43 = 32.0 / 9.0;
@@ -351,44 +373,43 @@ SIRIUS_HD __forceinline__ void dp45_step(const Spacetime<T>& s, const T x0[4], c
     xt[0] = x0[0];
     x1[0] = x0[0];
 #pragma unroll
-    for (int i = I0; i < 4; ++i) xt[i] = x0[i] + h * (T(a21) * d1.k[i]);
+    for (int i = I0; i < 4; ++i) xt[i] = stage_pt(x0[i], h * (T(a21) * d1.k[i]), lo, i);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) pt[i + 1] = p0[i + 1] + h * (T(a21) * d1.f[i]);
+    for (int i = 0; i < 3; ++i) pt[i + 1] = stage_pt(p0[i + 1], h * (T(a21) * d1.f[i]), lo, 5 + i);
     eval<T, FAM>(s, xt, pt, d2);
 #pragma unroll
-    for (int i = I0; i < 4; ++i) xt[i] = x0[i] + h * (T(a31) * d1.k[i] + T(a32) * d2.k[i]);
+    for (int i = I0; i < 4; ++i) xt[i] = stage_pt(x0[i], h * (T(a31) * d1.k[i] + T(a32) * d2.k[i]), lo, i);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) pt[i + 1] = p0[i + 1] + h * (T(a31) * d1.f[i] + T(a32) * d2.f[i]);
+    for (int i = 0; i < 3; ++i) pt[i + 1] = stage_pt(p0[i + 1], h * (T(a31) * d1.f[i] + T(a32) * d2.f[i]), lo, 5 + i);
     eval<T, FAM>(s, xt, pt, d3);
 #pragma unroll
-    for (int i = I0; i < 4; ++i) xt[i] = x0[i] + h * (T(a41) * d1.k[i] + T(a42) * d2.k[i] + T(a43) * d3.k[i]);
+    for (int i = I0; i < 4; ++i) xt[i] = stage_pt(x0[i], h * (T(a41) * d1.k[i] + T(a42) * d2.k[i] + T(a43) * d3.k[i]), lo, i);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) pt[i + 1] = p0[i + 1] + h * (T(a41) * d1.f[i] + T(a42) * d2.f[i] + T(a43) * d3.f[i]);
+    for (int i = 0; i < 3; ++i) pt[i + 1] = stage_pt(p0[i + 1], h * (T(a41) * d1.f[i] + T(a42) * d2.f[i] + T(a43) * d3.f[i]), lo, 5 + i);
     eval<T, FAM>(s, xt, pt, d4);
 #pragma unroll
     for (int i = I0; i < 4; ++i)
-        xt[i] = x0[i] + h * (T(a51) * d1.k[i] + T(a52) * d2.k[i] + T(a53) * d3.k[i] + T(a54) * d4.k[i]);
+        xt[i] = stage_pt(x0[i], h * (T(a51) * d1.k[i] + T(a52) * d2.k[i] + T(a53) * d3.k[i] + T(a54) * d4.k[i]), lo, i);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-        pt[i + 1] = p0[i + 1] + h * (T(a51) * d1.f[i] + T(a52) * d2.f[i] + T(a53) * d3.f[i] + T(a54) * d4.f[i]);
+        pt[i + 1] = stage_pt(p0[i + 1], h * (T(a51) * d1.f[i] + T(a52) * d2.f[i] + T(a53) * d3.f[i] + T(a54) * d4.f[i]), lo, 5 + i);
     eval<T, FAM>(s, xt, pt, d5);
 #pragma unroll
     for (int i = I0; i < 4; ++i)
-        xt[i] = x0[i] + h * (T(a61) * d1.k[i] + T(a62) * d2.k[i] + T(a63) * d3.k[i] + T(a64) * d4.k[i] +
-                             T(a65) * d5.k[i]);
+        xt[i] = stage_pt(x0[i], h * (T(a61) * d1.k[i] + T(a62) * d2.k[i] + T(a63) * d3.k[i] + T(a64) * d4.k[i] + T(a65) * d5.k[i]), lo, i);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-        pt[i + 1] = p0[i + 1] + h * (T(a61) * d1.f[i] + T(a62) * d2.f[i] + T(a63) * d3.f[i] + T(a64) * d4.f[i] +
-                                     T(a65) * d5.f[i]);
+        pt[i + 1] = stage_pt(p0[i + 1], h * (T(a61) * d1.f[i] + T(a62) * d2.f[i] + T(a63) * d3.f[i] + T(a64) * d4.f[i] + T(a65) * d5.f[i]), lo, 5 + i);
     eval<T, FAM>(s, xt, pt, d6);
     p1[0] = p0[0];
 #pragma unroll
     for (int i = I0; i < 4; ++i)
-        x1[i] = x0[i] + h * (T(b1) * d1.k[i] + T(b3) * d3.k[i] + T(b4) * d4.k[i] + T(b5) * d5.k[i] + T(b6) * d6.k[i]);
+        x1[i] = sum_lo(x0[i], h * (T(b1) * d1.k[i] + T(b3) * d3.k[i] + T(b4) * d4.k[i] + T(b5) * d5.k[i] +
+                                   T(b6) * d6.k[i]), lo, lo1, i);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-        p1[i + 1] = p0[i + 1] + h * (T(b1) * d1.f[i] + T(b3) * d3.f[i] + T(b4) * d4.f[i] + T(b5) * d5.f[i] +
-                                     T(b6) * d6.f[i]);
+        p1[i + 1] = sum_lo(p0[i + 1], h * (T(b1) * d1.f[i] + T(b3) * d3.f[i] + T(b4) * d4.f[i] + T(b5) * d5.f[i] +
+                                           T(b6) * d6.f[i]), lo, lo1, 5 + i);
     eval<T, FAM>(s, x1, p1, d7);
     if (want_err) {
         err[0] = T(0);

@@ -368,9 +368,11 @@ int prepare_batch(Handle& H, const SiriusHipCamera* cams, int n_cam, const Siriu
     tp.emission = B.family == SIRIUS_METRIC_KERR_SCHILD ? o.emission : SIRIUS_EMISSION_NONE;
     tp.sym_order = o.symplectic_order;
     tp.n_cam = n_cam;
-    // escalation machinery of the FP32 path: on when any criterion is (a zero criterion never fires)
+    // escalation machinery of the FP32 path: always on for the wormhole throat and the warp wall;
+    // for Kerr-Schild on when any of its criteria is (a zero criterion never fires)
     tp.escalate = (o.precision == SIRIUS_PRECISION_FP32_ESCALATE &&
-                   (o.escalate_margin > 0 || o.escalate_band > 0 || o.escalate_disk_edge > 0)) ? 1 : 0;
+                   (B.family != SIRIUS_METRIC_KERR_SCHILD || o.escalate_margin > 0 || o.escalate_band > 0 ||
+                    o.escalate_disk_edge > 0)) ? 1 : 0;
     tp.n_rays = off;
     if (tp.emission == SIRIUS_EMISSION_VOLUMETRIC && !(o.vol_h_over_r > 0 && o.vol_r_max > o.vol_r_min))
         return H.fail(SIRIUS_HIP_E_INVALID, "volumetric parameters");
@@ -534,8 +536,9 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
             return H.fail(SIRIUS_HIP_E_UNSUPPORTED, "the surrogate serves whole Kerr-Schild batches without volumetric emission");
         if (sur && (!H.mlp || !H.mlp->ready))
             return H.fail(SIRIUS_HIP_E_STATE, "use_surrogate needs a network (sirius_hip_mlp_init / set_params / surrogate_train)");
-        bool gate = !sur && !list_dev && B.family == SIRIUS_METRIC_KERR_SCHILD && B.tp.escalate &&
-                    (o.escalate_band > 0 || o.critical_band > 0);
+        // the gate runs for every whole Kerr-Schild batch: besides the critical flags and the FP64
+        // band it orders the queue (rays near the critical curve, the longest, first)
+        bool gate = !sur && !list_dev && B.family == SIRIUS_METRIC_KERR_SCHILD;
         bool band = (gate || sur) && o.escalate_band > 0;
         TraceParams t32 = B.tp;
         t32.gate_flags = (gate || sur) ? 1 : 0;
@@ -853,7 +856,7 @@ void sirius_hip_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->vol_r_min = 2.0;
     o->vol_r_max = 50.0;
     o->vol_tau_max = 10.0;
-    o->escalate_disk_edge = 0.02;
+    o->escalate_disk_edge = 0.0;
     o->escalate_rtol = 0.0;
     o->critical_band = 0.5;
     o->step_radial_exp = 0.0;

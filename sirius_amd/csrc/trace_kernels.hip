@@ -42,14 +42,15 @@ SIRIUS_HD __forceinline__ double null_drift64(const CamConst& c, const double x[
 // direction of the spatial momentum: p_i -> sigma p_i with H(sigma) = 0, i.e.
 // A sigma^2 + B sigma + C = 0, A = q.G q, B = 2 p_t (G q)^t, C = p_t^2 G^{tt}.
 template <int FAM>
-SIRIUS_HD __forceinline__ void renorm_energy64(const Spacetime<double>& s, const float xf[4], float pf[4]) {
-    double x[4] = {xf[0], xf[1], xf[2], xf[3]};
-    double q[4] = {0.0, pf[1], pf[2], pf[3]};
+SIRIUS_HD __forceinline__ void renorm_energy64(const Spacetime<double>& s, const float xf[4], float pf[4], float lo[8]) {
+    double x[4], q[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < 4; ++i) x[i] = (double)xf[i] + (double)lo[i];
+    for (int i = 1; i < 4; ++i) q[i] = (double)pf[i] + (double)lo[4 + i];
     double e[4] = {1.0, 0.0, 0.0, 0.0};
     double kq[4], ke[4];
     raise_index<double, FAM>(s, x, q, kq);
     raise_index<double, FAM>(s, x, e, ke);
-    double pt = pf[0];
+    double pt = (double)pf[0] + (double)lo[4];
     double A = q[1] * kq[1] + q[2] * kq[2] + q[3] * kq[3];
     double B = 2.0 * pt * kq[0];
     double C = pt * pt * ke[0];
@@ -59,9 +60,11 @@ SIRIUS_HD __forceinline__ void renorm_energy64(const Spacetime<double>& s, const
     double s1 = (-B + sq) / (2.0 * A), s2 = (-B - sq) / (2.0 * A);
     double sg = fabs(s1 - 1.0) < fabs(s2 - 1.0) ? s1 : s2;
     if (!(sg > 0.5 && sg < 2.0)) return;
-    pf[1] = (float)(q[1] * sg);
-    pf[2] = (float)(q[2] * sg);
-    pf[3] = (float)(q[3] * sg);
+    for (int i = 1; i < 4; ++i) {
+        double v = q[i] * sg;
+        pf[i] = (float)v;
+        lo[4 + i] = (float)(v - (double)pf[i]);
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -73,6 +76,7 @@ SIRIUS_HD __forceinline__ void renorm_energy64(const Spacetime<double>& s, const
 template <class T, int FAM>
 struct Ray {
     T x[4], p[4];
+    T lo[8];        // FP32: low parts of the compensated state (x^mu, then p_mu); unused in FP64
     Der<T> d;
     T h, lam;
     T tau;          // volumetric optical depth
@@ -92,7 +96,20 @@ SIRIUS_HD __forceinline__ void ray_start(Ray<T, FAM>& R, const CamConst* cams, i
     R.id = id;
     R.cam = cam;
     Spacetime<T> s = make_spacetime<T>(c);
-    init_state<T>(c, s, (int)(pix % c.width), (int)(pix / c.width), R.x, R.p);
+    if constexpr (sizeof(T) == 4) {
+        // FP32: the initial state in FP64, kept as high + low parts
+        Spacetime<double> sd = make_spacetime<double>(c);
+        double xd[4], pd[4];
+        init_state<double>(c, sd, (int)(pix % c.width), (int)(pix / c.width), xd, pd);
+        for (int i = 0; i < 4; ++i) {
+            R.x[i] = (T)xd[i];
+            R.p[i] = (T)pd[i];
+            R.lo[i] = (T)(xd[i] - (double)R.x[i]);
+            R.lo[4 + i] = (T)(pd[i] - (double)R.p[i]);
+        }
+    } else {
+        init_state<T>(c, s, (int)(pix % c.width), (int)(pix / c.width), R.x, R.p);
+    }
     eval<T, FAM>(s, R.x, R.p, R.d);
     R.h = T(tp.h_init);
     if (tp.far_slope > 0) {
@@ -125,7 +142,12 @@ SIRIUS_HD __forceinline__ void ray_store(const Ray<T, FAM>& R, const DevOut& o) 
     o.steps[id] = R.attempts;
     o.rejected[id] = R.rejected;
     o.landings[id] = R.landings;
-    for (int i = 0; i < 4; ++i) { o.state[8 * id + i] = (double)R.x[i]; o.state[8 * id + 4 + i] = (double)R.p[i]; }
+    for (int i = 0; i < 4; ++i) {
+        double xi = (double)R.x[i], pi = (double)R.p[i];
+        if constexpr (sizeof(T) == 4) { xi += (double)R.lo[i]; pi += (double)R.lo[4 + i]; }
+        o.state[8 * id + i] = xi;
+        o.state[8 * id + 4 + i] = pi;
+    }
 }
 
 // Outputs of one finished ray from its stored class and state (all FP64).
@@ -279,11 +301,19 @@ SIRIUS_HD __forceinline__ T land(Ray<T, FAM>& R, const Spacetime<T>& s, bool pla
     return th;
 }
 
+// The step's low parts become the state's (t and p_t are not stepped: their low parts stay).
+template <class T>
+SIRIUS_HD __forceinline__ void commit_lo(T lo[8], const T lo1[8]) {
+#pragma unroll
+    for (int j = 1; j < 8; ++j)
+        if (j != 4) lo[j] = lo1[j];
+}
+
 // FP32 landing: the state at fraction th of the step from the step's dense output (x, y, z and
 // p_x, p_y, p_z; t and p_t are not stepped in FP32). Landings end the ray.
 template <class T, int FAM>
 SIRIUS_HD __forceinline__ void land_dense(Ray<T, FAM>& R, const T x1[4], const T p1[4], const Der<T>& d1,
-                                          const T dn[6], T th) {
+                                          const T dn[6], const T lo1[8], T th) {
     T xl[4], pl[4];
     dense_pos(R.x, R.d.k, x1, d1.k, dn, R.h, th, xl);
     pl[0] = R.p[0];
@@ -291,6 +321,11 @@ SIRIUS_HD __forceinline__ void land_dense(Ray<T, FAM>& R, const T x1[4], const T
     for (int i = 1; i < 4; ++i) pl[i] = dense_eval(R.p[i], p1[i], R.h, R.d.f[i - 1], d1.f[i - 1], dn[2 + i], th);
     R.lam = fma(th, R.h, R.lam);
     for (int i = 0; i < 4; ++i) { R.x[i] = xl[i]; R.p[i] = pl[i]; }
+    // low parts of the compensated state, linear across the step (they are below the high
+    // parts' rounding unit)
+#pragma unroll
+    for (int j = 1; j < 8; ++j)
+        if (j != 4) R.lo[j] = fma(th, lo1[j] - R.lo[j], R.lo[j]);
 }
 
 // One DP45 attempt with error control, events and acceptance. Returns ST_RUN, ST_DONE (the
@@ -317,8 +352,9 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
     T x1[4], p1[4], err[4];
     Der<T> d1;
     // FP32: the step's dense output locates and lands events (no further evaluations)
-    T dn[6];
-    dp45_step<T, FAM, TIME>(s, R.x, R.p, R.d, R.h, x1, p1, d1, err, true, fp32 ? dn : nullptr);
+    T dn[6], lo1[8];
+    dp45_step<T, FAM, TIME>(s, R.x, R.p, R.d, R.h, x1, p1, d1, err, true, fp32 ? dn : nullptr,
+                            fp32 ? R.lo : nullptr, lo1);
     R.attempts++;
     T e = T(0);
 #pragma unroll
@@ -392,7 +428,7 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
         if (plane || locate) {
             if constexpr (fp32) {
                 T th = plane ? th_disk : locate_event_dense(s, kind, target, R.x, R.d.k, x1, d1.k, dn, R.h);
-                land_dense<T, FAM>(R, x1, p1, d1, dn, th);
+                land_dense<T, FAM>(R, x1, p1, d1, dn, lo1, th);
             } else {
                 T th = plane ? th_disk : locate_event(s, kind, target, R.x, R.d.k, x1, d1.k, R.h);
                 land<T, FAM>(R, s, plane, th);
@@ -400,6 +436,7 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
         } else {
             R.lam += R.h;
             for (int i = 0; i < 4; ++i) { R.x[i] = x1[i]; R.p[i] = p1[i]; }
+            if constexpr (fp32) commit_lo(R.lo, lo1);
             R.d = d1;
         }
         R.cls = ev;
@@ -419,6 +456,7 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
     if (tp.step_rexp > 0 && r1 < r0) hprop *= pow(r1 / r0, T(tp.step_rexp));
     R.h = next_step(hprop, e, T(tp.h_min), hmax2);
     for (int i = 0; i < 4; ++i) { R.x[i] = x1[i]; R.p[i] = p1[i]; }
+    if constexpr (fp32) commit_lo(R.lo, lo1);
     R.d = d1;
     if constexpr (fp32) {
         // escalation of a warp ray entering the bubble wall (|r_s - R| < 3 / sigma, or inside):
@@ -438,7 +476,7 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
         // FP64 re-imposition of the null constraint, energy preserving
         if (fabs(R.d.pk) > T(1e-6) * R.p[0] * R.p[0]) {
             Spacetime<double> sd = make_spacetime<double>(c);
-            renorm_energy64<FAM>(sd, R.x, R.p);
+            renorm_energy64<FAM>(sd, R.x, R.p, R.lo);
             eval<T, FAM>(s, R.x, R.p, R.d);
             R.flags |= SIRIUS_FLAG_RENORM;
         }
@@ -549,17 +587,12 @@ __global__ __launch_bounds__(256) void k_trace_persistent(SIRIUS_PERSISTENT_PARA
     persistent_body<T, FAM, EMIT, MULTI>(SIRIUS_PERSISTENT_ARGS);
 }
 
-// FP32 variants: with SIRIUS_FP32_WAVES > 0 the register budget is capped for that many waves per
-// SIMD. Measured at 1080p: uncapped (186 VGPRs, 2 waves) 3.68 ms, 3 waves (168 VGPRs, 8 spills)
-// 4.09 ms, 4 waves (128 VGPRs, 57 spills) 4.50 ms; so the default leaves it uncapped.
-#ifndef SIRIUS_FP32_WAVES
-#define SIRIUS_FP32_WAVES 0
-#endif
+// FP32 variants. The thin-disk tracers, whose natural allocation lies just above the 168 VGPRs
+// of three waves per SIMD, are held to that budget (no spills at 168; three waves instead of two);
+// the others keep the compiler's allocation.
+constexpr int fp32_min_waves(int emit) { return emit == SIRIUS_EMISSION_THIN_DISK ? 3 : 1; }
 template <int FAM, int EMIT, bool MULTI>
-__global__ __launch_bounds__(256)
-#if SIRIUS_FP32_WAVES > 0
-__attribute__((amdgpu_waves_per_eu(SIRIUS_FP32_WAVES)))
-#endif
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fp32_min_waves(EMIT))))
 void k_trace_persistent32(SIRIUS_PERSISTENT_PARAMS) {
     persistent_body<float, FAM, EMIT, MULTI>(SIRIUS_PERSISTENT_ARGS);
 }
@@ -690,6 +723,7 @@ __global__ __launch_bounds__(256) void k_symplectic_grid(const CamConst* __restr
         rprev = r;
     }
     for (int i = 0; i < 4; ++i) { R.x[i] = q[i]; R.p[i] = p[i]; }
+    for (int j = 0; j < 8; ++j) R.lo[j] = T(0);  // the symplectic state is not compensated
     eval<T, FAM>(s, R.x, R.p, R.d);
     R.attempts = n;
     R.cls = cls;

@@ -102,8 +102,11 @@ def test_persistent_queue_equals_grid_launch(acc):
     a, sa = acc.trace_batch([cam], [m], o)
     o.schedule = abi.SCHEDULE_GRID
     b, sb = acc.trace_batch([cam], [m], o)
-    for k in ("cls", "flags", "theta", "phi", "lam", "disk_r", "redshift", "steps", "intensity"):
+    for k in ("cls", "theta", "phi", "lam", "disk_r", "redshift", "steps", "intensity"):
         assert np.array_equal(getattr(a, k), getattr(b, k)), k
+    # the critical flag comes from the persistent schedule's gate (the grid schedule has none)
+    assert np.array_equal(a.flags & ~np.uint8(abi.FLAG_CRITICAL), b.flags)
+    assert ((a.flags & abi.FLAG_CRITICAL) != 0).any()
     assert sa["steps_bulk"] == sb["steps_bulk"]
     # refill thresholds and launch shapes do not change results either
     o.schedule = abi.SCHEDULE_PERSISTENT
@@ -121,7 +124,8 @@ def test_escalated_rays_are_bitwise_fp64(acc):
     m = abi.metric_ks(1.0, 0.9)
     cam = abi.camera(50.0, 75.0, 20.0, 480, 270, disk=(0.0, 30.0))
     o = O.default_opts(abi.PRECISION_FP32_ESCALATE)
-    o.escalate_band, o.escalate_margin, o.escalate_rtol = 0.25, 1.0, 1e-8  # every criterion on
+    o.escalate_band, o.escalate_margin, o.escalate_disk_edge = 0.25, 1.0, 0.02  # every criterion on
+    o.escalate_rtol = 1e-8
     got, st = acc.trace_batch([cam], [m], o)
     o64 = O.default_opts(abi.PRECISION_FP32_ESCALATE)
     o64.precision = abi.PRECISION_FP64
