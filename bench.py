@@ -41,6 +41,22 @@ def workload():
     return cam, metric
 
 
+def calibration_copy(nbytes):
+    """One device-to-device copy of nbytes through the HIP runtime (its copy kernel reads and
+    writes exactly nbytes), for scaling the HBM byte counters of a profile."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), nbytes) == 0 and hip.hipMalloc(C.byref(b), nbytes) == 0
+    assert hip.hipMemcpy(b, a, nbytes, 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def time_frames(acc, cam, metric, opts, steps, warmup, n_stats=3):
     """Wall time per frame over `steps` frames (no statistics pass inside the timed region),
     then `n_stats` frames with statistics for the HIP-event kernel times and step counts."""
@@ -153,6 +169,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device (a copy kernel of known size that "
+                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs scale against)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this backend runs on one GPU; --gpus {args.gpus} is not supported "
@@ -162,6 +181,8 @@ def main():
     from sirius_amd import abi
     from sirius_amd.accelerator import HipAccelerator, default_opts
     acc = HipAccelerator()
+    if args.copy_calibration > 0:
+        calibration_copy(args.copy_calibration << 20)
     cam, metric = workload()
     n_rays = W * H
 

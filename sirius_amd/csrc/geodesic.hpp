@@ -60,7 +60,8 @@ struct TraceParams {
     double esc_edge;
     double step_rexp;  // inward step-proposal exponent (0: the reference's controller)
     int32_t max_steps, n_bins, emission, sym_order;
-    int32_t n_cam, escalate, gate_flags, pad1;  // gate_flags: the gate wrote per-ray flags
+    int32_t n_cam, escalate, gate_flags;  // gate_flags: the gate wrote per-ray flags
+    int32_t spectrum32;  // FP32 path: thin-disk Planck bins of FP32-traced rays in FP32 (g, T in FP64)
     int64_t n_rays;
     // persistent queue: rays listed here are taken before the kernel's own list (the rays
     // nearest the critical curve, the longest, start first); null for none
@@ -647,21 +648,40 @@ SIRIUS_HD __forceinline__ double emitter_ut(const CamConst& c, const Spacetime<d
     return vv < 0 ? 1.0 / sqrt(-vv) : -1.0;
 }
 
-// Thin disk: g = (p.u_obs)/(p.u_emit), I_k = x_k^3 / (e^{x_k/(g tau)} - 1).
+// Spectral bin centres x_k = x_min (x_max / x_min)^((k + 1/2) / n_bins): the first centre and the
+// ratio of neighbours (each centre is the previous one times the ratio).
+SIRIUS_HD __forceinline__ void bin_geometry(const TraceParams& tp, double& x0, double& q) {
+    q = exp(log(tp.x_max / tp.x_min) / tp.n_bins);
+    x0 = tp.x_min * sqrt(q);
+}
+
+// Thin disk: g = (p.u_obs)/(p.u_emit), I_k = x_k^3 / (e^{x_k/(g tau)} - 1). The redshift and the
+// temperature are FP64; the Planck bins are FP64, or FP32 (bins32: rays of the FP32 path, whose
+// intensities are held to a 1e-3 relative tolerance).
 SIRIUS_HD __forceinline__ void disk_emission(const CamConst& c, const TraceParams& tp, const double x[4], const double p[4],
-                                     double r, double& gf, float* I, int stride) {
+                                     double r, double& gf, float* I, int stride, bool bins32) {
     Spacetime<double> s = make_spacetime<double>(c);
     double Om = kepler_omega(c, r);
     double ut = emitter_ut(c, s, x, Om);
     double pphi = x[1] * p[2] - x[2] * p[1];
     gf = (p[0] * c.obs_norm) / (ut * (p[0] + Om * pphi));
     double f = page_thorne_f(c, r) / r;
-    double tau = (c.fmax > 0 && f > 0) ? pow(f / c.fmax, 0.25) : 0.0;
-    double lr = log(tp.x_max / tp.x_min);
-    for (int k = 0; k < tp.n_bins; ++k) {
-        double xk = tp.x_min * exp(lr * (k + 0.5) / tp.n_bins);
-        double v = (tau > 0 && gf > 0 && ut > 0) ? xk * xk * xk / expm1(xk / (gf * tau)) : 0.0;
-        I[(int64_t)k * stride] = (float)v;
+    double tau = (c.fmax > 0 && f > 0) ? sqrt(sqrt(f / c.fmax)) : 0.0;  // (f / f_max)^(1/4)
+    const bool live = tau > 0 && gf > 0 && ut > 0;
+    double xk, q;
+    bin_geometry(tp, xk, q);
+    const double inv = live ? 1.0 / (gf * tau) : 0.0;
+    for (int k = 0; k < tp.n_bins; ++k, xk *= q) {
+        float v = 0.f;
+        if (live) {
+            if (bins32) {
+                float xf = (float)xk;
+                v = xf * xf * xf / expm1f((float)(xk * inv));
+            } else {
+                v = (float)(xk * xk * xk / expm1(xk * inv));
+            }
+        }
+        I[(int64_t)k * stride] = v;
     }
 }
 

@@ -361,6 +361,7 @@ int prepare_batch(Handle& H, const SiriusHipCamera* cams, int n_cam, const Siriu
     tp.esc_edge = o.escalate_disk_edge;
     tp.step_rexp = o.step_radial_exp;
     tp.gate_flags = 0;
+    tp.spectrum32 = o.precision == SIRIUS_PRECISION_FP32_ESCALATE ? 1 : 0;
     tp.first_list = nullptr;
     tp.first_count = nullptr;
     tp.max_steps = o.max_steps;

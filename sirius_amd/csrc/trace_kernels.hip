@@ -191,7 +191,9 @@ SIRIUS_HD __forceinline__ void ray_outputs(const CamConst& c, const TraceParams&
         if (cls == SIRIUS_RAY_DISK) {
             double gf;
             double rd = kerr_r(sd, xd[1], xd[2], xd[3]);  // FP64 radius of the landed state
-            disk_emission(c, tp, xd, pd, rd, gf, o.intensity + id, (int)o.n_rays);
+            // rays re-traced in FP64 keep FP64 bins
+            bool bins32 = tp.spectrum32 && !(o.flags[id] & SIRIUS_FLAG_ESCALATED);
+            disk_emission(c, tp, xd, pd, rd, gf, o.intensity + id, (int)o.n_rays, bins32);
             o.redshift[id] = (float)gf;
         } else {
             for (int k = 0; k < tp.n_bins; ++k) o.intensity[(int64_t)k * o.n_rays + id] = 0.f;
@@ -258,9 +260,9 @@ SIRIUS_HD __forceinline__ void vol_accumulate(Ray<T, FAM>& R, const CamConst& c,
         double dtau = tp.vol_kappa * (double)rho * nue * dl;
         double w = (double)rho * (nuo * nuo * nuo / (nue * nue)) * dl * exp(-((double)R.tau + 0.5 * dtau));
         double tv = pow(6.0 / (double)r, 0.75);
-        double lr = log(tp.x_max / tp.x_min);
-        for (int k = 0; k < tp.n_bins; ++k) {
-            double xk = tp.x_min * exp(lr * (k + 0.5) / tp.n_bins);
+        double xk, qk;
+        bin_geometry(tp, xk, qk);
+        for (int k = 0; k < tp.n_bins; ++k, xk *= qk) {
             double xe = xk / gf;
             o.vol_acc[(int64_t)k * o.n_rays + R.id] += w * xe * xe * xe / expm1(xe / tv);
         }
