@@ -2,12 +2,15 @@
 FOV 20 deg, Novikov-Thorne disk from the ISCO to 30 M, 16 spectral bins).
 
 One step = one full frame traced on the GPU with every result left in device memory.
-  value          the FP32 path with FP64 escalation (and the MLP surrogate once trained)
+  value          the FP32 path (compensated FP32 state, FP64 null-constraint re-normalisation,
+                 escalation criteria at their defaults)
   fp64_plain     the plain FP64 integrator on the same frame (the comparison baseline)
   accuracy       the FP32 frame against the FP64 frame, pixel by pixel
+  with_edge_escalation
+                 the same frame with inner-edge disk crossings re-traced in FP64
   roofline       dominant kernel (the FP32 persistent tracer): algorithmic flops per launch
                  (1,326 per Kerr-Schild DP45 attempt) over its HIP-event time, against the
-                 FP32 vector peak
+                 FP32 vector peak; traffic from the profile's HBM byte counters
   cpu_baseline   the reference's own CPU integrator (oracle/_ref, compiled from its sources)
                  on every pixel of the frame, on this host's cores (about 15 s)
   surrogate      ablation: the FP32 path with the MLP surrogate answering far-field rays, after
@@ -15,7 +18,8 @@ One step = one full frame traced on the GPU with every result left in device mem
                  (training excluded from the frame time, reported apart), with its accuracy on
                  the rays it answered
 
-python bench.py [--gpus 1] [--steps K] [--warmup W]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-accuracy]
+                [--no-surrogate] [--copy-calibration MIB]
 """
 import argparse
 import json
@@ -31,6 +35,9 @@ sys.path.insert(0, ROOT)
 METRIC = "Mrays/s 1080p Kerr a=0.9: plain FP64 vs FP32+surrogate at ≤1e-4 rad error"
 FP32_VALU_PEAK_TFLOPS = 157.3
 FP64_VALU_PEAK_TFLOPS = 78.6
+# HBM bytes per launch of the FP32 bulk tracer on this frame, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs scaled by a copy kernel of known size (profiles/trace_kerr_a0.9_1080p_counters.md)
+BULK_HBM_BYTES = 231.5e6
 W, H = 1920, 1080
 
 
@@ -220,7 +227,9 @@ def main():
                    "surrogate": False},
         "roofline": {"bound": "valu", "kernel": "k_trace_persistent32<0,1,false>", "achieved": achieved,
                      "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_VALU_PEAK_TFLOPS,
-                     "traffic": None, "flops_per_launch": s["flops"], "avg_launch_ms": kern_ms},
+                     "traffic": BULK_HBM_BYTES, "traffic_unit": "bytes per launch",
+                     "traffic_source": "profiles/trace_kerr_a0.9_1080p_counters.md",
+                     "flops_per_launch": s["flops"], "avg_launch_ms": kern_ms},
         "geodesic_steps_per_s": s["steps_bulk"] / (kern_ms * 1e-3),
         "rays_escalated": s["rays_escalated"],
         "escalate_ms": float(np.mean([x["escalate_ms"] for x in st32])),
