@@ -454,13 +454,24 @@ SIRIUS_HD __forceinline__ T dense_eval(T y0, T y1, T h, T k0, T k1, T r5, T th) 
     return fma(th, u, y0);
 }
 
+// x^e for x > 0. FP32 on the device: the hardware base-2 logarithm and exponential (step-size
+// factors are heuristics; a general powf spends ~40 instructions on range and special cases).
+// FP64, and host builds of this header: pow.
+template <class T>
+SIRIUS_HD __forceinline__ T pow_pos(T x, T e) {
+#ifdef __HIP_DEVICE_COMPILE__
+    if constexpr (sizeof(T) == 4) return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(x));
+#endif
+    return pow(x, e);
+}
+
 // computeOptimalStep (PHGD001A.cpp:471-483): h * clamp(0.9 err^-1/5, 0.1, 2), clamped.
 template <class T>
 SIRIUS_HD __forceinline__ T next_step(T h, T err, T hmin, T hmax) {
     T hn;
     if (err < T(1e-15)) hn = h * T(2);
     else {
-        T fac = T(0.9) * pow(err, T(-0.2));
+        T fac = T(0.9) * pow_pos(err, T(-0.2));
         fac = fac < T(0.1) ? T(0.1) : (fac > T(2) ? T(2) : fac);
         hn = h * fac;
     }

@@ -455,7 +455,7 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
     T hprop = R.h;
     // an inward step predicts the next step's error growth: the local error of a step of a
     // given length rises steeply as r falls, so the proposal scales with (r1 / r0)^exp
-    if (tp.step_rexp > 0 && r1 < r0) hprop *= pow(r1 / r0, T(tp.step_rexp));
+    if (tp.step_rexp > 0 && r1 < r0) hprop *= pow_pos(r1 / r0, T(tp.step_rexp));
     R.h = next_step(hprop, e, T(tp.h_min), hmax2);
     for (int i = 0; i < 4; ++i) { R.x[i] = x1[i]; R.p[i] = p1[i]; }
     if constexpr (fp32) commit_lo(R.lo, lo1);
