@@ -102,6 +102,12 @@ template <class T>
 struct D3 {
     T v, g0, g1, g2;
 };
+// a * b + c with one rounding in the working precision (fmaf for FP32, also in host builds of
+// this header, where a plain fma call on floats would resolve to the double overload).
+template <class T> SIRIUS_HD __forceinline__ T tfma(T a, T b, T c) {
+    if constexpr (sizeof(T) == 4) return fmaf(a, b, c);
+    else return fma(a, b, c);
+}
 template <class T> SIRIUS_HD __forceinline__ D3<T> dcst(T v) { return {v, T(0), T(0), T(0)}; }
 template <class T> SIRIUS_HD __forceinline__ D3<T> operator+(D3<T> a, D3<T> b) { return {a.v + b.v, a.g0 + b.g0, a.g1 + b.g1, a.g2 + b.g2}; }
 template <class T> SIRIUS_HD __forceinline__ D3<T> operator-(D3<T> a, D3<T> b) { return {a.v - b.v, a.g0 - b.g0, a.g1 - b.g1, a.g2 - b.g2}; }
@@ -111,6 +117,17 @@ template <class T> SIRIUS_HD __forceinline__ D3<T> operator*(D3<T> a, D3<T> b) {
 template <class T> SIRIUS_HD __forceinline__ D3<T> operator*(D3<T> a, T s) { return {a.v * s, a.g0 * s, a.g1 * s, a.g2 * s}; }
 template <class T> SIRIUS_HD __forceinline__ D3<T> operator+(D3<T> a, T s) { return {a.v + s, a.g0, a.g1, a.g2}; }
 template <class T> SIRIUS_HD __forceinline__ D3<T> operator-(D3<T> a, T s) { return {a.v - s, a.g0, a.g1, a.g2}; }
+// a * S and a + c S for a coordinate seed S = (s; unit tangent along AXIS): only the nonzero
+// tangent terms are formed. The compiler cannot drop a product with a zero tangent itself
+// (0 * t is not 0 for an infinite or NaN t), and the seeds' zero tangents would otherwise cost a
+// multiply or an add each in every evaluation.
+template <int AXIS, class T> SIRIUS_HD __forceinline__ D3<T> dmul_seed(D3<T> a, T s) {
+    return {a.v * s, AXIS == 0 ? tfma(a.g0, s, a.v) : a.g0 * s, AXIS == 1 ? tfma(a.g1, s, a.v) : a.g1 * s,
+            AXIS == 2 ? tfma(a.g2, s, a.v) : a.g2 * s};
+}
+template <int AXIS, class T> SIRIUS_HD __forceinline__ D3<T> dadd_seed(D3<T> a, T c, T s) {
+    return {tfma(c, s, a.v), AXIS == 0 ? a.g0 + c : a.g0, AXIS == 1 ? a.g1 + c : a.g1, AXIS == 2 ? a.g2 + c : a.g2};
+}
 template <class T> SIRIUS_HD __forceinline__ D3<T> dinv(D3<T> a) {
     T r = T(1) / a.v, r2 = -r * r;
     return {r, a.g0 * r2, a.g1 * r2, a.g2 * r2};
@@ -191,22 +208,27 @@ SIRIUS_HD __forceinline__ void metric_cov(const Spacetime<T>& s, const T x[4], T
 // ------------------------------------------------------------------------------------------
 template <class T>
 SIRIUS_HD __forceinline__ T rhs_ks(const Spacetime<T>& s, const T x[4], const T p[4], T k[4], T dp[3]) {
-    D3<T> X = {x[1], T(1), T(0), T(0)};
-    D3<T> Y = {x[2], T(0), T(1), T(0)};
-    D3<T> Z = {x[3], T(0), T(0), T(1)};
-    D3<T> R2 = X * X + Y * Y + Z * Z;
-    D3<T> w = R2 - s.a2;
-    D3<T> disc = w * w + Z * Z * (T(4) * s.a2);
-    if (disc.v < T(1e-20)) disc = dcst(T(1e-20));
+    // derivatives along the coordinate seeds x, y, z; products with a seed go through dmul_seed /
+    // dadd_seed so no zero tangent is multiplied
+    const T X = x[1], Y = x[2], Z = x[3];
+    const T R2v = X * X + Y * Y + Z * Z;
+    D3<T> w = {R2v - s.a2, T(2) * X, T(2) * Y, T(2) * Z};   // R^2 - a^2
+    D3<T> disc = w * w;                                      // (R^2 - a^2)^2 + 4 a^2 z^2
+    disc.v = tfma(T(4) * s.a2, Z * Z, disc.v);
+    disc.g2 = tfma(T(8) * s.a2, Z, disc.g2);
+    disc.v = fmax(disc.v, T(1e-20));  // guards only the ring singularity, deep inside the horizon
     D3<T> r2 = (w + dsqrt(disc)) * T(0.5);
-    if (r2.v < T(1e-10)) r2 = dcst(T(1e-10));
+    r2.v = fmax(r2.v, T(1e-10));
     D3<T> r = dsqrt(r2);
     D3<T> iden = dinv(r2 + s.a2);
-    D3<T> l1 = (r * X + Y * s.a) * iden;
-    D3<T> l2 = (r * Y - X * s.a) * iden;
-    D3<T> l3 = Z * dinv(r);
+    D3<T> l1 = dadd_seed<1>(dmul_seed<0>(r, X), s.a, Y) * iden;   // (r x + a y) / (r^2 + a^2)
+    D3<T> l2 = dadd_seed<0>(dmul_seed<1>(r, Y), -s.a, X) * iden;  // (r y - a x) / (r^2 + a^2)
+    D3<T> l3 = dmul_seed<2>(dinv(r), Z);                          // z / r
     D3<T> num = (r * (T(2) * s.M) - s.Q2) * r2;
-    D3<T> H = num * dinv(r2 * r2 + Z * Z * s.a2);
+    D3<T> den = r2 * r2;                                          // r^4 + a^2 z^2
+    den.v = tfma(s.a2 * Z, Z, den.v);
+    den.g2 = tfma(T(2) * s.a2, Z, den.g2);
+    D3<T> H = num * dinv(den);
     // k = (eta - H l l) p, l^mu = (-1, l1, l2, l3)
     T lp = -p[0] + l1.v * p[1] + l2.v * p[2] + l3.v * p[3];
     T Hl = H.v * lp;
@@ -217,7 +239,7 @@ SIRIUS_HD __forceinline__ T rhs_ks(const Spacetime<T>& s, const T x[4], const T 
     T lam3 = T(0);
     if (s.Lambda != T(0)) {
         // g = g0 + c e0 e0^T with c = -Lambda R^2 / 3; g^-1 by Sherman-Morrison.
-        T c = -s.Lambda * R2.v / T(3);
+        T c = -s.Lambda * R2v / T(3);
         T u0 = -T(1) - H.v;
         T fac = c * k[0] / (T(1) + c * u0);
         k[0] -= fac * u0;
@@ -445,13 +467,13 @@ SIRIUS_HD __forceinline__ void dp45_step(const Spacetime<T>& s, const T x0[4], c
 template <class T>
 SIRIUS_HD __forceinline__ T dense_eval(T y0, T y1, T h, T k0, T k1, T r5, T th) {
     T d = y1 - y0;
-    T b = fma(h, k0, -d);
-    T c = d - fma(h, k1, b);
+    T b = tfma(h, k0, -d);
+    T c = d - tfma(h, k1, b);
     T t1 = T(1) - th;
-    T u = fma(t1, r5, c);
-    u = fma(th, u, b);
-    u = fma(t1, u, d);
-    return fma(th, u, y0);
+    T u = tfma(t1, r5, c);
+    u = tfma(th, u, b);
+    u = tfma(t1, u, d);
+    return tfma(th, u, y0);
 }
 
 // x^e for x > 0. FP32 on the device: the hardware base-2 logarithm and exponential (step-size

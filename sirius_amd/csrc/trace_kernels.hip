@@ -321,13 +321,13 @@ SIRIUS_HD __forceinline__ void land_dense(Ray<T, FAM>& R, const T x1[4], const T
     pl[0] = R.p[0];
 #pragma unroll
     for (int i = 1; i < 4; ++i) pl[i] = dense_eval(R.p[i], p1[i], R.h, R.d.f[i - 1], d1.f[i - 1], dn[2 + i], th);
-    R.lam = fma(th, R.h, R.lam);
+    R.lam = tfma(th, R.h, R.lam);
     for (int i = 0; i < 4; ++i) { R.x[i] = xl[i]; R.p[i] = pl[i]; }
     // low parts of the compensated state, linear across the step (they are below the high
     // parts' rounding unit)
 #pragma unroll
     for (int j = 1; j < 8; ++j)
-        if (j != 4) R.lo[j] = fma(th, lo1[j] - R.lo[j], R.lo[j]);
+        if (j != 4) R.lo[j] = tfma(th, lo1[j] - R.lo[j], R.lo[j]);
 }
 
 // One DP45 attempt with error control, events and acceptance. Returns ST_RUN, ST_DONE (the
