@@ -13,13 +13,16 @@ One step = one full frame traced on the GPU with every result left in device mem
                  FP32 vector peak; traffic from the profile's HBM byte counters
   cpu_baseline   the reference's own CPU integrator (oracle/_ref, compiled from its sources)
                  on every pixel of the frame, on this host's cores (about 15 s)
+  other_workloads
+                 the 64-camera random batch (volumetric, autotuned launch) and the wormhole and
+                 warp scenes at 1024x1024: Mrays/s, geodesic steps/s, null-constraint drift
   surrogate      ablation: the FP32 path with the MLP surrogate answering far-field rays, after
                  a one-time training on 2^20 integrator-labelled rays of 256 random cameras
                  (training excluded from the frame time, reported apart), with its accuracy on
                  the rays it answered
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-accuracy]
-                [--no-surrogate] [--copy-calibration MIB]
+                [--no-surrogate] [--no-extra] [--copy-calibration MIB]
 """
 import argparse
 import json
@@ -64,18 +67,20 @@ def calibration_copy(nbytes):
     hip.hipFree(b)
 
 
-def time_frames(acc, cam, metric, opts, steps, warmup, n_stats=3):
+def time_frames(acc, cam, metric, opts, steps, warmup, n_stats=3, multi=False):
     """Wall time per frame over `steps` frames (no statistics pass inside the timed region),
-    then `n_stats` frames with statistics for the HIP-event kernel times and step counts."""
+    then `n_stats` frames with statistics for the HIP-event kernel times and step counts.
+    multi: cam and metric are lists (one batch of several cameras)."""
+    cams, metrics = (cam, metric) if multi else ([cam], [metric])
     for _ in range(warmup):
-        acc.trace_resident([cam], [metric], opts, stats=False)
+        acc.trace_resident(cams, metrics, opts, stats=False)
     acc.synchronise()
     t0 = time.perf_counter()
     for _ in range(steps):
-        acc.trace_resident([cam], [metric], opts, stats=False)
+        acc.trace_resident(cams, metrics, opts, stats=False)
     acc.synchronise()
     dt = time.perf_counter() - t0
-    stats = [acc.trace_resident([cam], [metric], opts) for _ in range(n_stats)]
+    stats = [acc.trace_resident(cams, metrics, opts) for _ in range(n_stats)]
     return dt / steps, stats
 
 
@@ -98,6 +103,45 @@ def accuracy(fast, ref, crit, band=0.5):
         out["intensity_rel_err_max"] = float(rel.max())
         out["intensity_rel_err_p999"] = float(np.percentile(rel, 99.9))
         out["intensity_l1_rel_err"] = float(np.abs(If - Ir).sum() / Ir.sum())
+    return out
+
+
+def other_workloads(acc, steps, warmup):
+    """The batched and non-Kerr workloads, one GPU each, FP32 path: the 64-camera random
+    Kerr-Newman-de Sitter batch (volumetric emission, 8 bins, launch shape from the autotuner) and
+    the Morris-Thorne and Alcubierre scenes at 1024x1024, with geodesic steps/s and the largest
+    null-constraint drift |g^{mu nu} p_mu p_nu| of the final states."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from workloads import random_spacetime_batch, warp_scene, wormhole_scene
+    from sirius_amd import abi
+    from sirius_amd.accelerator import default_opts
+
+    def measure(cams, metrics, o, n_steps, n_warm):
+        n = sum(c.width * c.height for c in cams)
+        ms, st = time_frames(acc, cams[0] if len(cams) == 1 else cams, metrics[0] if len(metrics) == 1 else metrics,
+                             o, n_steps, n_warm, multi=len(cams) > 1)
+        s = st[-1]
+        k = float(np.mean([x["kernel_ms"] for x in st]))
+        return {"rays": n, "ms_per_step": ms * 1e3, "mrays_per_s": n / ms / 1e6,
+                "geodesic_steps_per_s": (s["steps_bulk"] + s["steps_escalated"]) / (ms),
+                "steps_per_ray": s["steps_bulk"] / n, "rays_escalated": s["rays_escalated"],
+                "null_drift_max": s["null_drift_max"], "null_drift_median": s["null_drift_median"],
+                "tracer_kernel_ms": k}
+
+    out = {}
+    cams, metrics = random_spacetime_batch()
+    o = default_opts(abi.PRECISION_FP32_ESCALATE)
+    o.emission = abi.EMISSION_VOLUMETRIC
+    o.n_bins = 8
+    acc.autotune(cams, metrics, o)
+    r = measure(cams, metrics, o, max(1, steps // 2), 1)
+    r["launch"] = {"block_size": o.block_size, "blocks_per_cu": o.blocks_per_cu, "refill_min": o.refill_min}
+    r["workload"] = "64 random Kerr-Newman-de Sitter cameras, volumetric, 8 bins"
+    out["batch_64_cameras"] = r
+    for name, (cam, metric) in (("wormhole_1024", wormhole_scene(1024)), ("warp_1024", warp_scene(1024))):
+        o = default_opts(abi.PRECISION_FP32_ESCALATE)
+        o.emission = abi.EMISSION_NONE
+        out[name] = measure([cam], [metric], o, steps, warmup)
     return out
 
 
@@ -176,6 +220,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched and non-Kerr workloads")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a copy kernel of known size that "
                          "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs scale against)")
@@ -252,6 +297,8 @@ def main():
             "value": n_rays / mse / 1e6, "ms_per_step": mse * 1e3, "rays_escalated": ste[-1]["rays_escalated"],
             "escalate_ms": float(np.mean([x["escalate_ms"] for x in ste])),
             "accuracy_vs_fp64": accuracy(acc.fetch_results(), ref, crit)}
+    if not args.no_extra:
+        result["other_workloads"] = other_workloads(acc, args.steps, args.warmup)
     if not args.no_surrogate and ref is not None:
         result["surrogate"] = surrogate_ablation(acc, cam, metric, ref, args.steps, args.warmup)
     if not args.no_cpu_baseline:

@@ -192,8 +192,9 @@ typedef struct SiriusHipStats {
     double total_ms;             /* HIP-event time of the whole batch                       */
     double flops;                /* algorithmic flops of the bulk kernel (fixed per step)   */
     double flops_escalated;      /* algorithmic flops of the escalation kernel              */
-    double null_drift_max;
-    double null_drift_median;
+    double null_drift_max;       /* largest |g^{mu nu} p_mu p_nu| of the final states         */
+    double null_drift_median;    /* its median over the batch, to 1/16 decade (log10 bins;
+                                  * 0 when the middle ray's drift is below 1e-20)             */
     double audit_ms;             /* HIP-event time of the surrogate audit (FP64 re-traces)   */
     double audit_dir_err_max;    /* largest escape-direction error of audited answers (rad) */
     uint64_t audit_class_mismatch; /* audited answers whose class the integrator disputes    */

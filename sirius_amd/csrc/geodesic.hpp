@@ -69,6 +69,9 @@ struct TraceParams {
     const uint32_t* first_count;
 };
 
+// Null-drift histogram of the statistics pass: log10 bins of 1/16 decade over [1e-20, 1).
+constexpr int SIRIUS_DRIFT_BINS = 320;
+
 // Finite test usable on host and device: x - x is NaN exactly when x is NaN or infinite.
 template <class T>
 SIRIUS_HD __forceinline__ bool sfinite(T x) {

@@ -36,7 +36,8 @@ hipError_t launch_trace_persistent(bool fp64, int family, int emission, bool mul
 hipError_t launch_symplectic(bool fp64, int family, const CamConst* cams, const TraceParams& tp, const DevOut& o,
                              int block, hipStream_t st);
 int persistent_blocks_per_cu(bool fp64, int family, int emission, bool multi, int block);
-hipError_t launch_reduce_stats(const DevOut& o, int64_t n, unsigned long long* acc, hipStream_t st);
+hipError_t launch_reduce_stats(const DevOut& o, int64_t n, unsigned long long* acc, uint32_t* hist,
+                               hipStream_t st);
 hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float* tabs, int64_t n, float* out,
                                     hipStream_t st);
 hipError_t launch_gate(const CamConst* cams, int n_cam, const float* tabs, int64_t n, double band, double crit_band,
@@ -655,10 +656,16 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
     if (stats) {
         unsigned long long* acc = (unsigned long long*)H.scratch("stat_acc", 8 * 8, &rc);
         if (rc) return rc;
+        uint32_t* hist = (uint32_t*)H.scratch("stat_hist", 4 * SIRIUS_DRIFT_BINS, &rc);
+        if (rc) return rc;
         if ((e = launch_fill((uint32_t*)acc, 16, 0u, H.stream)) != hipSuccess) return H.hip_fail(e, "fill");
-        if ((e = launch_reduce_stats(dout, B.n, acc, H.stream)) != hipSuccess) return H.hip_fail(e, "reduce");
+        if ((e = launch_fill(hist, SIRIUS_DRIFT_BINS, 0u, H.stream)) != hipSuccess) return H.hip_fail(e, "fill");
+        if ((e = launch_reduce_stats(dout, B.n, acc, hist, H.stream)) != hipSuccess) return H.hip_fail(e, "reduce");
         unsigned long long h[8] = {0};
         if ((e = hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, H.stream)) != hipSuccess)
+            return H.hip_fail(e, "stats copy");
+        uint32_t hh[SIRIUS_DRIFT_BINS];
+        if ((e = hipMemcpyAsync(hh, hist, sizeof(hh), hipMemcpyDeviceToHost, H.stream)) != hipSuccess)
             return H.hip_fail(e, "stats copy");
         uint32_t wk[32] = {0};
         if (persistent) {
@@ -674,6 +681,17 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
         double dm;
         std::memcpy(&dm, &h[3], sizeof(dm));
         stats->null_drift_max = std::max(stats->null_drift_max, dm);
+        {
+            // median from the log10 histogram: the centre of the bin holding the middle ray
+            uint64_t total = 0, run = 0;
+            for (int b = 0; b < SIRIUS_DRIFT_BINS; ++b) total += hh[b];
+            int mb = 0;
+            for (; mb < SIRIUS_DRIFT_BINS; ++mb) {
+                run += hh[mb];
+                if (2 * run >= total) break;
+            }
+            stats->null_drift_median = (total == 0 || mb == 0) ? 0.0 : std::pow(10.0, (mb + 0.5) / 16.0 - 20.0);
+        }
         stats->rays_escalated += h[4];
         stats->rays_audited += h[6];
         unsigned long long bulk = h[0];

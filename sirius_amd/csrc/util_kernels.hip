@@ -12,7 +12,14 @@ namespace sirius {
 // acc[4] escalated rays, acc[5] surrogate rays, acc[6] audited rays, acc[7] attempts of the
 // escalated rays. Grid-stride loop, wave then block reduction in LDS, one atomic per block
 // and counter.
-__global__ __launch_bounds__(256) void k_reduce_stats(DevOut o, int64_t n, unsigned long long* acc) {
+// hist (SIRIUS_DRIFT_BINS counters): the null drift of every ray binned by log10, 1/16 decade per
+// bin from 1e-20 (bin 0 also holds smaller values and zero) to 1 (the last bin holds larger
+// values), for its median; per-block counts in LDS, one atomic per nonzero bin per block.
+__global__ __launch_bounds__(256) void k_reduce_stats(DevOut o, int64_t n, unsigned long long* acc,
+                                                      uint32_t* hist) {
+    __shared__ uint32_t h[SIRIUS_DRIFT_BINS];
+    for (int b = threadIdx.x; b < SIRIUS_DRIFT_BINS; b += blockDim.x) h[b] = 0u;
+    __syncthreads();
     unsigned long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double dmax = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -27,6 +34,9 @@ __global__ __launch_bounds__(256) void k_reduce_stats(DevOut o, int64_t n, unsig
         v[6] += (f & SIRIUS_FLAG_AUDITED) ? 1 : 0;
         double d = (double)o.null_drift[i];
         dmax = d > dmax ? d : dmax;
+        float lb = d > 0.0 ? (log10f((float)d) + 20.0f) * 16.0f : 0.0f;
+        int b = lb > 0.0f ? (int)lb : 0;
+        atomicAdd(&h[b < SIRIUS_DRIFT_BINS ? b : SIRIUS_DRIFT_BINS - 1], 1u);
     }
     for (int off = 32; off > 0; off >>= 1) {
         for (int k = 0; k < 8; ++k) v[k] += __shfl_xor(v[k], off);
@@ -50,13 +60,16 @@ __global__ __launch_bounds__(256) void k_reduce_stats(DevOut o, int64_t n, unsig
         for (int j = 0; j < (int)(blockDim.x >> 6); ++j) t = pmax[j] > t ? pmax[j] : t;
         atomicMax(&acc[3], (unsigned long long)__double_as_longlong(t));
     }
+    for (int b = threadIdx.x; b < SIRIUS_DRIFT_BINS; b += blockDim.x)
+        if (h[b]) atomicAdd(&hist[b], h[b]);
 }
 
-hipError_t launch_reduce_stats(const DevOut& o, int64_t n, unsigned long long* acc, hipStream_t st) {
+hipError_t launch_reduce_stats(const DevOut& o, int64_t n, unsigned long long* acc, uint32_t* hist,
+                               hipStream_t st) {
     int64_t blocks = (n + 255) / 256;
     if (blocks > 512) blocks = 512;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_reduce_stats, dim3((unsigned)blocks), dim3(256), 0, st, o, n, acc);
+    hipLaunchKernelGGL(k_reduce_stats, dim3((unsigned)blocks), dim3(256), 0, st, o, n, acc, hist);
     return hipGetLastError();
 }
 

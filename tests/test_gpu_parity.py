@@ -72,6 +72,11 @@ def test_fp64_full_frame_matches_oracle(acc, golden, fixture):
     assert np.all(out.cls <= abi.RAY_DISK)
     assert st["steps_bulk"] == out.steps.astype(np.int64).sum()
     assert out.null_drift.max() < 2e-6
+    # the statistics pass's drift maximum and median (1/16-decade histogram) against the outputs
+    assert st["null_drift_max"] == float(out.null_drift.max())
+    med = float(np.median(out.null_drift.astype(np.float64)))
+    if med > 1e-19:
+        assert abs(np.log10(st["null_drift_median"]) - np.log10(med)) <= 1.0 / 16 + 1e-3, (st["null_drift_median"], med)
 
 
 @pytest.mark.parametrize("fixture", ["ref_trace_c1.npz", "ref_trace_c2.npz"])
