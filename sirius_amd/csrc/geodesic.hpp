@@ -668,20 +668,22 @@ SIRIUS_HD __forceinline__ double page_thorne_f(const CamConst& c, double r) {
 }
 
 // Kepler angular velocity of a prograde circular Kerr-Newman orbit at BL radius r.
-SIRIUS_HD __forceinline__ double kepler_omega(const CamConst& c, double r) {
-    double sq = sqrt(fmax(c.M * r - c.Q2, 0.0));
-    return sq / (r * r + c.a * sq);
+template <class T = double>
+SIRIUS_HD __forceinline__ T kepler_omega(const CamConst& c, T r) {
+    T sq = sqrt(fmax(T(c.M) * r - T(c.Q2), T(0)));
+    return sq / (r * r + T(c.a) * sq);
 }
 
 // u^t of the circular emitter at x with angular velocity Om (u = u^t (d_t + Om d_phi)).
-SIRIUS_HD __forceinline__ double emitter_ut(const CamConst& c, const Spacetime<double>& s, const double x[4], double Om) {
-    double g[4][4];
+template <class T = double>
+SIRIUS_HD __forceinline__ T emitter_ut(const CamConst& c, const Spacetime<T>& s, const T x[4], T Om) {
+    T g[4][4];
     metric_cov(s, x, g);
-    double v[4] = {1.0, -Om * x[2], Om * x[1], 0.0};
-    double vv = 0;
+    T v[4] = {T(1), -Om * x[2], Om * x[1], T(0)};
+    T vv = T(0);
     for (int i = 0; i < 4; ++i)
         for (int j = 0; j < 4; ++j) vv += g[i][j] * v[i] * v[j];
-    return vv < 0 ? 1.0 / sqrt(-vv) : -1.0;
+    return vv < T(0) ? T(1) / sqrt(-vv) : T(-1);
 }
 
 // Spectral bin centres x_k = x_min (x_max / x_min)^((k + 1/2) / n_bins): the first centre and the

@@ -229,7 +229,8 @@ SIRIUS_HD __forceinline__ void vol_accumulate(Ray<T, FAM>& R, const CamConst& c,
                                               const Der<T>& d1) {
     const T gn[3] = {T(0.1127016653792583), T(0.5), T(0.8872983346207417)};
     const T gw[3] = {T(5.0 / 18.0), T(8.0 / 18.0), T(5.0 / 18.0)};
-    Spacetime<double> sd = make_spacetime<double>(c);
+    double xk0, qk0;
+    bin_geometry(tp, xk0, qk0);
     for (int q = 0; q < 3; ++q) {
         T xq[4];
         hermite(R.x, R.d.k, x1, d1.k, R.h, gn[q], xq);
@@ -239,34 +240,32 @@ SIRIUS_HD __forceinline__ void vol_accumulate(Ray<T, FAM>& R, const CamConst& c,
         T sig = T(tp.vol_h_over_r) * r;
         T rho = T(tp.vol_j0) / (r * r) * exp(-xq[3] * xq[3] / (T(2) * sig * sig)) *
                 vol_window(r, r_lo, T(tp.vol_r_max));
-        // emitter velocity and frequencies in FP64; p by cubic Hermite with dp/dlam
-        double th = (double)gn[q], t2 = th * th, t3 = t2 * th, hh = (double)R.h;
-        double h00 = 2 * t3 - 3 * t2 + 1, h10 = t3 - 2 * t2 + th, h01 = 3 * t2 - 2 * t3, h11 = t3 - t2;
-        double xd[4] = {(double)xq[0], (double)xq[1], (double)xq[2], (double)xq[3]};
-        double pd[4];
-        pd[0] = (double)R.p[0];
+        // emitter velocity, frequencies and the spectrum in the path's precision (each node's
+        // contribution then joins an FP64 accumulator); p by cubic Hermite with dp/dlam
+        T th = gn[q], t2 = th * th, t3 = t2 * th, hh = R.h;
+        T h00 = T(2) * t3 - T(3) * t2 + T(1), h10 = t3 - T(2) * t2 + th, h01 = T(3) * t2 - T(2) * t3, h11 = t3 - t2;
+        T pq[4];
+        pq[0] = R.p[0];
         for (int i = 1; i < 4; ++i)
-            pd[i] = h00 * (double)R.p[i] + h10 * hh * (double)R.d.f[i - 1] + h01 * (double)p1[i] +
-                    h11 * hh * (double)d1.f[i - 1];
-        double rc = sqrt(fmax(xd[1] * xd[1] + xd[2] * xd[2] - c.a * c.a, 1e-12));
-        double Om = kepler_omega(c, rc);
-        double ut = emitter_ut(c, sd, xd, Om);
-        if (ut < 0) continue;
-        double nue = -ut * (pd[0] + Om * (xd[1] * pd[2] - xd[2] * pd[1]));
-        double nuo = -pd[0] * c.obs_norm;
-        if (nue <= 0 || nuo <= 0) continue;
-        double gf = nuo / nue;
-        double dl = (double)gw[q] * hh;
-        double dtau = tp.vol_kappa * (double)rho * nue * dl;
-        double w = (double)rho * (nuo * nuo * nuo / (nue * nue)) * dl * exp(-((double)R.tau + 0.5 * dtau));
-        double tv = pow(6.0 / (double)r, 0.75);
-        double xk, qk;
-        bin_geometry(tp, xk, qk);
+            pq[i] = h00 * R.p[i] + h10 * hh * R.d.f[i - 1] + h01 * p1[i] + h11 * hh * d1.f[i - 1];
+        T rc = sqrt(fmax(xq[1] * xq[1] + xq[2] * xq[2] - T(c.a * c.a), T(1e-12)));
+        T Om = kepler_omega<T>(c, rc);
+        T ut = emitter_ut<T>(c, s, xq, Om);
+        if (ut < T(0)) continue;
+        T nue = -ut * (pq[0] + Om * (xq[1] * pq[2] - xq[2] * pq[1]));
+        T nuo = -pq[0] * T(c.obs_norm);
+        if (nue <= T(0) || nuo <= T(0)) continue;
+        T gf = nuo / nue;
+        T dl = gw[q] * hh;
+        T dtau = T(tp.vol_kappa) * rho * nue * dl;
+        T w = rho * (nuo * nuo * nuo / (nue * nue)) * dl * exp(-(R.tau + T(0.5) * dtau));
+        T tv = pow_pos(T(6) / r, T(0.75));
+        T xk = T(xk0), qk = T(qk0);
         for (int k = 0; k < tp.n_bins; ++k, xk *= qk) {
-            double xe = xk / gf;
-            o.vol_acc[(int64_t)k * o.n_rays + R.id] += w * xe * xe * xe / expm1(xe / tv);
+            T xe = xk / gf;
+            o.vol_acc[(int64_t)k * o.n_rays + R.id] += (double)(w * xe * xe * xe / expm1(xe / tv));
         }
-        R.tau += T(dtau);
+        R.tau += dtau;
     }
 }
 
