@@ -300,6 +300,50 @@ int sirius_hip_surrogate_train(SiriusHipHandle h, const SiriusHipCamera* cams, i
                                const SiriusHipTraceOpts* label_opts, int epochs, int batch,
                                float lr, uint64_t seed, float* loss);
 
+/* ---- procedural star field (celestial spheres) ---------------------------------------- */
+/* One catalogue star, field for field the reference's StarEntry
+ * (src/Sirius.Core/Environment/PHSF001A.h:38-46): unit direction, distance (pc), apparent
+ * magnitude, B-V colour index, effective temperature (K). */
+typedef struct SiriusHipStar {
+    float dir[3];
+    float distance_pc;
+    float magnitude;
+    float color_bv;
+    float temperature_K;
+    float reserved;
+} SiriusHipStar;
+
+/* The generator and sampler parameters of the reference's StarfieldConfig (PHSF001A.h:85-121);
+ * its galactic-model fields, which its generator does not read, are left out. */
+typedef struct SiriusHipStarfield {
+    uint32_t star_count;      /* candidates drawn; those brighter than magnitude_limit are kept */
+    uint32_t seed;            /* std::mt19937 seed                                              */
+    float min_distance_pc, max_distance_pc;
+    float magnitude_limit;
+    float aperture_mm;        /* depth-of-field blur (0: none)                                  */
+    float focus_distance_pc;
+    float brightness_scale;
+    int32_t enable_dof;
+    int32_t reserved;
+} SiriusHipStarfield;
+
+/* The reference's defaults (StarfieldConfig, PHSF001A.h:85-98). */
+void sirius_hip_starfield_default(SiriusHipStarfield* cfg);
+/* StarfieldGenerator::generate (PHSF001A.h:133-151), on the host: the catalogue for cfg (after
+ * the reference's validate() clamps); writes at most `capacity` stars, *count receives the
+ * catalogue's size. No device needed. */
+int sirius_hip_starfield_generate(const SiriusHipStarfield* cfg, SiriusHipStar* stars, int capacity, int* count);
+/* Installs a catalogue as celestial sphere `side` (0: the sky, or the near side of a wormhole;
+ * 1: a wormhole's far side). sirius_hip_launch then adds each escaped ray's star light
+ * (StarfieldGenerator::sampleStarfield, PHSF001A.h:153-201, observer at the origin) to the
+ * framebuffer. n = 0 removes the sphere. Has no counterpart in the reference's C API, whose
+ * star field lives in the CPU session. */
+int sirius_hip_set_starfield(SiriusHipHandle h, int side, const SiriusHipStarfield* cfg, const SiriusHipStar* stars,
+                             int n);
+/* Star light of sphere `side` at n unit directions dirs[3n] (host arrays), rgb[3n] out, computed
+ * on the device by the kernel the shading uses. */
+int sirius_hip_sample_starfield(SiriusHipHandle h, int side, const float* dirs, int n, float* rgb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -163,3 +163,133 @@ def mlp_loss_grad(params, x, cls, dirs, round_bf16=False):
                                   cls.ctypes.data_as(C.POINTER(C.c_int32)), dirs.ctypes.data_as(C.POINTER(C.c_float)),
                                   len(x), _d(g), int(round_bf16))
     return loss, g
+
+
+# ------------------------------------------------------------------------------------------
+# Star field (celestial spheres): restatement of StarfieldGenerator::generate / generateStar
+# (src/Sirius.Core/Environment/PHSF001A.h:133-151, 206-249) and ::sampleStarfield
+# (PHSF001A.h:153-201). The generator draws from a 32-bit Mersenne twister (std::mt19937, from
+# its published definition) through libstdc++'s float canonical draw (one 32-bit word, rounded
+# to float, divided by 2^32, kept below 1), and evaluates its formulas in float with the C
+# library's float functions; the sampler evaluates the point-spread sum in float32 (numpy),
+# products and sums unfused, in the reference's order.
+# ------------------------------------------------------------------------------------------
+_f32 = np.float32
+_libm = None
+
+
+def _m():
+    global _libm
+    if _libm is None:
+        _libm = C.CDLL("libm.so.6")
+        for name in ("acosf", "sinf", "cosf", "cbrtf", "log10f", "sqrtf"):
+            getattr(_libm, name).restype = C.c_float
+            getattr(_libm, name).argtypes = [C.c_float]
+        _libm.powf.restype = C.c_float
+        _libm.powf.argtypes = [C.c_float, C.c_float]
+    return _libm
+
+
+class MT19937:
+    """32-bit Mersenne twister (Matsumoto & Nishimura 1998), the std::mt19937 engine."""
+
+    def __init__(self, seed):
+        self.mt = [0] * 624
+        self.mt[0] = seed & 0xFFFFFFFF
+        for i in range(1, 624):
+            self.mt[i] = (1812433253 * (self.mt[i - 1] ^ (self.mt[i - 1] >> 30)) + i) & 0xFFFFFFFF
+        self.i = 624
+
+    def next(self):
+        if self.i >= 624:
+            for k in range(624):
+                y = (self.mt[k] & 0x80000000) | (self.mt[(k + 1) % 624] & 0x7FFFFFFF)
+                self.mt[k] = self.mt[(k + 397) % 624] ^ (y >> 1) ^ (0x9908B0DF if y & 1 else 0)
+            self.i = 0
+        y = self.mt[self.i]
+        self.i += 1
+        y ^= y >> 11
+        y ^= (y << 7) & 0x9D2C5680
+        y ^= (y << 15) & 0xEFC60000
+        y ^= y >> 18
+        return y & 0xFFFFFFFF
+
+    def uniform_float(self):
+        """std::uniform_real_distribution<float>(0, 1) over this engine (libstdc++)."""
+        r = _f32(self.next()) / _f32(4294967296.0)
+        return r if r < _f32(1.0) else np.nextafter(_f32(1.0), _f32(0.0))
+
+
+def _validate_starfield(count, dmin, dmax, maglim):
+    dmin = max(_f32(dmin), _f32(0.1))
+    dmax = max(_f32(dmax), _f32(dmin + _f32(1.0)))
+    return min(int(count), 10000000), dmin, dmax, min(max(_f32(maglim), _f32(0.0)), _f32(20.0))
+
+
+def starfield_generate(count, dmin, dmax, maglim, seed):
+    """The catalogue: (n, 8) float32 rows (direction, distance, magnitude, B-V, temperature, 0)."""
+    m = _m()
+    count, dmin, dmax, maglim = _validate_starfield(count, dmin, dmax, maglim)
+    g = MT19937(seed)
+    U = g.uniform_float
+    pi = _f32(np.pi)
+    rows = []
+    for _ in range(count):
+        th = _f32(m.acosf(_f32(2.0) * U() - _f32(1.0)))
+        ph = _f32(2.0) * pi * U()
+        st, ct = _f32(m.sinf(th)), _f32(m.cosf(th))
+        d = (st * _f32(m.cosf(ph)), st * _f32(m.sinf(ph)), ct)
+        u = U()
+        dist = _f32(m.cbrtf(u * (dmax * dmax * dmax) + (_f32(1.0) - u) * (dmin * dmin * dmin)))
+        absm = _f32(-2.0) + _f32(12.0) * _f32(m.powf(U(), _f32(0.3)))
+        mag = absm + _f32(5.0) * _f32(m.log10f(dist / _f32(10.0)))
+        t = (absm + _f32(2.0)) / _f32(12.0)
+        bv = min(max(_f32(-0.3) + _f32(2.0) * t + _f32(0.3) * (U() - _f32(0.5)), _f32(-0.4)), _f32(2.0))
+        temp = _f32(4600.0) / (_f32(0.92) + bv) if bv > _f32(-0.1) else _f32(30000.0) + _f32(10000.0) * (-bv - _f32(0.1))
+        temp = min(max(temp, _f32(2500.0)), _f32(50000.0))
+        if mag <= maglim:
+            rows.append((d[0], d[1], d[2], dist, mag, bv, temp, _f32(0.0)))
+    return np.array(rows, np.float32).reshape(-1, 8)
+
+
+def _star_colour(T):
+    m = _m()
+    T = _f32(T) if T > 0 else _f32(5778.0)
+    tn = T / _f32(6500.0)
+    if T <= _f32(6500.0):
+        c = [_f32(1.0), _f32(m.powf(tn, _f32(0.4))), _f32(m.powf(tn, _f32(0.8)))]
+    else:
+        c = [_f32(m.powf(_f32(1.0) / tn, _f32(0.3))), _f32(m.powf(_f32(1.0) / tn, _f32(0.15))), _f32(1.0)]
+    mx = max(c)
+    return [x / mx for x in c]
+
+
+def starfield_sample(stars, dirs, aperture_mm=50.0, focus_pc=100.0, brightness=1.0, dof=True):
+    """Star light (n, 3) at unit directions dirs (n, 3), observer at the origin."""
+    m = _m()
+    stars = np.asarray(stars, np.float32)
+    dirs = np.asarray(dirs, np.float32)
+    out = np.zeros((len(dirs), 3), np.float32)
+    b = _f32(brightness)
+    with np.errstate(over="ignore", under="ignore", invalid="ignore"):
+        for s in stars:
+            dx, dy, dz = s[0] * s[3] - _f32(0.0), s[1] * s[3] - _f32(0.0), s[2] * s[3] - _f32(0.0)
+            d = _f32(m.sqrtf(dx * dx + dy * dy + dz * dz))
+            if d < _f32(1e-6):
+                continue
+            px, py, pz = dx / d, dy / d, dz / d
+            sig_dof = _f32(0.0)
+            if dof and aperture_mm > 0:
+                sig_dof = _f32(aperture_mm) * _f32(abs(_f32(1.0) / s[3] - _f32(1.0) / _f32(focus_pc))) * _f32(1e-3)
+            sigma = _f32(m.sqrtf(_f32(1e-6) * _f32(1e-6) + sig_dof * sig_dof))
+            c = (dirs[:, 0] * px + dirs[:, 1] * py) + dirs[:, 2] * pz
+            ang = np.arccos(np.clip(c, _f32(-1.0), _f32(1.0))).astype(np.float32)
+            psf = np.exp((_f32(-0.5) * ang * ang) / (sigma * sigma)).astype(np.float32)
+            keep = psf >= _f32(1e-6)
+            if not keep.any():
+                continue
+            inten = (_f32(m.powf(_f32(10.0), _f32(-0.4) * s[4])) * psf) * b
+            col = _star_colour(s[6])
+            for k in range(3):
+                out[keep, k] = out[keep, k] + col[k] * inten[keep]
+    return out

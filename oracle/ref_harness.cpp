@@ -13,6 +13,7 @@
 //   Geodesic::integrateStepRK45       Sirius.Core/Geodesic/PHGD001A.cpp:530-641
 //   AccretionDiskD (ISCO, Page-Thorne quadrature)  Sirius.Core/Disk/PHAD001A.h:88-314
 //   SymplecticIntegratorD             Sirius.Core/Symplectic/PHSI001A.h:136-455
+//   StarfieldGenerator                Sirius.Core/Environment/PHSF001A.h:126-272
 // What the harness supplies itself: GeodesicTracer::initializeLightray is private, so its
 // steps (GTRC001A.cpp:15-134, including the float casts) are repeated here; the step loop
 // retries rejected steps (GeodesicTracer::trace ends a ray on the first rejection,
@@ -23,10 +24,12 @@
 #include "PHMT102A.h"
 #include "PHAD001A.h"
 #include "PHSI001A.h"
+#include "Environment/PHSF001A.h"
 
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <vector>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -277,6 +280,39 @@ void ref_symplectic_run(double a, double r, double th, double pr, double pth, do
     *maxH = st.maxHError;
     *dE = st.finalE - st.initialE;
     *dL = st.finalLz - st.initialLz;
+}
+
+// StarfieldGenerator::generate (PHSF001A.h:133-151): the catalogue, 8 floats per star in
+// StarEntry order; returns the number of stars (at most cap are written).
+int ref_starfield_generate(uint32_t count, float dmin, float dmax, float maglim, uint32_t seed, float* out,
+                           int cap) {
+    StarfieldConfig c;
+    c.star_count = count;
+    c.min_distance_pc = dmin;
+    c.max_distance_pc = dmax;
+    c.magnitude_limit = maglim;
+    c.seed = seed;
+    StarfieldGenerator g(c);
+    std::vector<StarEntry> stars = g.generate();
+    for (size_t i = 0; i < stars.size() && (int)i < cap; ++i) std::memcpy(out + 8 * i, &stars[i], 8 * sizeof(float));
+    return (int)stars.size();
+}
+
+// StarfieldGenerator::sampleStarfield (PHSF001A.h:153-201) at m unit directions (camera at the
+// origin, no parallax offset): rgb, 3 floats per direction.
+void ref_starfield_sample(float aperture_mm, float focus_pc, float brightness, int dof, const float* stars, int n,
+                          const float* dirs, int m, float* rgb) {
+    StarfieldConfig c;
+    c.aperture_mm = aperture_mm;
+    c.focus_distance_pc = focus_pc;
+    c.brightness_scale = brightness;
+    c.enable_dof = dof != 0;
+    StarfieldGenerator g(c);
+    std::vector<StarEntry> cat(n);
+    for (int i = 0; i < n; ++i) std::memcpy(&cat[i], stars + 8 * i, 8 * sizeof(float));
+    for (int j = 0; j < m; ++j)
+        g.sampleStarfield(dirs[3 * j], dirs[3 * j + 1], dirs[3 * j + 2], 0.f, 0.f, 0.f, cat, rgb[3 * j], rgb[3 * j + 1],
+                          rgb[3 * j + 2]);
 }
 
 }  // extern "C"

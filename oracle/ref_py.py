@@ -40,6 +40,10 @@ def lib():
         L.ref_pt_q.argtypes = [d, d]
         L.ref_circular.argtypes = [d, d, dp, dp, dp]
         L.ref_symplectic_run.argtypes = [d, d, d, d, d, d, d, i, d, dp, dp, dp]
+        f, u32 = C.c_float, C.c_uint32
+        L.ref_starfield_generate.restype = i
+        L.ref_starfield_generate.argtypes = [u32, f, f, f, u32, ip, i]
+        L.ref_starfield_sample.argtypes = [f, f, f, i, ip, i, ip, i, ip]
         _lib = L
     return _lib
 
@@ -115,3 +119,21 @@ def symplectic_run(a, r, th, pr, pth, E, Lz, n, h):
     mh, de, dl = C.c_double(), C.c_double(), C.c_double()
     lib().ref_symplectic_run(a, r, th, pr, pth, E, Lz, n, h, C.byref(mh), C.byref(de), C.byref(dl))
     return mh.value, de.value, dl.value
+
+
+def starfield_generate(count, dmin, dmax, maglim, seed):
+    """StarfieldGenerator::generate: (n, 8) float32 rows in StarEntry order."""
+    cap = int(count)
+    out = np.zeros((max(cap, 1), 8), np.float32)
+    n = lib().ref_starfield_generate(count, dmin, dmax, maglim, seed, out.ctypes.data, cap)
+    return out[:n].copy()
+
+
+def starfield_sample(aperture_mm, focus_pc, brightness, dof, stars, dirs):
+    """StarfieldGenerator::sampleStarfield at each unit direction (observer at the origin)."""
+    stars = np.ascontiguousarray(stars, np.float32)
+    dirs = np.ascontiguousarray(dirs, np.float32)
+    rgb = np.zeros((len(dirs), 3), np.float32)
+    lib().ref_starfield_sample(aperture_mm, focus_pc, brightness, int(dof), stars.ctypes.data, len(stars),
+                               dirs.ctypes.data, len(dirs), rgb.ctypes.data)
+    return rgb

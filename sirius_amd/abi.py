@@ -92,6 +92,21 @@ class Stats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Star(C.Structure):
+    """SiriusHipStar (the reference's StarEntry field order)."""
+    _fields_ = [("dir", C.c_float * 3), ("distance_pc", C.c_float), ("magnitude", C.c_float),
+                ("color_bv", C.c_float), ("temperature_K", C.c_float), ("reserved", C.c_float)]
+
+
+class Starfield(C.Structure):
+    """SiriusHipStarfield (generator and sampler parameters)."""
+    _fields_ = [("star_count", C.c_uint32), ("seed", C.c_uint32),
+                ("min_distance_pc", C.c_float), ("max_distance_pc", C.c_float),
+                ("magnitude_limit", C.c_float), ("aperture_mm", C.c_float),
+                ("focus_distance_pc", C.c_float), ("brightness_scale", C.c_float),
+                ("enable_dof", C.c_int32), ("reserved", C.c_int32)]
+
+
 def metric_ks(M=1.0, a=0.0, Q=0.0, Lambda=0.0):
     return Metric(family=METRIC_KERR_SCHILD, M=M, a=a, Q=Q, Lambda=Lambda)
 

@@ -73,8 +73,34 @@ def lib():
     L.sirius_hip_surrogate_train.argtypes = [H, C.POINTER(abi.Camera), i, C.POINTER(abi.Metric), i,
                                              C.POINTER(abi.TraceOpts), i, i, C.c_float, C.c_uint64,
                                              C.POINTER(C.c_float)]
+    L.sirius_hip_starfield_default.argtypes = [C.POINTER(abi.Starfield)]
+    L.sirius_hip_starfield_default.restype = None
+    L.sirius_hip_starfield_generate.argtypes = [C.POINTER(abi.Starfield), C.c_void_p, i, C.POINTER(C.c_int)]
+    L.sirius_hip_set_starfield.argtypes = [H, i, C.POINTER(abi.Starfield), C.c_void_p, i]
+    L.sirius_hip_sample_starfield.argtypes = [H, i, C.c_void_p, i, C.c_void_p]
     _lib = L
     return L
+
+
+def starfield_default():
+    """The reference's StarfieldConfig defaults (host only)."""
+    cfg = abi.Starfield()
+    lib().sirius_hip_starfield_default(C.byref(cfg))
+    return cfg
+
+
+def starfield_generate(cfg):
+    """The catalogue of cfg as an array of abi.Star (host only: no device needed)."""
+    L = lib()
+    n = C.c_int(0)
+    rc = L.sirius_hip_starfield_generate(C.byref(cfg), None, 0, C.byref(n))
+    if rc:
+        raise SiriusHipError(rc, "starfield_generate")
+    stars = (abi.Star * max(n.value, 1))()
+    rc = L.sirius_hip_starfield_generate(C.byref(cfg), stars, n.value, C.byref(n))
+    if rc:
+        raise SiriusHipError(rc, "starfield_generate")
+    return stars, n.value
 
 
 def default_opts(precision=abi.PRECISION_FP64):
@@ -147,6 +173,20 @@ class HipAccelerator:
         h, w = rgba.shape[:2]
         self._check(self._L.sirius_hip_upload_background(self._h, rgba.ctypes.data, w, h))
         return True
+
+    def set_starfield(self, side, cfg, stars, n=None):
+        """Installs a star catalogue (abi.Star array) as celestial sphere `side` (0 sky / near
+        side, 1 a wormhole's far side); n = 0 removes it."""
+        n = len(stars) if n is None else n
+        self._check(self._L.sirius_hip_set_starfield(self._h, side, C.byref(cfg) if cfg is not None else None,
+                                                     stars if n else None, n))
+
+    def sample_starfield(self, side, dirs):
+        """Star light (n, 3) of sphere `side` at unit directions dirs (n, 3), on the device."""
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        rgb = np.zeros((len(dirs), 3), np.float32)
+        self._check(self._L.sirius_hip_sample_starfield(self._h, side, dirs.ctypes.data, len(dirs), rgb.ctypes.data))
+        return rgb
 
     def synchronise(self):
         self._check(self._L.sirius_hip_synchronize(self._h))

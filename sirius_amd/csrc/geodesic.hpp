@@ -72,6 +72,10 @@ struct TraceParams {
 // Null-drift histogram of the statistics pass: log10 bins of 1/16 decade over [1e-20, 1).
 constexpr int SIRIUS_DRIFT_BINS = 320;
 
+// Floats per device star record of a celestial sphere: unit direction (3), cosine cut,
+// sigma^2 of the point-spread function, 10^(-0.4 m), colour (3), padding (3).
+constexpr int SIRIUS_STAR_REC = 12;
+
 // Finite test usable on host and device: x - x is NaN exactly when x is NaN or infinite.
 template <class T>
 SIRIUS_HD __forceinline__ bool sfinite(T x) {

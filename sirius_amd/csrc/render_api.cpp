@@ -13,6 +13,8 @@
 #include "sirius_internal.hpp"
 
 namespace sirius {
+hipError_t launch_star_shade(const DevOut& o, int64_t n, const float* rec0, int n0, float b0, const float* rec1, int n1,
+                             float b1, float* fb, hipStream_t st);
 hipError_t launch_shade(const DevOut& o, int64_t n, int width, int height, int n_bins, const uint8_t* bg, int bgw,
                         int bgh, float* fb, hipStream_t st);
 hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float* tabs, int64_t n, float* out,
@@ -151,6 +153,9 @@ int sirius_hip_launch(SiriusHipHandle hh, const SiriusHipCamera* cam, const Siri
     hipError_t e = launch_shade(dout, B.n, c.width, c.height, opts->n_bins, H->bg_dev, H->bg_w, H->bg_h, H->fb_dev,
                                 H->stream);
     if (e != hipSuccess) return H->hip_fail(e, "shade");
+    e = launch_star_shade(dout, B.n, H->star_dev[0], H->star_n[0], H->star_brightness[0], H->star_dev[1],
+                          H->star_n[1], H->star_brightness[1], H->fb_dev, H->stream);
+    if (e != hipSuccess) return H->hip_fail(e, "star shade");
     e = hipMemcpyAsync(H->fb_host.data(), H->fb_dev, H->fb_host.size() * sizeof(float), hipMemcpyDeviceToHost,
                        H->stream);
     if (e != hipSuccess) return H->hip_fail(e, "framebuffer copy");

@@ -210,6 +210,11 @@ void Handle::release() {
     for (auto& kv : bufs) if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     bufs.clear();
     cam_dev_held = crit_dev_held = nullptr;
+    for (int i = 0; i < 2; ++i) {
+        if (star_dev[i]) (void)hipFree(star_dev[i]);
+        star_dev[i] = nullptr;
+        star_n[i] = 0;
+    }
     if (fb_dev) (void)hipFree(fb_dev);
     fb_dev = nullptr;
     if (bg_dev) (void)hipFree(bg_dev);

@@ -83,6 +83,10 @@ struct Handle {
     int last_bins = 0;
     bool last_band = false;  // the last FP32 trace ran the FP64 band on the second stream
     bool last_audit = false; // the last FP32 trace audited surrogate answers
+    // celestial spheres (sirius_hip_set_starfield): device star records, count, brightness
+    float* star_dev[2] = {nullptr, nullptr};
+    int star_n[2] = {0, 0};
+    float star_brightness[2] = {1.f, 1.f};
     // surrogate network
     MlpState* mlp = nullptr;
 

@@ -225,4 +225,116 @@ hipError_t launch_shade(const DevOut& o, int64_t n, int width, int height, int n
     return hipGetLastError();
 }
 
+
+// ------------------------------------------------------------------------------------------
+// Celestial spheres: StarfieldGenerator::sampleStarfield (PHSF001A.h:153-201) for an observer at
+// the origin. Star records (SIRIUS_STAR_REC floats, see starfield.cpp) are staged through LDS in
+// tiles of 256; each lane sums the Gaussian point-spread of every star of the tile whose
+// direction lies within its cosine cut. The view-dependent float steps are the reference's, in
+// its order and unfused, so the cosine a pixel sees - which decides whether a point-like star
+// lights it - matches the reference's bit for bit.
+// ------------------------------------------------------------------------------------------
+namespace {
+constexpr int STAR_TILE = 256;
+
+__device__ __forceinline__ void star_light(const float* __restrict__ tile, int cnt, float brightness, float vx,
+                                           float vy, float vz, float& r, float& g, float& b) {
+    // Every product and sum rounds on its own: contraction into fused multiply-adds is off in
+    // this scope (HIP's __fmul_rn / __fadd_rn would not prevent it: their bodies are plain
+    // operators that still fuse once inlined), and the division is the IEEE one.
+#pragma clang fp contract(off)
+    for (int j = 0; j < cnt; ++j) {
+        const float* s = tile + j * SIRIUS_STAR_REC;
+        float c = (vx * s[0] + vy * s[1]) + vz * s[2];
+        if (c < s[3]) continue;
+        // acos and exp correctly rounded to float (FP64, then rounded): near a cosine of 1 the
+        // angle jumps by ~3.5e-4 rad per float step, so the spread of a point-like star depends
+        // on every bit of it; the C library's float acos is correctly rounded there too
+        float ang = (float)acos((double)fminf(fmaxf(c, -1.0f), 1.0f));
+        float psf = (float)exp((double)__fdiv_rn((-0.5f * ang) * ang, s[4]));
+        if (psf < 1e-6f) continue;
+        float inten = (s[5] * psf) * brightness;
+        r = r + s[6] * inten;
+        g = g + s[7] * inten;
+        b = b + s[8] * inten;
+    }
+}
+
+// Block-wide pass over one catalogue; every thread of the block takes part in the staging,
+// lanes with want == false skip the sums.
+__device__ __forceinline__ void star_pass(float* tile, const float* __restrict__ rec, int n_star, float brightness,
+                                          bool want, float vx, float vy, float vz, float& r, float& g, float& b) {
+    for (int base = 0; base < n_star; base += STAR_TILE) {
+        int cnt = min(STAR_TILE, n_star - base);
+        __syncthreads();
+        for (int k = threadIdx.x; k < cnt * SIRIUS_STAR_REC; k += blockDim.x)
+            tile[k] = rec[(int64_t)base * SIRIUS_STAR_REC + k];
+        __syncthreads();
+        if (want) star_light(tile, cnt, brightness, vx, vy, vz, r, g, b);
+    }
+}
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_star_dirs(const float* __restrict__ rec, int n_star, float brightness,
+                                                   const float* __restrict__ dirs, int n, float* __restrict__ rgb) {
+    __shared__ float tile[STAR_TILE * SIRIUS_STAR_REC];
+    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    bool act = id < n;
+    float vx = act ? dirs[3 * id] : 0.f, vy = act ? dirs[3 * id + 1] : 0.f, vz = act ? dirs[3 * id + 2] : 0.f;
+    float r = 0.f, g = 0.f, b = 0.f;
+    star_pass(tile, rec, n_star, brightness, act, vx, vy, vz, r, g, b);
+    if (act) {
+        rgb[3 * id] = r;
+        rgb[3 * id + 1] = g;
+        rgb[3 * id + 2] = b;
+    }
+}
+
+// Star light added to the framebuffer of escaped rays: the sky (or a wormhole's near side) from
+// sphere 0, a wormhole's far side (SIRIUS_FLAG_FAR_SIDE) from sphere 1; the direction is the
+// ray's escape direction (theta, phi).
+__global__ __launch_bounds__(256) void k_star_shade(DevOut o, int64_t n, const float* __restrict__ rec0, int n0,
+                                                    float b0, const float* __restrict__ rec1, int n1, float b1,
+                                                    float* __restrict__ fb) {
+    __shared__ float tile[STAR_TILE * SIRIUS_STAR_REC];
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool esc = id < n && o.cls[id] == SIRIUS_RAY_ESCAPED;
+    bool far = esc && (o.flags[id] & SIRIUS_FLAG_FAR_SIDE);
+    float vx = 0.f, vy = 0.f, vz = 0.f;
+    if (esc) {
+        // the unit escape direction in FP64, each component rounded once to float: a host that
+        // forms it the same way gets the same bits (to a double rounding near a float midpoint),
+        // and next to a point-like star one ulp of the direction decides whether it lights
+        const double th = o.theta[id], ph = o.phi[id];
+        const double st = sin(th);
+        vx = (float)(st * cos(ph));
+        vy = (float)(st * sin(ph));
+        vz = (float)cos(th);
+    }
+    float r = 0.f, g = 0.f, b = 0.f;
+    star_pass(tile, rec0, n0, b0, esc && !far, vx, vy, vz, r, g, b);
+    star_pass(tile, rec1, n1, b1, far, vx, vy, vz, r, g, b);
+    if (esc) {
+        fb[4 * id + 0] += r;
+        fb[4 * id + 1] += g;
+        fb[4 * id + 2] += b;
+    }
+}
+
+hipError_t launch_star_dirs(const float* rec, int n_star, float brightness, const float* dirs, int n, float* rgb,
+                            hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_star_dirs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rec, n_star, brightness, dirs,
+                       n, rgb);
+    return hipGetLastError();
+}
+
+hipError_t launch_star_shade(const DevOut& o, int64_t n, const float* rec0, int n0, float b0, const float* rec1, int n1,
+                             float b1, float* fb, hipStream_t st) {
+    if (n <= 0 || (n0 <= 0 && n1 <= 0)) return hipSuccess;
+    hipLaunchKernelGGL(k_star_shade, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, o, n, rec0, n0, b0, rec1, n1,
+                       b1, fb);
+    return hipGetLastError();
+}
+
 }  // namespace sirius

@@ -3,6 +3,7 @@
 place) and from the FP64 oracle (the build's "pure FP64" columns).
 
 Run in the build container (it needs the reference tree): python tests/golden/make_golden.py
+(`python tests/golden/make_golden.py starfield` writes only the star-field fixture).
 Every fixture records its seed. The reference's GeodesicTracer::trace cannot be used as-is
 (it ends a ray on its first rejected step, GTRC001A.cpp:269-277), so the trace fixtures use the
 reference's Geodesic::integrateStepRK45 inside the harness loop of oracle/ref_harness.cpp.
@@ -90,7 +91,43 @@ def trace_fixture(name, M, a, r_obs, incl, fov, w, h, disk, n_bins, pix, seed):
                         fp64_steps=out.steps, fp64_state=out.state, crit=crit)
 
 
+def starfield_fixture(seed=0x51121008):
+    """The reference's star catalogue generator and sampler: two catalogues (the two celestial
+    spheres of the wormhole scene: 4096 candidates, 1 to 1000 pc, magnitude limit 20, seeds 42
+    and 43; and the reference's default limit with a clamped magnitude limit) and the sampled
+    star light at directions exactly on stars, at offsets of 1e-5 to 1e-2 rad from stars, and
+    at random directions, with and without the depth-of-field blur."""
+    rng = np.random.default_rng(seed)
+    cats = {"near": RF.starfield_generate(4096, 1.0, 1000.0, 20.0, 42),
+            "far": RF.starfield_generate(4096, 1.0, 1000.0, 20.0, 43),
+            "clamped": RF.starfield_generate(2000, 1.0, 10000.0, 25.0, 7)}
+    out = {"seed": seed}
+    for name, cat in cats.items():
+        out[f"cat_{name}"] = cat
+    cat = cats["near"]
+    on = cat[:64, :3]
+    off = []
+    for ang in (1e-5, 1e-4, 1e-3, 1e-2):
+        v = cat[64:96, :3].astype(np.float64)
+        t = rng.normal(size=v.shape)
+        t -= (t * v).sum(1, keepdims=True) * v
+        t /= np.linalg.norm(t, axis=1, keepdims=True)
+        d = np.cos(ang) * v + np.sin(ang) * t
+        off.append(d / np.linalg.norm(d, axis=1, keepdims=True))
+    rnd = rng.normal(size=(256, 3))
+    rnd /= np.linalg.norm(rnd, axis=1, keepdims=True)
+    dirs = np.concatenate([on, np.concatenate(off), rnd]).astype(np.float32)
+    out["dirs"] = dirs
+    out["rgb_dof"] = RF.starfield_sample(50.0, 100.0, 1.0, 1, cat, dirs)
+    out["rgb_nodof"] = RF.starfield_sample(0.0, 100.0, 2.0, 0, cat, dirs)
+    out["rgb_far_dof"] = RF.starfield_sample(50.0, 100.0, 1.0, 1, cats["far"], dirs)
+    np.savez_compressed(os.path.join(HERE, "ref_starfield.npz"), **out)
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "starfield":
+        starfield_fixture()
+        return
     if not RF.available():
         sys.exit("oracle/_ref/libsirius_ref.so missing: run make -C oracle ref (needs the reference tree)")
     metric_points()
@@ -103,6 +140,7 @@ def main():
     seed = 0x51121002
     pix = np.sort(np.random.default_rng(seed).choice(1920 * 1080, 3000, replace=False)).astype(np.int64)
     trace_fixture("ref_trace_c2.npz", 1.0, 0.9, 50.0, 75.0, 20.0, 1920, 1080, (0.0, 30.0), 16, pix, seed=seed)
+    starfield_fixture()
 
 
 if __name__ == "__main__":
