@@ -60,6 +60,9 @@ def test_wormhole_spheres_shade_escaped_rays(acc):
     o = O.default_opts(abi.PRECISION_FP32_ESCALATE)
     o.emission = abi.EMISSION_NONE
     acc.initialise(w, h)
+    # a black sky (the session's handle may hold an earlier test's background): escaped pixels
+    # then start at exactly 0, and star light of ~1e-7 is not lost against a bright background
+    acc.uploadBackground(np.zeros((1, 1, 4), np.uint8))
     near, n_near = starfield_generate(_cfg(42))
     far, n_far = starfield_generate(_cfg(43))
     print("[case] wormhole launch with two spheres", flush=True)
