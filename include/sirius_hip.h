@@ -42,7 +42,8 @@ enum {
     SIRIUS_HIP_E_DEVICE = -2,      /* HIP runtime error or no usable gfx950 device         */
     SIRIUS_HIP_E_NOMEM = -3,       /* device allocation failed                             */
     SIRIUS_HIP_E_STATE = -4,       /* call out of order (not initialised, no network, ...) */
-    SIRIUS_HIP_E_UNSUPPORTED = -5  /* option combination not implemented                   */
+    SIRIUS_HIP_E_UNSUPPORTED = -5, /* option combination not implemented                   */
+    SIRIUS_HIP_E_IO = -6           /* a file could not be written                          */
 };
 
 /* ---- enums ---------------------------------------------------------------------------- */
@@ -227,8 +228,9 @@ size_t sirius_hip_get_frame_buffer_size(SiriusHipHandle h);
 /* Equirectangular RGBA8 background (IAccelerator::uploadBackground, ACIB001A.h:176). */
 int sirius_hip_upload_background(SiriusHipHandle h, const unsigned char* rgba, int width, int height);
 int sirius_hip_synchronize(SiriusHipHandle h);
-/* Copies the most recent trace_batch results held in library device memory into host
- * buffers (host pointers; null ones skipped). */
+/* Copies the results of the most recent trace_batch or launch held in library device memory
+ * into host buffers (host pointers; null ones skipped). SIRIUS_HIP_E_STATE once a later call
+ * has replaced the library's result arrays. */
 int sirius_hip_fetch_results(SiriusHipHandle h, SiriusHipRayOutput* host_out);
 
 /* ---- physics: batched trace (new; no reference counterpart) --------------------------- */
@@ -343,6 +345,80 @@ int sirius_hip_set_starfield(SiriusHipHandle h, int side, const SiriusHipStarfie
 /* Star light of sphere `side` at n unit directions dirs[3n] (host arrays), rgb[3n] out, computed
  * on the device by the kernel the shading uses. */
 int sirius_hip_sample_starfield(SiriusHipHandle h, int side, const float* dirs, int n, float* rgb);
+
+/* ---- Image writers ----------------------------------------------------------------------
+ * A float RGBA frame (the framebuffer, or a caller's array) is converted on the device and
+ * packed into a file on the host. Each format decodes to exactly the pixels of the reference
+ * writer named beside it, for the same float RGBA input. */
+#define SIRIUS_IMAGE_PPM 0      /* BufferWriter::writePPM (BFIO001A.h:44-68): ACES tonemap of
+                                   value x exposure, 1/2.2 gamma, truncated to 8 bits; P6 RGB  */
+#define SIRIUS_IMAGE_PNG 1      /* PNGWriter::writeRGBA (OUPN001A.cpp:51-72): sRGB 8-bit RGBA */
+#define SIRIUS_IMAGE_EXR 2      /* EXRWriter::writeEXR, RGBA (OUEW001A.cpp:109-178,181-190):
+                                   channels A B G R as half (tinyexr's rounding), ZIP blocks of
+                                   16 scanlines                                               */
+#define SIRIUS_IMAGE_PFM 3      /* EXRWriter::writePFM (OUEW001A.h:110-126): RGB float, rows
+                                   bottom to top, little-endian                               */
+#define SIRIUS_IMAGE_RAW 4      /* BufferWriter::writeRaw (BFIO001A.h:71-86): int32 width,
+                                   height, 4, then the RGBA floats                             */
+#define SIRIUS_IMAGE_PPM_SRGB 5 /* EXRWriter::writePPM (OUEW001A.h:97-107): sRGB 8-bit, P6 RGB */
+
+typedef struct SiriusHipImageOpts {
+    int32_t format;      /* SIRIUS_IMAGE_*                                                    */
+    float exposure;      /* SIRIUS_IMAGE_PPM only (writePPM's exposure); default 1            */
+    int32_t zlib_level;  /* PNG and EXR: 0..9, or -1 for zlib's default (6)                   */
+    int32_t threads;     /* host threads that compress; 0: min(16, hardware threads)          */
+} SiriusHipImageOpts;
+
+typedef struct SiriusHipImageStats {
+    double convert_ms;       /* HIP-event time of the device conversion kernels               */
+    double copy_ms;          /* device -> host copy of the converted payload                 */
+    double pack_ms;          /* host compression and container                                */
+    double total_ms;         /* the whole call                                                */
+    uint64_t payload_bytes;  /* converted bytes copied from the device                       */
+    uint64_t file_bytes;
+} SiriusHipImageStats;
+
+void sirius_hip_default_image_opts(SiriusHipImageOpts* o, int format);
+/* Encodes a frame into library-owned memory: *data (valid until the next encode on this handle
+ * or its destruction) and *size. rgba null: the framebuffer of sirius_hip_launch (width and
+ * height 0 or its size); otherwise width x height RGBA floats, device memory if on_device,
+ * else host memory. stats may be null. Has no counterpart in the reference, whose writers take
+ * a host array and a path (below). */
+int sirius_hip_encode_image(SiriusHipHandle h, const float* rgba, int width, int height, int on_device,
+                            const SiriusHipImageOpts* o, const unsigned char** data, size_t* size,
+                            SiriusHipImageStats* stats);
+/* The same, written to `path` (PNGWriter::writeRGBA, EXRWriter::writeEXR, BufferWriter::writePPM
+ * and their kin; like them it fails for an empty frame or null pixels). */
+int sirius_hip_write_image(SiriusHipHandle h, const char* path, const float* rgba, int width, int height,
+                           int on_device, const SiriusHipImageOpts* o, SiriusHipImageStats* stats);
+
+/* ---- Ray result dump ------------------------------------------------------------------------
+ * The results of the last trace_batch or launch, with the cameras, metrics and options that made
+ * them, in one file, for audits that re-trace sampled rays on the CPU. Has no counterpart in
+ * the reference. Layout (little-endian):
+ *   0   char[8]  "SIRRAYS1"
+ *   8   uint32   header bytes (128)          12  uint32  n_bins
+ *   16  uint64   n_rays                      24  uint32  fields (SIRIUS_RAYS_* bits)
+ *   28  int32    n_cameras                   32  uint32  sizeof(SiriusHipCamera)
+ *   36  int32    n_metrics                   40  uint32  sizeof(SiriusHipMetric)
+ *   44  uint32   sizeof(SiriusHipTraceOpts), 0 without options; the rest of the header is zero
+ * then the camera, metric and option records, and then each field present, in bit order, each
+ * starting on an 8-byte boundary: n_rays entries, intensity n_bins x n_rays (bin-major), state
+ * n_rays x 8 doubles. */
+#define SIRIUS_RAYS_CLS (1u << 0)        /* uint8  */
+#define SIRIUS_RAYS_FLAGS (1u << 1)      /* uint8  */
+#define SIRIUS_RAYS_THETA (1u << 2)      /* float  */
+#define SIRIUS_RAYS_PHI (1u << 3)        /* float  */
+#define SIRIUS_RAYS_LAMBDA (1u << 4)     /* float  */
+#define SIRIUS_RAYS_DISK_R (1u << 5)     /* float  */
+#define SIRIUS_RAYS_REDSHIFT (1u << 6)   /* float  */
+#define SIRIUS_RAYS_NULL_DRIFT (1u << 7) /* float  */
+#define SIRIUS_RAYS_STEPS (1u << 8)      /* uint32 */
+#define SIRIUS_RAYS_INTENSITY (1u << 9)  /* float  */
+#define SIRIUS_RAYS_STATE (1u << 10)     /* double */
+#define SIRIUS_RAYS_ALL 0x7ffu
+int sirius_hip_write_rays(SiriusHipHandle h, const char* path, uint32_t fields, const SiriusHipCamera* cams,
+                          int n_cams, const SiriusHipMetric* metrics, int n_metrics, const SiriusHipTraceOpts* opts);
 
 #ifdef __cplusplus
 }

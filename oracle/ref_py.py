@@ -137,3 +137,93 @@ def starfield_sample(aperture_mm, focus_pc, brightness, dof, stars, dirs):
     lib().ref_starfield_sample(aperture_mm, focus_pc, brightness, int(dof), stars.ctypes.data, len(stars),
                                dirs.ctypes.data, len(dirs), rgb.ctypes.data)
     return rgb
+
+
+_img = None
+IMAGE_LIB_PATH = os.path.join(_HERE, "_ref", "libsirius_ref_image.so")
+
+
+def image_lib():
+    """The reference's image writers (PNGWriter, EXRWriter, BufferWriter) and the decoders
+    vendored with it (stb_image, tinyexr's loader)."""
+    global _img
+    if _img is None:
+        L = C.CDLL(IMAGE_LIB_PATH)
+        s, i, p, f = C.c_char_p, C.c_int, C.c_void_p, C.c_float
+        L.ref_write_png_rgba.argtypes = [s, i, i, p]
+        L.ref_write_exr_rgba.argtypes = [s, i, i, p]
+        L.ref_write_ppm_aces.argtypes = [s, p, i, i, f]
+        L.ref_write_ppm_srgb.argtypes = [s, i, i, p]
+        L.ref_write_pfm.argtypes = [s, i, i, p]
+        L.ref_write_raw.argtypes = [s, p, i, i, i]
+        L.ref_load_png.argtypes = [C.c_char_p, i, C.POINTER(i), C.POINTER(i), p, i]
+        L.ref_load_exr.argtypes = [C.c_char_p, i, C.POINTER(i), C.POINTER(i), p, i]
+        _img = L
+    return _img
+
+
+def _written(fn, *args):
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "out")
+        assert fn(path.encode(), *args) == 1, fn.__name__
+        with open(path, "rb") as fh:
+            return fh.read()
+
+
+def write_png(rgba):
+    a = np.ascontiguousarray(rgba, np.float32)
+    return _written(image_lib().ref_write_png_rgba, a.shape[1], a.shape[0], a.ctypes.data)
+
+
+def write_exr(rgba):
+    a = np.ascontiguousarray(rgba, np.float32)
+    return _written(image_lib().ref_write_exr_rgba, a.shape[1], a.shape[0], a.ctypes.data)
+
+
+def write_ppm_aces(rgba, exposure=1.0):
+    a = np.ascontiguousarray(rgba, np.float32)
+    L = image_lib()
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "out")
+        assert L.ref_write_ppm_aces(path.encode(), a.ctypes.data, a.shape[1], a.shape[0], exposure) == 1
+        with open(path, "rb") as fh:
+            return fh.read()
+
+
+def write_ppm_srgb(rgb):
+    a = np.ascontiguousarray(rgb, np.float32)
+    return _written(image_lib().ref_write_ppm_srgb, a.shape[1], a.shape[0], a.ctypes.data)
+
+
+def write_pfm(rgb):
+    a = np.ascontiguousarray(rgb, np.float32)
+    return _written(image_lib().ref_write_pfm, a.shape[1], a.shape[0], a.ctypes.data)
+
+
+def write_raw(rgba):
+    a = np.ascontiguousarray(rgba, np.float32)
+    L = image_lib()
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "out")
+        assert L.ref_write_raw(path.encode(), a.ctypes.data, a.shape[1], a.shape[0], a.shape[2]) == 1
+        with open(path, "rb") as fh:
+            return fh.read()
+
+
+def load_png(data, h, w):
+    out = np.zeros((h, w, 4), np.uint8)
+    wi, hi = C.c_int(), C.c_int()
+    assert image_lib().ref_load_png(data, len(data), C.byref(wi), C.byref(hi), out.ctypes.data, out.nbytes) == 1
+    assert (wi.value, hi.value) == (w, h)
+    return out
+
+
+def load_exr(data, h, w):
+    out = np.zeros((h, w, 4), np.float32)
+    wi, hi = C.c_int(), C.c_int()
+    assert image_lib().ref_load_exr(data, len(data), C.byref(wi), C.byref(hi), out.ctypes.data, out.size) == 1
+    assert (wi.value, hi.value) == (w, h)
+    return out

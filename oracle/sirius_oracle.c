@@ -1175,6 +1175,8 @@ int oracle_abi_layout(int64_t* out, int n) {
         (int64_t)offsetof(SiriusHipCamera, width), (int64_t)offsetof(SiriusHipTraceOpts, n_bins),
         (int64_t)offsetof(SiriusHipTraceOpts, vol_tau_max), (int64_t)offsetof(SiriusHipRayOutput, on_device),
         (int64_t)offsetof(SiriusHipStats, kernel_ms), (int64_t)offsetof(SiriusHipStats, null_drift_median),
+        (int64_t)sizeof(SiriusHipImageOpts), (int64_t)sizeof(SiriusHipImageStats),
+        (int64_t)offsetof(SiriusHipImageStats, payload_bytes),
     };
     int m = (int)(sizeof(v) / sizeof(v[0]));
     for (int i = 0; i < n && i < m; ++i) out[i] = v[i];

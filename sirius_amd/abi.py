@@ -9,7 +9,7 @@ import numpy as np
 
 ABI_VERSION = 1
 
-OK, E_INVALID, E_DEVICE, E_NOMEM, E_STATE, E_UNSUPPORTED = 0, -1, -2, -3, -4, -5
+OK, E_INVALID, E_DEVICE, E_NOMEM, E_STATE, E_UNSUPPORTED, E_IO = 0, -1, -2, -3, -4, -5, -6
 METRIC_KERR_SCHILD, METRIC_MORRIS_THORNE, METRIC_ALCUBIERRE = 0, 1, 2
 INTEGRATOR_DP45, INTEGRATOR_SYMPLECTIC = 0, 1
 PRECISION_FP64, PRECISION_FP32_ESCALATE = 0, 1
@@ -19,6 +19,10 @@ RAY_ESCAPED, RAY_CAPTURED, RAY_DISK, RAY_MAX_STEPS, RAY_SPIRAL, RAY_FAILED = 0, 
 FLAG_ESCALATED, FLAG_SURROGATE, FLAG_RENORM, FLAG_AUDITED, FLAG_FAR_SIDE, FLAG_CRITICAL = 1, 2, 4, 8, 16, 32
 
 MLP_IN, MLP_HIDDEN, MLP_LAYERS, MLP_OUT = 5, 256, 4, 6
+IMAGE_PPM, IMAGE_PNG, IMAGE_EXR, IMAGE_PFM, IMAGE_RAW, IMAGE_PPM_SRGB = 0, 1, 2, 3, 4, 5
+RAYS_FIELDS = ("cls", "flags", "theta", "phi", "lambda", "disk_r", "redshift", "null_drift", "steps",
+               "intensity", "state")  # bit i of SIRIUS_RAYS_* is RAYS_FIELDS[i]
+RAYS_ALL = (1 << len(RAYS_FIELDS)) - 1
 
 
 class Metric(C.Structure):
@@ -105,6 +109,18 @@ class Starfield(C.Structure):
                 ("magnitude_limit", C.c_float), ("aperture_mm", C.c_float),
                 ("focus_distance_pc", C.c_float), ("brightness_scale", C.c_float),
                 ("enable_dof", C.c_int32), ("reserved", C.c_int32)]
+
+
+class ImageOpts(C.Structure):
+    _fields_ = [("format", C.c_int32), ("exposure", C.c_float), ("zlib_level", C.c_int32), ("threads", C.c_int32)]
+
+
+class ImageStats(C.Structure):
+    _fields_ = [("convert_ms", C.c_double), ("copy_ms", C.c_double), ("pack_ms", C.c_double),
+                ("total_ms", C.c_double), ("payload_bytes", C.c_uint64), ("file_bytes", C.c_uint64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 def metric_ks(M=1.0, a=0.0, Q=0.0, Lambda=0.0):

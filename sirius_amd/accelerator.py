@@ -78,6 +78,15 @@ def lib():
     L.sirius_hip_starfield_generate.argtypes = [C.POINTER(abi.Starfield), C.c_void_p, i, C.POINTER(C.c_int)]
     L.sirius_hip_set_starfield.argtypes = [H, i, C.POINTER(abi.Starfield), C.c_void_p, i]
     L.sirius_hip_sample_starfield.argtypes = [H, i, C.c_void_p, i, C.c_void_p]
+    L.sirius_hip_default_image_opts.argtypes = [C.POINTER(abi.ImageOpts), i]
+    L.sirius_hip_default_image_opts.restype = None
+    L.sirius_hip_encode_image.argtypes = [H, C.c_void_p, i, i, i, C.POINTER(abi.ImageOpts),
+                                          C.POINTER(C.POINTER(C.c_ubyte)), C.POINTER(C.c_size_t),
+                                          C.POINTER(abi.ImageStats)]
+    L.sirius_hip_write_image.argtypes = [H, C.c_char_p, C.c_void_p, i, i, i, C.POINTER(abi.ImageOpts),
+                                         C.POINTER(abi.ImageStats)]
+    L.sirius_hip_write_rays.argtypes = [H, C.c_char_p, C.c_uint32, C.POINTER(abi.Camera), i,
+                                        C.POINTER(abi.Metric), i, C.POINTER(abi.TraceOpts)]
     _lib = L
     return L
 
@@ -187,6 +196,45 @@ class HipAccelerator:
         rgb = np.zeros((len(dirs), 3), np.float32)
         self._check(self._L.sirius_hip_sample_starfield(self._h, side, dirs.ctypes.data, len(dirs), rgb.ctypes.data))
         return rgb
+
+    # ---- image writers and the ray result dump ----
+    def _image_args(self, fmt, rgba, exposure, zlib_level, threads):
+        o = abi.ImageOpts()
+        self._L.sirius_hip_default_image_opts(C.byref(o), fmt)
+        o.exposure, o.zlib_level, o.threads = exposure, zlib_level, threads
+        if rgba is None:  # the framebuffer of the last launch
+            return o, None, 0, 0, 0, None
+        if hasattr(rgba, "data_ptr"):  # device memory (float32, contiguous, (h, w, 4))
+            h, w = int(rgba.shape[0]), int(rgba.shape[1])
+            return o, rgba.data_ptr(), w, h, 1, rgba
+        a = np.ascontiguousarray(rgba, np.float32)
+        h, w = a.shape[:2]
+        return o, a.ctypes.data, w, h, 0, a
+
+    def encode_image(self, fmt, rgba=None, exposure=1.0, zlib_level=-1, threads=0):
+        """The file bytes of a frame ((h, w, 4) float RGBA on the host, or device memory with
+        data_ptr() and shape; None:
+        the framebuffer) in format `fmt` (abi.IMAGE_*), and the call's ImageStats."""
+        o, ptr, w, h, dev, keep = self._image_args(fmt, rgba, exposure, zlib_level, threads)
+        data, size, st = C.POINTER(C.c_ubyte)(), C.c_size_t(), abi.ImageStats()
+        self._check(self._L.sirius_hip_encode_image(self._h, ptr, w, h, dev, C.byref(o), C.byref(data),
+                                                    C.byref(size), C.byref(st)))
+        return C.string_at(data, size.value), st
+
+    def write_image(self, path, fmt, rgba=None, exposure=1.0, zlib_level=-1, threads=0):
+        o, ptr, w, h, dev, keep = self._image_args(fmt, rgba, exposure, zlib_level, threads)
+        st = abi.ImageStats()
+        self._check(self._L.sirius_hip_write_image(self._h, os.fsencode(path), ptr, w, h, dev, C.byref(o),
+                                                   C.byref(st)))
+        return st
+
+    def write_rays(self, path, cams=(), metrics=(), opts=None, fields=abi.RAYS_ALL):
+        """Dumps the last trace's results with the cameras, metrics and options that made them
+        (read back with sirius_amd.raydump.read_rays)."""
+        ca = (abi.Camera * max(len(cams), 1))(*cams)
+        ma = (abi.Metric * max(len(metrics), 1))(*metrics)
+        self._check(self._L.sirius_hip_write_rays(self._h, os.fsencode(path), fields, ca, len(cams), ma,
+                                                  len(metrics), C.byref(opts) if opts is not None else None))
 
     def synchronise(self):
         self._check(self._L.sirius_hip_synchronize(self._h))

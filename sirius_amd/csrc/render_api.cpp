@@ -148,6 +148,11 @@ int sirius_hip_launch(SiriusHipHandle hh, const SiriusHipCamera* cam, const Siri
     if (rc) return rc;
     DevOut dout;
     if ((rc = bind_outputs(*H, nullptr, B.n, opts->n_bins, dout, opts->emission == SIRIUS_EMISSION_VOLUMETRIC))) return rc;
+    // the frame's rays overwrite the library's result arrays: they are now the last results
+    // (sirius_hip_fetch_results, sirius_hip_write_rays)
+    if (!H->last) H->last = new DevOut;
+    *H->last = dout;
+    H->last_bins = opts->n_bins;
     rc = run_trace(*H, B, *opts, dout, nullptr, nullptr, 0, nullptr, false);
     if (rc) return rc;
     hipError_t e = launch_shade(dout, B.n, c.width, c.height, opts->n_bins, H->bg_dev, H->bg_w, H->bg_h, H->fb_dev,

@@ -219,6 +219,11 @@ void Handle::release() {
     fb_dev = nullptr;
     if (bg_dev) (void)hipFree(bg_dev);
     bg_dev = nullptr;
+    if (pinned) (void)hipHostFree(pinned);
+    pinned = nullptr;
+    pinned_bytes = 0;
+    image.clear();
+    image.shrink_to_fit();
     mlp_release(*this);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -284,6 +289,18 @@ void* Handle::scratch(const char* name, size_t bytes, int* rc) {
     // a replaced buffer no longer holds an upload (a new allocation may reuse its address)
     if (b.ptr == cam_dev_held) cam_dev_held = nullptr;
     if (b.ptr == crit_dev_held) crit_dev_held = nullptr;
+    // nor the last trace's results: fetching them afterwards is an error, not a read of freed memory
+    if (b.ptr && last) {
+        const void* held[] = {last->cls,      last->flags,    last->theta, last->phi,       last->lam,
+                              last->disk_r,   last->redshift, last->null_drift, last->steps, last->rejected,
+                              last->landings, last->intensity, last->state, last->vol_acc};
+        for (const void* p : held)
+            if (p == b.ptr) {
+                delete last;
+                last = nullptr;
+                break;
+            }
+    }
     if (b.ptr) (void)hipFree(b.ptr);
     b.ptr = nullptr;
     b.bytes = 0;

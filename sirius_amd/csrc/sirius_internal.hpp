@@ -87,6 +87,10 @@ struct Handle {
     float* star_dev[2] = {nullptr, nullptr};
     int star_n[2] = {0, 0};
     float star_brightness[2] = {1.f, 1.f};
+    // image writers: the last encoded file and the pinned host staging of converted payloads
+    std::vector<unsigned char> image;
+    unsigned char* pinned = nullptr;
+    size_t pinned_bytes = 0;
     // surrogate network
     MlpState* mlp = nullptr;
 

@@ -124,9 +124,68 @@ def starfield_fixture(seed=0x51121008):
     np.savez_compressed(os.path.join(HERE, "ref_starfield.npz"), **out)
 
 
+def image_frames(seed=0x51121009):
+    """Three float RGBA frames for the image writers. ldr (35 x 37): values across [-0.2, 1.3],
+    around the sRGB threshold 0.0031308 (it, and the floats next to it), small values down to
+    1e-6 and HDR values up to 50 for the ACES curve, alpha over [-0.1, 1.1]. hdr (35 x 37): NaN,
+    +-Inf, +-0, float denormals, half denormals, exact half ties, the half overflow boundary
+    (65504, 65519, 65520) and log-uniform values of both signs from 1e-9 to 1e6. noise (20 x 33):
+    random finite floats of random exponents, which deflate does not shrink (tinyexr then stores
+    the block plain)."""
+    rng = np.random.default_rng(seed)
+    h, w = 35, 37
+    ldr = rng.uniform(-0.2, 1.3, (h, w, 4)).astype(np.float32)
+    ldr[..., 3] = rng.uniform(-0.1, 1.1, (h, w))
+    t = np.float32(0.0031308)
+    special = [t, np.nextafter(t, np.float32(0)), np.nextafter(t, np.float32(1)), 0.0, 1.0, 0.5, 1e-6, 1e-4, 1e-3,
+               0.04045, 0.99999994, 1.0000001]
+    flat = ldr.reshape(-1, 4)
+    flat[:len(special), :3] = np.array(special, np.float32)[:, None]
+    flat[40:140, :3] = np.exp(rng.uniform(np.log(1e-6), np.log(1e-2), (100, 3))).astype(np.float32)
+    flat[200:300, :3] = rng.uniform(1.0, 50.0, (100, 3)).astype(np.float32)
+    hdr = (np.exp(rng.uniform(np.log(1e-9), np.log(1e6), (h, w, 4))) * rng.choice([-1.0, 1.0], (h, w, 4))).astype(np.float32)
+    ties = [1.0 + 2.0 ** -11, 1.0 + 3 * 2.0 ** -11, 2.0 ** -15 * (1 + 2.0 ** -11), 2.0 ** -20, 3 * 2.0 ** -25,
+            2.0 ** -25, 2.0 ** -24, 1.5 * 2.0 ** -24, 65504.0, 65519.0, 65520.0, 1e5, 6.1e-5, 6.0e-8]
+    specials = [np.nan, np.inf, -np.inf, 0.0, -0.0, 1e-40, -1e-41] + ties + [-v for v in ties]
+    fh = hdr.reshape(-1, 4)
+    fh[:len(specials), :] = np.array(specials, np.float32)[:, None]
+    fh[len(specials):2 * len(specials), :] = np.roll(np.array(specials, np.float32), 1)[:, None]
+    noise = rng.integers(0, 2 ** 32, (20, 33, 4), dtype=np.uint64).astype(np.uint32)
+    noise = noise & np.uint32(0x8FFFFFFF) | np.uint32(0x30000000)  # finite, exponents spread
+    return {"ldr": ldr, "hdr": hdr, "noise": noise.view(np.float32)}
+
+
+def images_fixture():
+    """The reference writers' files for the frames of image_frames(): PNG (PNGWriter::writeRGBA)
+    and the two PPMs (BufferWriter::writePPM at exposure 1 and 1.7, EXRWriter::writePPM) of the
+    ldr frame; EXR (EXRWriter::writeEXR), PFM (EXRWriter::writePFM) and the raw dump
+    (BufferWriter::writeRaw) of the hdr frame; EXR of the noise frame; and the decodings by the
+    reference's vendored stb_image and tinyexr."""
+    fr = image_frames()
+    out = {f"frame_{k}": v for k, v in fr.items()}
+    b = lambda data: np.frombuffer(data, np.uint8)
+    ldr, hdr = fr["ldr"], fr["hdr"]
+    png = RF.write_png(ldr)
+    out["png_ldr"] = b(png)
+    out["png_ldr_stb"] = RF.load_png(png, *ldr.shape[:2])
+    out["ppm_aces_ldr_e1"] = b(RF.write_ppm_aces(ldr, 1.0))
+    out["ppm_aces_ldr_e1.7"] = b(RF.write_ppm_aces(ldr, 1.7))
+    out["ppm_srgb_ldr"] = b(RF.write_ppm_srgb(ldr[..., :3]))
+    for name in ("hdr", "noise"):
+        exr = RF.write_exr(fr[name])
+        out[f"exr_{name}"] = b(exr)
+        out[f"exr_{name}_tinyexr"] = RF.load_exr(exr, *fr[name].shape[:2])
+    out["pfm_hdr"] = b(RF.write_pfm(hdr[..., :3]))
+    out["raw_hdr"] = b(RF.write_raw(hdr))
+    np.savez_compressed(os.path.join(HERE, "ref_images.npz"), **out)
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "starfield":
         starfield_fixture()
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "images":
+        images_fixture()
         return
     if not RF.available():
         sys.exit("oracle/_ref/libsirius_ref.so missing: run make -C oracle ref (needs the reference tree)")
@@ -141,6 +200,7 @@ def main():
     pix = np.sort(np.random.default_rng(seed).choice(1920 * 1080, 3000, replace=False)).astype(np.int64)
     trace_fixture("ref_trace_c2.npz", 1.0, 0.9, 50.0, 75.0, 20.0, 1920, 1080, (0.0, 30.0), 16, pix, seed=seed)
     starfield_fixture()
+    images_fixture()
 
 
 if __name__ == "__main__":

@@ -42,7 +42,8 @@ def test_struct_layout_matches_c():
     want = [C.sizeof(abi.Metric), C.sizeof(abi.Camera), C.sizeof(abi.TraceOpts), C.sizeof(abi.RayOutput),
             C.sizeof(abi.Stats), abi.Camera.width.offset, abi.TraceOpts.n_bins.offset,
             abi.TraceOpts.vol_tau_max.offset, abi.RayOutput.on_device.offset, abi.Stats.kernel_ms.offset,
-            abi.Stats.null_drift_median.offset]
+            abi.Stats.null_drift_median.offset, C.sizeof(abi.ImageOpts), C.sizeof(abi.ImageStats),
+            abi.ImageStats.payload_bytes.offset]
     assert list(out[:n]) == want
 
 
