@@ -142,6 +142,55 @@ def other_workloads(acc, steps, warmup):
         o = default_opts(abi.PRECISION_FP32_ESCALATE)
         o.emission = abi.EMISSION_NONE
         out[name] = measure([cam], [metric], o, steps, warmup)
+    out["image_writers_1080p"] = image_writers(acc, steps)
+    return out
+
+
+# Algorithmic HBM bytes per pixel of each format's device conversion: the float RGBA frame read
+# (16 B) plus the payload written; PNG also writes and re-reads its 8-bit pixels for the row
+# filters (4 + 4 B) before writing the filtered rows (4 B).
+IMAGE_BYTES_PER_PIXEL = {"png": 16 + 4 + 4 + 4, "exr": 16 + 8, "ppm": 16 + 3}
+
+
+def image_writers(acc, steps):
+    """The 1080p Kerr frame written to files as PNG, EXR and PPM from the framebuffer in device
+    memory: the device conversion (HIP events), the copy of the payload to pinned host memory, the
+    host compression on up to 16 threads and the whole call, per format; beside it the reference's
+    own writers
+    (PNGWriter::writeRGBA, EXRWriter::writeEXR, BufferWriter::writePPM, compiled in place) on the
+    same frame on one host core."""
+    from sirius_amd import abi
+    from sirius_amd.accelerator import default_opts
+    cam, metric = workload()
+    acc.initialise(W, H)
+    acc.launch(cam, metric, default_opts(abi.PRECISION_FP32_ESCALATE))
+    frame = acc.getFrameBuffer().reshape(H, W, 4).copy()
+    out = {"frame": "1920x1080 RGBA float (the Kerr a=0.9 benchmark frame, shaded)"}
+    fmts = (("png", abi.IMAGE_PNG), ("exr", abi.IMAGE_EXR), ("ppm", abi.IMAGE_PPM))
+    import tempfile
+    tmp = tempfile.mkdtemp()
+    for name, fmt in fmts:
+        path = os.path.join(tmp, "frame." + name)
+        acc.write_image(path, fmt)  # warm: staging buffers and threads
+        # the whole call, file written, as the reference writers' time includes theirs
+        sts = [acc.write_image(path, fmt) for _ in range(max(3, steps))]
+        mean = lambda k: float(np.mean([getattr(x, k) for x in sts]))
+        conv = mean("convert_ms")
+        out[name] = {"total_ms": mean("total_ms"), "convert_ms": conv, "copy_ms": mean("copy_ms"),
+                     "pack_ms": mean("pack_ms"), "file_bytes": int(sts[-1].file_bytes),
+                     "convert_GBps": IMAGE_BYTES_PER_PIXEL[name] * W * H / (conv * 1e-3) / 1e9}
+    try:
+        from oracle import ref_py as RF
+        if os.path.exists(RF.IMAGE_LIB_PATH):
+            for name, fn in (("png", RF.write_png), ("exr", RF.write_exr),
+                             ("ppm", lambda f: RF.write_ppm_aces(f, 1.0))):
+                t0 = time.perf_counter()
+                fn(frame)
+                ms = (time.perf_counter() - t0) * 1e3
+                out[name]["reference_cpu_ms"] = ms
+                out[name]["speedup_vs_reference_cpu"] = ms / out[name]["total_ms"]
+    except Exception as e:  # the reference build is optional on a machine
+        out["reference_cpu"] = f"unavailable: {e}"
     return out
 
 
