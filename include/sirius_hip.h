@@ -346,6 +346,49 @@ int sirius_hip_set_starfield(SiriusHipHandle h, int side, const SiriusHipStarfie
  * on the device by the kernel the shading uses. */
 int sirius_hip_sample_starfield(SiriusHipHandle h, int side, const float* dirs, int n, float* rgb);
 
+/* ---- Ray bundles ---------------------------------------------------------------------------
+ * The geodesic deviation of each pixel's ray: two deviation vectors, the derivatives of the ray's
+ * state with respect to the image coordinates (u, v), carried along the ray as the linearised
+ * flow (the geodesic deviation equation in Hamiltonian form), and from them the derivative of
+ * the ray's outcome: the Jacobian of the escape direction (theta, phi) for an escaped ray, in rad
+ * per pixel, or of the equatorial hit point (X, Y) for a ray ending on the thin disk, in M per
+ * pixel; and the beam, the image of a circle of radius size pixels at the camera: semi-axes and
+ * the major axis's angle (on the sky from e_theta towards e_phi, in radians; on the disk from
+ * the X axis). Replaces the ray bundle of the reference's device kernel
+ * (RDOP002A.cu:4239-4354, 5014-5047, 5364-5439), whose parameters are LaunchParams::rayBundle
+ * (RDOP003A.h:1063-1069). */
+#define SIRIUS_BUNDLE_TERMINATED 1 /* beam unusable (captured, too elongated or too small): point-sampled */
+
+typedef struct SiriusHipRayBundle {
+    int32_t enabled;           /* sirius_hip_launch averages star light over each pixel's beam      */
+    int32_t use_full_riemann;  /* accepted for the reference's field; the deviation is always the
+                                  exact linearised flow                                             */
+    float initial_size;        /* beam radius at the camera, pixels; default 0.2887 (the standard
+                                  deviation of a unit box pixel)                                    */
+    float min_bundle_size;     /* beams whose semi-major axis is below this are terminated
+                                  (default 1e-6, the reference's)                                   */
+    float max_ellipse_ratio;   /* beams more elongated than this are terminated (default 100)       */
+    int32_t reserved0;
+} SiriusHipRayBundle;
+
+typedef struct SiriusHipBundleOutput {
+    uint8_t* cls;        /* SIRIUS_RAY_* of the bundle's central ray                          */
+    uint8_t* flags;      /* SIRIUS_BUNDLE_TERMINATED                                          */
+    float* jac;          /* 4 * n_rays, component-major: da/du, da/dv, db/du, db/dv           */
+    float* beam;         /* 3 * n_rays, component-major: semi-major, semi-minor, orientation  */
+    int32_t on_device;   /* as SiriusHipRayOutput.on_device                                   */
+    int32_t reserved0;
+} SiriusHipBundleOutput;
+
+void sirius_hip_default_ray_bundle(SiriusHipRayBundle* b);
+/* Bundle parameters of later sirius_hip_launch calls (enabled = 0: point-sampled stars). */
+int sirius_hip_set_ray_bundle(SiriusHipHandle h, const SiriusHipRayBundle* b);
+/* Traces every pixel's ray bundle in FP64 (DP45 with opts' tolerances, step limits and
+ * far-field cap; precision is ignored) and writes the bundle outputs; beams use the parameters
+ * of sirius_hip_set_ray_bundle (its defaults if never set). */
+int sirius_hip_trace_bundle(SiriusHipHandle h, const SiriusHipCamera* cams, int n_cam, const SiriusHipMetric* metrics,
+                            int n_metric, const SiriusHipTraceOpts* opts, SiriusHipBundleOutput* out);
+
 /* ---- Image writers ----------------------------------------------------------------------
  * A float RGBA frame (the framebuffer, or a caller's array) is converted on the device and
  * packed into a file on the host. Each format decodes to exactly the pixels of the reference

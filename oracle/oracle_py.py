@@ -293,3 +293,56 @@ def starfield_sample(stars, dirs, aperture_mm=50.0, focus_pc=100.0, brightness=1
             for k in range(3):
                 out[keep, k] = out[keep, k] + col[k] * inten[keep]
     return out
+
+
+# ---- ray bundles: derivatives of traced outcomes with respect to the image point ------------
+def set_subpixel(du, dv):
+    """Offsets every later ray's image point by (du, dv) pixels (0, 0 restores the pixel centres)."""
+    L = lib()
+    L.oracle_set_subpixel.argtypes = [C.c_double, C.c_double]
+    L.oracle_set_subpixel.restype = None
+    L.oracle_set_subpixel(du, dv)
+
+
+def trace_subpixel(cams, metrics, opts, ray_ids, du, dv):
+    set_subpixel(du, dv)
+    try:
+        return trace(cams, metrics, opts, ray_ids=ray_ids, with_state=True)[0]
+    finally:
+        set_subpixel(0.0, 0.0)
+
+
+def outcome_coords(metric_, rays):
+    """Per ray, from the final FP64 state: the escape direction (theta, phi) of k^i (escaped rays of
+    the Cartesian families) and the position (X, Y) (rays on the disk), as (2, n) each."""
+    n = len(rays.cls)
+    ang = np.zeros((2, n))
+    xy = np.zeros((2, n))
+    for i in range(n):
+        x, p = rays.state[i, :4], rays.state[i, 4:]
+        k = metric(metric_, x)[1] @ p
+        d = k[1:] / np.linalg.norm(k[1:])
+        ang[0, i] = np.arccos(np.clip(d[2], -1.0, 1.0))
+        ang[1, i] = np.arctan2(d[1], d[0])
+        xy[:, i] = x[1], x[2]
+    return ang, xy
+
+
+def lens_jacobian(cams, metrics, opts, ray_ids, eps=0.01):
+    """Central differences over +-eps pixel of whole FP64 traces: J (4, n) = d(a, b)/d(u, v) with
+    (a, b) = (theta, phi) for escaped rays and (X, Y) for disk rays (the bundle outputs' layout),
+    the central classes, and a mask of rays whose four offset traces end in the same class."""
+    base = trace(cams, metrics, opts, ray_ids=ray_ids, with_state=True)[0]
+    m = metrics[cams[0].metric_index]
+    J = np.zeros((4, len(ray_ids)))
+    same = np.ones(len(ray_ids), bool)
+    for a in (0, 1):
+        plus = trace_subpixel(cams, metrics, opts, ray_ids, eps if a == 0 else 0.0, eps if a == 1 else 0.0)
+        minus = trace_subpixel(cams, metrics, opts, ray_ids, -eps if a == 0 else 0.0, -eps if a == 1 else 0.0)
+        same &= (plus.cls == base.cls) & (minus.cls == base.cls)
+        (ap, xyp), (am, xym) = outcome_coords(m, plus), outcome_coords(m, minus)
+        dph = np.remainder(ap[1] - am[1] + np.pi, 2 * np.pi) - np.pi
+        esc = base.cls == 0
+        J[a] = np.where(esc, (ap[0] - am[0]) / (2 * eps), (xyp[0] - xym[0]) / (2 * eps))
+        J[2 + a] = np.where(esc, dph / (2 * eps), (xyp[1] - xym[1]) / (2 * eps))
+    return J, base.cls, same

@@ -340,11 +340,20 @@ void oracle_default_opts(SiriusHipTraceOpts* o, int precision) {
 /* ========================================================================================
  * Camera (CMBS001A.h:120-185) and initial state (GTRC001A.cpp:15-134), in FP64
  * ======================================================================================== */
+/* Offset (in pixels) added to every ray's image point; 0 except while a test differentiates
+ * traces with respect to the image coordinates (oracle_set_subpixel). */
+static double g_sub_u = 0.0, g_sub_v = 0.0;
+
+void oracle_set_subpixel(double du, double dv) {
+    g_sub_u = du;
+    g_sub_v = dv;
+}
+
 static void camera_direction(const SiriusHipCamera* c, int px, int py, double n_sph[3]) {
     double aspect = (double)c->width / c->height;
     double th = tan(c->fov_deg * M_PI / 360.0);
-    double sx = (2.0 * (px + 0.5) / c->width - 1.0) * aspect;
-    double sy = 1.0 - 2.0 * (py + 0.5) / c->height;
+    double sx = (2.0 * (px + 0.5 + g_sub_u) / c->width - 1.0) * aspect;
+    double sy = 1.0 - 2.0 * (py + 0.5 + g_sub_v) / c->height;
     double dx = sx * th, dy = sy * th, dz = -1.0;
     double cy = cos(c->yaw), sy_ = sin(c->yaw), cp = cos(c->pitch), sp = sin(c->pitch);
     double cr = cos(c->roll), sr = sin(c->roll);

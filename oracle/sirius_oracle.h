@@ -38,6 +38,9 @@ double oracle_disk_tmax_f_over_r(double a, double r_in, double r_out);
 double oracle_rhs(const SiriusHipMetric* m, const double x[4], const double p[4], double dx[4],
                   double dp[4]);
 /* Initial state of pixel (px, py) of a camera: x^mu and p_mu (null, future-directed). */
+/* Offsets every later ray's image point by (du, dv) pixels (a test differentiating traces with
+ * respect to the image coordinates); 0, 0 restores the pixel centres. */
+void oracle_set_subpixel(double du, double dv);
 void oracle_camera_state(const SiriusHipCamera* cam, const SiriusHipMetric* m, int px, int py,
                          double x[4], double p[4]);
 /* Distance (M) of the pixel's (alpha, beta) from the critical curve, negative inside. */

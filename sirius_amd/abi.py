@@ -111,6 +111,19 @@ class Starfield(C.Structure):
                 ("enable_dof", C.c_int32), ("reserved", C.c_int32)]
 
 
+BUNDLE_TERMINATED = 1
+
+
+class RayBundle(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("use_full_riemann", C.c_int32), ("initial_size", C.c_float),
+                ("min_bundle_size", C.c_float), ("max_ellipse_ratio", C.c_float), ("reserved0", C.c_int32)]
+
+
+class BundleOutput(C.Structure):
+    _fields_ = [("cls", C.c_void_p), ("flags", C.c_void_p), ("jac", C.c_void_p), ("beam", C.c_void_p),
+                ("on_device", C.c_int32), ("reserved0", C.c_int32)]
+
+
 class ImageOpts(C.Structure):
     _fields_ = [("format", C.c_int32), ("exposure", C.c_float), ("zlib_level", C.c_int32), ("threads", C.c_int32)]
 

@@ -78,6 +78,11 @@ def lib():
     L.sirius_hip_starfield_generate.argtypes = [C.POINTER(abi.Starfield), C.c_void_p, i, C.POINTER(C.c_int)]
     L.sirius_hip_set_starfield.argtypes = [H, i, C.POINTER(abi.Starfield), C.c_void_p, i]
     L.sirius_hip_sample_starfield.argtypes = [H, i, C.c_void_p, i, C.c_void_p]
+    L.sirius_hip_default_ray_bundle.argtypes = [C.POINTER(abi.RayBundle)]
+    L.sirius_hip_default_ray_bundle.restype = None
+    L.sirius_hip_set_ray_bundle.argtypes = [H, C.POINTER(abi.RayBundle)]
+    L.sirius_hip_trace_bundle.argtypes = [H, C.POINTER(abi.Camera), i, C.POINTER(abi.Metric), i,
+                                          C.POINTER(abi.TraceOpts), C.POINTER(abi.BundleOutput)]
     L.sirius_hip_default_image_opts.argtypes = [C.POINTER(abi.ImageOpts), i]
     L.sirius_hip_default_image_opts.restype = None
     L.sirius_hip_encode_image.argtypes = [H, C.c_void_p, i, i, i, C.POINTER(abi.ImageOpts),
@@ -89,6 +94,12 @@ def lib():
                                         C.POINTER(abi.Metric), i, C.POINTER(abi.TraceOpts)]
     _lib = L
     return L
+
+
+def ray_bundle_default():
+    b = abi.RayBundle()
+    lib().sirius_hip_default_ray_bundle(C.byref(b))
+    return b
 
 
 def starfield_default():
@@ -196,6 +207,32 @@ class HipAccelerator:
         rgb = np.zeros((len(dirs), 3), np.float32)
         self._check(self._L.sirius_hip_sample_starfield(self._h, side, dirs.ctypes.data, len(dirs), rgb.ctypes.data))
         return rgb
+
+    # ---- ray bundles ----
+    def set_ray_bundle(self, enabled=True, initial_size=None, min_bundle_size=None, max_ellipse_ratio=None):
+        """Ray-bundle parameters of later launches (LaunchParams::rayBundle); None keeps the default."""
+        b = ray_bundle_default()
+        b.enabled = int(enabled)
+        for k, v in (("initial_size", initial_size), ("min_bundle_size", min_bundle_size),
+                     ("max_ellipse_ratio", max_ellipse_ratio)):
+            if v is not None:
+                setattr(b, k, v)
+        self._check(self._L.sirius_hip_set_ray_bundle(self._h, C.byref(b)))
+        return b
+
+    def trace_bundle(self, cams, metrics, opts):
+        """Every pixel's ray bundle: dict of cls, flags, jac (4, n) and beam (3, n)."""
+        cams, metrics = list(cams), list(metrics)
+        ca = (abi.Camera * len(cams))(*cams)
+        ma = (abi.Metric * len(metrics))(*metrics)
+        n = sum(c.width * c.height for c in cams)
+        out = {"cls": np.zeros(n, np.uint8), "flags": np.zeros(n, np.uint8),
+               "jac": np.zeros((4, n), np.float32), "beam": np.zeros((3, n), np.float32)}
+        bo = abi.BundleOutput(cls=out["cls"].ctypes.data, flags=out["flags"].ctypes.data,
+                              jac=out["jac"].ctypes.data, beam=out["beam"].ctypes.data, on_device=0)
+        self._check(self._L.sirius_hip_trace_bundle(self._h, ca, len(cams), ma, len(metrics), C.byref(opts),
+                                                    C.byref(bo)))
+        return out
 
     # ---- image writers and the ray result dump ----
     def _image_args(self, fmt, rgba, exposure, zlib_level, threads):

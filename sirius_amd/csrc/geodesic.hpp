@@ -577,10 +577,11 @@ SIRIUS_HD __forceinline__ T locate_event_dense(const Spacetime<T>& s, int kind, 
 // ------------------------------------------------------------------------------------------
 // Camera and initial state (CMBS001A.h:120-185, GTRC001A.cpp:15-106)
 // ------------------------------------------------------------------------------------------
+// Direction through the image point (u, v) in pixel units (a pixel's centre is (px + 1/2, py + 1/2)).
 template <class T>
-SIRIUS_HD __forceinline__ void camera_dir(const CamConst& c, int px, int py, T nsph[3]) {
-    T sx = (T(2) * (T(px) + T(0.5)) / T(c.width) - T(1)) * T(c.aspect);
-    T sy = T(1) - T(2) * (T(py) + T(0.5)) / T(c.height);
+SIRIUS_HD __forceinline__ void camera_dir_uv(const CamConst& c, T u, T v, T nsph[3]) {
+    T sx = (T(2) * u / T(c.width) - T(1)) * T(c.aspect);
+    T sy = T(1) - T(2) * v / T(c.height);
     T dx = sx * T(c.tan_half), dy = sy * T(c.tan_half), dz = T(-1);
     T cr = T(c.croll), sr = T(c.sroll), cp = T(c.cpitch), sp = T(c.spitch), cy = T(c.cyaw), sy_ = T(c.syaw);
     T rx = dx * cr - dy * sr, ry = dx * sr + dy * cr;
@@ -590,6 +591,11 @@ SIRIUS_HD __forceinline__ void camera_dir(const CamConst& c, int px, int py, T n
     nsph[0] = rz2 * il;
     nsph[1] = -ry2 * il;
     nsph[2] = rx2 * il;
+}
+
+template <class T>
+SIRIUS_HD __forceinline__ void camera_dir(const CamConst& c, int px, int py, T nsph[3]) {
+    camera_dir_uv<T>(c, T(px) + T(0.5), T(py) + T(0.5), nsph);
 }
 
 // Future-directed k^0 of MTTN001A.cpp:171-237 (same root choice and fallbacks). With
@@ -630,9 +636,7 @@ SIRIUS_HD __forceinline__ void camera_cart_dir(const CamConst& c, int px, int py
 }
 
 template <class T>
-SIRIUS_HD __forceinline__ void init_state(const CamConst& c, const Spacetime<T>& s, int px, int py, T x[4], T p[4]) {
-    T ns[3];
-    camera_dir<T>(c, px, py, ns);
+SIRIUS_HD __forceinline__ void init_state_ns(const CamConst& c, const Spacetime<T>& s, const T ns[3], T x[4], T p[4]) {
     T k[4], g[4][4];
     if (c.family == 1) {
         T nr, nt;
@@ -656,6 +660,21 @@ SIRIUS_HD __forceinline__ void init_state(const CamConst& c, const Spacetime<T>&
         for (int nu = 0; nu < 4; ++nu) acc += g[mu][nu] * k[nu];
         p[mu] = acc;
     }
+}
+
+template <class T>
+SIRIUS_HD __forceinline__ void init_state(const CamConst& c, const Spacetime<T>& s, int px, int py, T x[4], T p[4]) {
+    T ns[3];
+    camera_dir<T>(c, px, py, ns);
+    init_state_ns<T>(c, s, ns, x, p);
+}
+
+// Initial state of the ray through the image point (u, v) (pixel units).
+template <class T>
+SIRIUS_HD __forceinline__ void init_state_uv(const CamConst& c, const Spacetime<T>& s, T u, T v, T x[4], T p[4]) {
+    T ns[3];
+    camera_dir_uv<T>(c, u, v, ns);
+    init_state_ns<T>(c, s, ns, x, p);
 }
 
 // ------------------------------------------------------------------------------------------

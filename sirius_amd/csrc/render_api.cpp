@@ -10,11 +10,14 @@
 
 #include "../../include/sirius_hip.h"
 #include "geodesic.hpp"
+#include "sirius_bundle.hpp"
 #include "sirius_internal.hpp"
 
 namespace sirius {
+int bundle_for_launch(Handle& H, Batch& B, int emission, int n_cam, BundleOut& bo);
 hipError_t launch_star_shade(const DevOut& o, int64_t n, const float* rec0, int n0, float b0, const float* rec1, int n1,
-                             float b1, float* fb, hipStream_t st);
+                             float b1, const float* bjac, const uint8_t* bcls, const uint8_t* bflags, float bsize,
+                             float* fb, hipStream_t st);
 hipError_t launch_shade(const DevOut& o, int64_t n, int width, int height, int n_bins, const uint8_t* bg, int bgw,
                         int bgh, float* fb, hipStream_t st);
 hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float* tabs, int64_t n, float* out,
@@ -158,8 +161,17 @@ int sirius_hip_launch(SiriusHipHandle hh, const SiriusHipCamera* cam, const Siri
     hipError_t e = launch_shade(dout, B.n, c.width, c.height, opts->n_bins, H->bg_dev, H->bg_w, H->bg_h, H->fb_dev,
                                 H->stream);
     if (e != hipSuccess) return H->hip_fail(e, "shade");
+    // with a ray bundle, star light is averaged over each pixel's beam (the bundle pass traces
+    // the frame's rays again in FP64 with their deviation)
+    BundleOut bo{};
+    const bool stars = (H->star_dev[0] && H->star_n[0] > 0) || (H->star_dev[1] && H->star_n[1] > 0);
+    if (H->bundle.enabled && stars && opts->integrator == SIRIUS_INTEGRATOR_DP45) {
+        const int emission = opts->emission == SIRIUS_EMISSION_THIN_DISK ? SIRIUS_EMISSION_THIN_DISK : 0;
+        if ((rc = bundle_for_launch(*H, B, emission, 1, bo))) return rc;
+    }
     e = launch_star_shade(dout, B.n, H->star_dev[0], H->star_n[0], H->star_brightness[0], H->star_dev[1],
-                          H->star_n[1], H->star_brightness[1], H->fb_dev, H->stream);
+                          H->star_n[1], H->star_brightness[1], bo.jac, bo.cls, bo.flags, H->bundle.initial_size,
+                          H->fb_dev, H->stream);
     if (e != hipSuccess) return H->hip_fail(e, "star shade");
     e = hipMemcpyAsync(H->fb_host.data(), H->fb_dev, H->fb_host.size() * sizeof(float), hipMemcpyDeviceToHost,
                        H->stream);

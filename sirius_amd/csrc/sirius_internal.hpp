@@ -87,6 +87,8 @@ struct Handle {
     float* star_dev[2] = {nullptr, nullptr};
     int star_n[2] = {0, 0};
     float star_brightness[2] = {1.f, 1.f};
+    // ray bundle of launch (sirius_hip_set_ray_bundle)
+    SiriusHipRayBundle bundle = {0, 0, 0.28867513f, 1e-6f, 100.0f, 0};
     // image writers: the last encoded file and the pinned host staging of converted payloads
     std::vector<unsigned char> image;
     unsigned char* pinned = nullptr;

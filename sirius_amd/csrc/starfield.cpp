@@ -80,7 +80,7 @@ void star_colour(float T, float& r, float& g, float& b) {
 }
 
 // Device record of a star (SIRIUS_STAR_REC floats): normalised direction, cosine cut, sigma^2,
-// 10^(-0.4 m), colour. The direction is normalised as sampleStarfield does for an observer at
+// 10^(-0.4 m), colour, sine of the cut angle. The direction is normalised as sampleStarfield does for an observer at
 // the origin (direction times distance, then divided by its length), in the same float steps.
 void star_record(const SiriusHipStar& s, const SiriusHipStarfield& c, float* rec) {
     const float dx = s.dir[0] * s.distance_pc - 0.0f;
@@ -107,6 +107,7 @@ void star_record(const SiriusHipStar& s, const SiriusHipStarfield& c, float* rec
     // (it only skips stars; the 1e-6 test itself runs on the device)
     const double amax = 1.01 * (double)sigma * std::sqrt(2.0 * std::log(1e6));
     rec[3] = amax >= M_PI ? -2.0f : (float)std::cos(amax);
+    rec[9] = amax >= M_PI ? 0.0f : (float)std::sin(amax);  // with rec[3], widens the cut by a beam
     rec[4] = sigma * sigma;
     rec[5] = std::pow(10.0f, -0.4f * s.magnitude);  // StarEntry::intensity (PHSF001A.h:77-79)
     star_colour(s.temperature_K, rec[6], rec[7], rec[8]);

@@ -5,6 +5,7 @@
 
 #include "../../include/sirius_hip.h"
 #include "geodesic.hpp"
+#include "sirius_bundle.hpp"
 
 namespace sirius {
 
@@ -293,27 +294,109 @@ __global__ __launch_bounds__(256) void k_star_dirs(const float* __restrict__ rec
 // Star light added to the framebuffer of escaped rays: the sky (or a wormhole's near side) from
 // sphere 0, a wormhole's far side (SIRIUS_FLAG_FAR_SIDE) from sphere 1; the direction is the
 // ray's escape direction (theta, phi).
+// A pixel's beam on the sky: the footprint covariance c00, c01, c11 (rad^2) in the frame
+// (e_theta, e_phi) at the escape direction, and the cut's widening (cos, sin of its extent).
+struct Beam {
+    float eth[3], eph[3];
+    float c00, c01, c11;
+    float cb, sb;
+};
+
+// Star light averaged over a beam: the reference's point spread exp(-a^2 / (2 sigma^2)) (peak 1)
+// convolved with the beam's Gaussian footprint, evaluated in the tangent plane at the pixel's
+// direction: exp(-d^T S^-1 d / 2) sigma^2 / sqrt(det S), S = C + sigma^2 I. With a vanishing
+// footprint it is the point-sampled light; summed over pixels it keeps each star's integrated
+// light, which point sampling catches only where a pixel centre falls inside the spread.
+__device__ __forceinline__ void star_light_beam(const float* tile, int cnt, float brightness, float vx, float vy,
+                                                float vz, const Beam& bm, float& r, float& g, float& b) {
+    for (int j = 0; j < cnt; ++j) {
+        const float* s = tile + j * SIRIUS_STAR_REC;
+        const float c = vx * s[0] + vy * s[1] + vz * s[2];
+        // within the star's cut angle plus the beam's extent: cos(A + B) = cos A cos B - sin A sin B
+        if (c < s[3] * bm.cb - s[9] * bm.sb) continue;
+        const float d0 = s[0] * bm.eth[0] + s[1] * bm.eth[1] + s[2] * bm.eth[2];
+        const float d1 = s[0] * bm.eph[0] + s[1] * bm.eph[1] + s[2] * bm.eph[2];
+        const float s00 = bm.c00 + s[4], s11 = bm.c11 + s[4], s01 = bm.c01;
+        const float det = s00 * s11 - s01 * s01;
+        const float q = (s11 * d0 * d0 - 2.0f * s01 * d0 * d1 + s00 * d1 * d1) / det;
+        const float shape = expf(-0.5f * q);
+        if (shape < 1e-6f) continue;
+        const float inten = s[5] * shape * (s[4] / sqrtf(det)) * brightness;
+        r += s[6] * inten;
+        g += s[7] * inten;
+        b += s[8] * inten;
+    }
+}
+
+// One walk over a catalogue's tiles; each lane point-samples or, with `beam`, averages over its beam.
+__device__ __forceinline__ void star_pass_mixed(float* tile, const float* __restrict__ rec, int n_star,
+                                                float brightness, bool want, bool beam, float vx, float vy, float vz,
+                                                const Beam& bm, float& r, float& g, float& b) {
+    for (int base = 0; base < n_star; base += STAR_TILE) {
+        int cnt = min(STAR_TILE, n_star - base);
+        __syncthreads();
+        for (int k = threadIdx.x; k < cnt * SIRIUS_STAR_REC; k += blockDim.x)
+            tile[k] = rec[(int64_t)base * SIRIUS_STAR_REC + k];
+        __syncthreads();
+        if (want) {
+            if (beam) star_light_beam(tile, cnt, brightness, vx, vy, vz, bm, r, g, b);
+            else star_light(tile, cnt, brightness, vx, vy, vz, r, g, b);
+        }
+    }
+}
+
+// Star light of escaped pixels. With a bundle (bjac non-null), a pixel whose beam escaped and
+// was not terminated gets the light averaged over its beam (radius bsize pixels at the camera);
+// the others, and every pixel without a bundle, are point-sampled.
 __global__ __launch_bounds__(256) void k_star_shade(DevOut o, int64_t n, const float* __restrict__ rec0, int n0,
                                                     float b0, const float* __restrict__ rec1, int n1, float b1,
+                                                    const float* __restrict__ bjac, const uint8_t* __restrict__ bcls,
+                                                    const uint8_t* __restrict__ bflags, float bsize,
                                                     float* __restrict__ fb) {
     __shared__ float tile[STAR_TILE * SIRIUS_STAR_REC];
     int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool esc = id < n && o.cls[id] == SIRIUS_RAY_ESCAPED;
     bool far = esc && (o.flags[id] & SIRIUS_FLAG_FAR_SIDE);
     float vx = 0.f, vy = 0.f, vz = 0.f;
+    Beam bm;
+    bool beam = false;
     if (esc) {
         // the unit escape direction in FP64, each component rounded once to float: a host that
         // forms it the same way gets the same bits (to a double rounding near a float midpoint),
         // and next to a point-like star one ulp of the direction decides whether it lights
         const double th = o.theta[id], ph = o.phi[id];
-        const double st = sin(th);
-        vx = (float)(st * cos(ph));
-        vy = (float)(st * sin(ph));
-        vz = (float)cos(th);
+        const double st = sin(th), ct = cos(th), sp = sin(ph), cp = cos(ph);
+        vx = (float)(st * cp);
+        vy = (float)(st * sp);
+        vz = (float)ct;
+        if (bjac && bcls[id] == SIRIUS_RAY_ESCAPED && !(bflags[id] & SIRIUS_BUNDLE_TERMINATED)) {
+            // footprint covariance s^2 J J^T with J's rows d theta/d(u, v), sin(theta) d phi/d(u, v)
+            const double j00 = bjac[id], j01 = bjac[n + id];
+            const double j10 = st * bjac[2 * n + id], j11 = st * bjac[3 * n + id];
+            const double s2 = (double)bsize * bsize;
+            const double c00 = s2 * (j00 * j00 + j01 * j01), c11 = s2 * (j10 * j10 + j11 * j11);
+            const double c01 = s2 * (j00 * j10 + j01 * j11);
+            const double tr = c00 + c11, det = c00 * c11 - c01 * c01;
+            const double lmax = 0.5 * (tr + sqrt(fmax(0.0, tr * tr - 4.0 * det)));
+            const double ext = 1.01 * sqrt(2.0 * log(1e6)) * sqrt(lmax);
+            if (ext < 0.5) {  // a tangent-plane beam: small against a radian
+                beam = true;
+                bm.eth[0] = (float)(ct * cp); bm.eth[1] = (float)(ct * sp); bm.eth[2] = (float)(-st);
+                bm.eph[0] = (float)(-sp); bm.eph[1] = (float)cp; bm.eph[2] = 0.f;
+                bm.c00 = (float)c00; bm.c01 = (float)c01; bm.c11 = (float)c11;
+                bm.cb = (float)cos(ext);
+                bm.sb = (float)sin(ext);
+            }
+        }
     }
     float r = 0.f, g = 0.f, b = 0.f;
-    star_pass(tile, rec0, n0, b0, esc && !far, vx, vy, vz, r, g, b);
-    star_pass(tile, rec1, n1, b1, far, vx, vy, vz, r, g, b);
+    if (bjac) {
+        star_pass_mixed(tile, rec0, n0, b0, esc && !far, beam, vx, vy, vz, bm, r, g, b);
+        star_pass_mixed(tile, rec1, n1, b1, far, beam, vx, vy, vz, bm, r, g, b);
+    } else {
+        star_pass(tile, rec0, n0, b0, esc && !far, vx, vy, vz, r, g, b);
+        star_pass(tile, rec1, n1, b1, far, vx, vy, vz, r, g, b);
+    }
     if (esc) {
         fb[4 * id + 0] += r;
         fb[4 * id + 1] += g;
@@ -330,10 +413,11 @@ hipError_t launch_star_dirs(const float* rec, int n_star, float brightness, cons
 }
 
 hipError_t launch_star_shade(const DevOut& o, int64_t n, const float* rec0, int n0, float b0, const float* rec1, int n1,
-                             float b1, float* fb, hipStream_t st) {
+                             float b1, const float* bjac, const uint8_t* bcls, const uint8_t* bflags, float bsize,
+                             float* fb, hipStream_t st) {
     if (n <= 0 || (n0 <= 0 && n1 <= 0)) return hipSuccess;
     hipLaunchKernelGGL(k_star_shade, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, o, n, rec0, n0, b0, rec1, n1,
-                       b1, fb);
+                       b1, bjac, bcls, bflags, bsize, fb);
     return hipGetLastError();
 }
 
