@@ -143,6 +143,45 @@ def other_workloads(acc, steps, warmup):
         o.emission = abi.EMISSION_NONE
         out[name] = measure([cam], [metric], o, steps, warmup)
     out["image_writers_1080p"] = image_writers(acc, steps)
+    out["ray_bundle_1080p"] = ray_bundle(acc, steps)
+    return out
+
+
+def ray_bundle(acc, steps):
+    """The 1080p Kerr frame's ray bundles (geodesic deviation in FP64, results left in device
+    memory), and launch with the default star catalogue (100000 candidates, magnitude limit 12)
+    point-sampled and averaged over each pixel's beam."""
+    from sirius_amd import abi
+    from sirius_amd.accelerator import default_opts, starfield_default, starfield_generate
+    cam, metric = workload()
+    o = default_opts(abi.PRECISION_FP32_ESCALATE)
+    acc.trace_bundle([cam], [metric], o, resident=True)  # warm
+    ts = []
+    for _ in range(max(2, steps // 2)):
+        t0 = time.perf_counter()
+        acc.trace_bundle([cam], [metric], o, resident=True)
+        ts.append(time.perf_counter() - t0)
+    ms = float(np.mean(ts)) * 1e3
+    out = {"bundle_ms": ms, "bundle_mrays_per_s": W * H / ms / 1e3,
+           "tolerances": "the FP32 path's (rtol 1e-6, far-field cap 0.25 r), stepped in FP64"}
+    cfg = starfield_default()
+    stars, n = starfield_generate(cfg)
+    acc.initialise(W, H)
+    acc.set_starfield(0, cfg, stars, n)
+    try:
+        for name, on in (("launch_point_ms", False), ("launch_beam_ms", True)):
+            acc.set_ray_bundle(on)
+            acc.launch(cam, metric, o)
+            ts = []
+            for _ in range(max(2, steps // 2)):
+                t0 = time.perf_counter()
+                acc.launch(cam, metric, o)
+                ts.append(time.perf_counter() - t0)
+            out[name] = float(np.mean(ts)) * 1e3
+        out["stars"] = n
+    finally:
+        acc.set_ray_bundle(False)
+        acc.set_starfield(0, None, None, 0)
     return out
 
 

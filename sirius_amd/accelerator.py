@@ -220,11 +220,17 @@ class HipAccelerator:
         self._check(self._L.sirius_hip_set_ray_bundle(self._h, C.byref(b)))
         return b
 
-    def trace_bundle(self, cams, metrics, opts):
-        """Every pixel's ray bundle: dict of cls, flags, jac (4, n) and beam (3, n)."""
+    def trace_bundle(self, cams, metrics, opts, resident=False):
+        """Every pixel's ray bundle: dict of cls, flags, jac (4, n) and beam (3, n); with resident
+        the results stay in library device memory (nothing is copied, None is returned)."""
         cams, metrics = list(cams), list(metrics)
         ca = (abi.Camera * len(cams))(*cams)
         ma = (abi.Metric * len(metrics))(*metrics)
+        if resident:
+            bo = abi.BundleOutput(on_device=1)
+            self._check(self._L.sirius_hip_trace_bundle(self._h, ca, len(cams), ma, len(metrics), C.byref(opts),
+                                                        C.byref(bo)))
+            return None
         n = sum(c.width * c.height for c in cams)
         out = {"cls": np.zeros(n, np.uint8), "flags": np.zeros(n, np.uint8),
                "jac": np.zeros((4, n), np.float32), "beam": np.zeros((3, n), np.float32)}
