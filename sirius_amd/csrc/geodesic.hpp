@@ -215,51 +215,64 @@ SIRIUS_HD __forceinline__ void metric_cov(const Spacetime<T>& s, const T x[4], T
 // ------------------------------------------------------------------------------------------
 template <class T>
 SIRIUS_HD __forceinline__ T rhs_ks(const Spacetime<T>& s, const T x[4], const T p[4], T k[4], T dp[3]) {
-    // derivatives along the coordinate seeds x, y, z; products with a seed go through dmul_seed /
-    // dadd_seed so no zero tangent is multiplied
+    // Hamilton's equations for H = (eta^{mu nu} - H l^mu l^nu) p_mu p_nu / 2 with the position
+    // gradient in closed form. r depends on (x, y, z) through r^4 - (R^2 - a^2) r^2 - a^2 z^2 = 0:
+    //   dr/dx = x r / s, dr/dy = y r / s, dr/dz = z (r^2 + a^2) / (r s),
+    //   s = sqrt((R^2 - a^2)^2 + 4 a^2 z^2) = 2 r^2 - R^2 + a^2;
+    // then dH_i = H_r dr_i + H_z [i = z] and sum_j k_j d_i l_j = A_i + B dr_i.
     const T X = x[1], Y = x[2], Z = x[3];
     const T R2v = X * X + Y * Y + Z * Z;
-    D3<T> w = {R2v - s.a2, T(2) * X, T(2) * Y, T(2) * Z};   // R^2 - a^2
-    D3<T> disc = w * w;                                      // (R^2 - a^2)^2 + 4 a^2 z^2
-    disc.v = tfma(T(4) * s.a2, Z * Z, disc.v);
-    disc.g2 = tfma(T(8) * s.a2, Z, disc.g2);
-    disc.v = fmax(disc.v, T(1e-20));  // guards only the ring singularity, deep inside the horizon
-    D3<T> r2 = (w + dsqrt(disc)) * T(0.5);
-    r2.v = fmax(r2.v, T(1e-10));
-    D3<T> r = dsqrt(r2);
-    D3<T> iden = dinv(r2 + s.a2);
-    D3<T> l1 = dadd_seed<1>(dmul_seed<0>(r, X), s.a, Y) * iden;   // (r x + a y) / (r^2 + a^2)
-    D3<T> l2 = dadd_seed<0>(dmul_seed<1>(r, Y), -s.a, X) * iden;  // (r y - a x) / (r^2 + a^2)
-    D3<T> l3 = dmul_seed<2>(dinv(r), Z);                          // z / r
-    D3<T> num = (r * (T(2) * s.M) - s.Q2) * r2;
-    D3<T> den = r2 * r2;                                          // r^4 + a^2 z^2
-    den.v = tfma(s.a2 * Z, Z, den.v);
-    den.g2 = tfma(T(2) * s.a2, Z, den.g2);
-    D3<T> H = num * dinv(den);
+    const T w = R2v - s.a2;                           // R^2 - a^2
+    T disc = tfma(T(4) * s.a2, Z * Z, w * w);         // (R^2 - a^2)^2 + 4 a^2 z^2
+    disc = fmax(disc, T(1e-20));  // guards only the ring singularity, deep inside the horizon
+    const T sq = sqrt(disc);
+    T r2 = (w + sq) * T(0.5);
+    r2 = fmax(r2, T(1e-10));
+    const T r = sqrt(r2);
+    const T ir = T(1) / r;
+    const T rho = r2 + s.a2;
+    const T irho = T(1) / rho;
+    const T l1 = tfma(r, X, s.a * Y) * irho;          // (r x + a y) / (r^2 + a^2)
+    const T l2 = tfma(r, Y, -s.a * X) * irho;         // (r y - a x) / (r^2 + a^2)
+    const T l3 = Z * ir;                              // z / r
+    const T q = T(2) * s.M * r - s.Q2;                // 2 M r - Q^2
+    const T Dn = tfma(s.a2 * Z, Z, r2 * r2);          // r^4 + a^2 z^2
+    const T iDn = T(1) / Dn;
+    const T H = q * r2 * iDn;
     // k = (eta - H l l) p, l^mu = (-1, l1, l2, l3)
-    T lp = -p[0] + l1.v * p[1] + l2.v * p[2] + l3.v * p[3];
-    T Hl = H.v * lp;
+    T lp = -p[0] + l1 * p[1] + l2 * p[2] + l3 * p[3];
+    T Hl = H * lp;
     k[0] = -p[0] + Hl;
-    k[1] = p[1] - Hl * l1.v;
-    k[2] = p[2] - Hl * l2.v;
-    k[3] = p[3] - Hl * l3.v;
+    k[1] = p[1] - Hl * l1;
+    k[2] = p[2] - Hl * l2;
+    k[3] = p[3] - Hl * l3;
     T lam3 = T(0);
     if (s.Lambda != T(0)) {
         // g = g0 + c e0 e0^T with c = -Lambda R^2 / 3; g^-1 by Sherman-Morrison.
         T c = -s.Lambda * R2v / T(3);
-        T u0 = -T(1) - H.v;
+        T u0 = -T(1) - H;
         T fac = c * k[0] / (T(1) + c * u0);
         k[0] -= fac * u0;
-        k[1] -= fac * H.v * l1.v;
-        k[2] -= fac * H.v * l2.v;
-        k[3] -= fac * H.v * l3.v;
+        k[1] -= fac * H * l1;
+        k[2] -= fac * H * l2;
+        k[3] -= fac * H * l3;
         lam3 = s.Lambda / T(3);
     }
-    T lk = k[0] + l1.v * k[1] + l2.v * k[2] + l3.v * k[3];
-    T hl = T(0.5) * lk * lk, Hlk = H.v * lk, k00 = k[0] * k[0];
-    dp[0] = H.g0 * hl + Hlk * (l1.g0 * k[1] + l2.g0 * k[2] + l3.g0 * k[3]) - lam3 * x[1] * k00;
-    dp[1] = H.g1 * hl + Hlk * (l1.g1 * k[1] + l2.g1 * k[2] + l3.g1 * k[3]) - lam3 * x[2] * k00;
-    dp[2] = H.g2 * hl + Hlk * (l1.g2 * k[1] + l2.g2 * k[2] + l3.g2 * k[3]) - lam3 * x[3] * k00;
+    T lk = k[0] + l1 * k[1] + l2 * k[2] + l3 * k[3];
+    T hl = T(0.5) * lk * lk, Hlk = H * lk, k00 = k[0] * k[0];
+    // gradient of r
+    const T rs = r / sq;
+    const T drx = X * rs, dry = Y * rs, drz = Z * rho * ir / sq;
+    // H_r = (N_r - H Dn_r) / Dn with N = q r^2, N_r = r (6 M r - 2 Q^2), Dn_r = 4 r^3;
+    // H_z = -2 a^2 z H / Dn
+    const T Hr = (r * (T(6) * s.M * r - T(2) * s.Q2) - H * T(4) * r2 * r) * iDn;
+    const T Hz = -T(2) * s.a2 * Z * H * iDn;
+    const T A1 = (k[1] * r - k[2] * s.a) * irho, A2 = (k[1] * s.a + k[2] * r) * irho, A3 = k[3] * ir;
+    const T B = (k[1] * X + k[2] * Y - T(2) * r * (k[1] * l1 + k[2] * l2)) * irho - k[3] * l3 * ir;
+    const T Cr = hl * Hr + Hlk * B;
+    dp[0] = Cr * drx + Hlk * A1 - lam3 * X * k00;
+    dp[1] = Cr * dry + Hlk * A2 - lam3 * Y * k00;
+    dp[2] = Cr * drz + hl * Hz + Hlk * A3 - lam3 * Z * k00;
     return p[0] * k[0] + p[1] * k[1] + p[2] * k[2] + p[3] * k[3];
 }
 
