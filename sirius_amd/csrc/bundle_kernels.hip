@@ -8,15 +8,15 @@
 // the geodesic deviation equation is the tangent (variational) equation of
 // (dx/dlam, dp/dlam) = (dH/dp, -dH/dx), so the two deviation vectors are the derivatives of
 // the state (x^mu, p_mu) with respect to the image coordinates (u, v) of the ray's pixel. They
-// start as the derivative of the camera's initial state and are carried through every
-// accepted DP45 step as the derivative of that step's map (the step with its length held),
-// taken by central differences of the step at the perturbed states y +- eps t; the step
-// length is chosen on the central ray alone. At the end the derivative of the outcome follows:
-// of the escape direction (theta, phi) for an escaped ray (rad per pixel), or of the
-// equatorial hit point (X, Y) for a ray ending on the thin disk (M per pixel), the deviation
-// there projected along the ray onto the plane.
+// are carried by dual numbers: the camera, the initial state and every DP45 step run on
+// Dual<double, 1> values seeded with d/du, and again with d/dv, so each accepted step advances the
+// deviation by the exact derivative of that step's map (its length held; the length is chosen
+// on the central ray alone). At the end the derivative of the outcome follows: of the escape
+// direction (theta, phi) for an escaped ray (rad per pixel), or of the equatorial hit point
+// (X, Y) for a ray ending on the thin disk (M per pixel), the deviation there projected along
+// the ray onto the plane.
 //
-// One thread per ray, FP64. An accepted step costs the central step and four perturbed steps.
+// One thread per ray, FP64 values and tangents.
 #include <hip/hip_runtime.h>
 
 #include "../../include/sirius_hip.h"
@@ -64,8 +64,9 @@ SIRIUS_HD __forceinline__ void escape_angles(const CamConst& c, const Spacetime<
     ph = atan2(dir[1], dir[0]);
 }
 
-// Difference scale for a deviation t at state y: a perturbation of 1e-4 pixel, or less where
-// the deviation has grown, so that eps |t| stays below 1e-6 of the state's scale.
+// Difference scale for the escape direction along a deviation t at state y: a perturbation of
+// 1e-4 pixel, or less where the deviation has grown, so that eps |t| stays below 1e-6 of the
+// state's scale.
 SIRIUS_HD __forceinline__ double fd_eps(const double x[4], const double p[4], const double t[8]) {
     double ys = 1.0, tn = 0.0;
     for (int i = 1; i < 4; ++i) ys = fmax(ys, fabs(x[i]));
@@ -73,32 +74,6 @@ SIRIUS_HD __forceinline__ double fd_eps(const double x[4], const double p[4], co
     double eps = 1e-4;
     if (tn * eps > 1e-6 * ys) eps = 1e-6 * ys / tn;
     return eps;
-}
-
-// The deviations carried through one DP45 step of length h from (x, p): t <- dPhi_h . t.
-template <int FAM>
-SIRIUS_HD __forceinline__ void step_deviation(const Spacetime<double>& s, const double x[4], const double p[4],
-                                              double h, double t[2][8]) {
-    for (int a = 0; a < 2; ++a) {
-        const double eps = fd_eps(x, p, t[a]);
-        double yp[2][8];
-        for (int sg = 0; sg < 2; ++sg) {
-            const double e = sg ? -eps : eps;
-            double xs[4], ps[4], x1[4], p1[4], err[4];
-            for (int i = 0; i < 4; ++i) {
-                xs[i] = x[i] + e * t[a][i];
-                ps[i] = p[i] + e * t[a][4 + i];
-            }
-            Der<double> d0, d1;
-            eval<double, FAM>(s, xs, ps, d0);
-            dp45_step<double, FAM, true>(s, xs, ps, d0, h, x1, p1, d1, err, false);
-            for (int i = 0; i < 4; ++i) {
-                yp[sg][i] = x1[i];
-                yp[sg][4 + i] = p1[i];
-            }
-        }
-        for (int i = 0; i < 8; ++i) t[a][i] = (yp[0][i] - yp[1][i]) / (2.0 * eps);
-    }
 }
 
 // Footprint of a 2x2 Jacobian m (rows: the two outcome coordinates in an orthonormal frame;
@@ -118,31 +93,43 @@ SIRIUS_HD __forceinline__ void ellipse_of(const double m[2][2], double size, flo
 }  // namespace
 
 template <int FAM, int EMIT>
-__global__ __launch_bounds__(64) void k_trace_bundle(const CamConst* __restrict__ cams, int n_cam, TraceParams tp,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_trace_bundle(const CamConst* __restrict__ cams, int n_cam, TraceParams tp,
                                                      BundleParams bp, BundleOut bo) {
     const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= bo.n) return;
     const CamConst& c = cams[find_cam(cams, n_cam, id)];
     const int64_t pix = id - c.offset;
     const double u0 = (double)(pix % c.width) + 0.5, v0 = (double)(pix / c.width) + 0.5;
+    // one tangent at a time: a Dual<double, 1> pass per image axis keeps the step's seven stage
+    // derivatives in registers (two tangents at once spill); the value parts of both passes are
+    // the same arithmetic, so the central ray is the same in each
+    using D = Dual<double, 1>;
     const Spacetime<double> s = make_spacetime<double>(c);
+    const Spacetime<D> sD = make_spacetime<D>(c);
     double x[4], p[4], t[2][8];
-    init_state_uv<double>(c, s, u0, v0, x, p);
-    {
-        // the camera's initial state differentiated in u and v (central, 1e-3 pixel)
-        const double e0 = 1e-3;
-        for (int a = 0; a < 2; ++a) {
-            double xp[4], pp[4], xm[4], pm[4];
-            init_state_uv<double>(c, s, u0 + (a == 0 ? e0 : 0.0), v0 + (a == 1 ? e0 : 0.0), xp, pp);
-            init_state_uv<double>(c, s, u0 - (a == 0 ? e0 : 0.0), v0 - (a == 1 ? e0 : 0.0), xm, pm);
-            for (int i = 0; i < 4; ++i) {
-                t[a][i] = (xp[i] - xm[i]) / (2.0 * e0);
-                t[a][4 + i] = (pp[i] - pm[i]) / (2.0 * e0);
-            }
+    Der<D> dA[2];  // FSAL derivative of each pass at the step's start
+    for (int a = 0; a < 2; ++a) {
+        D X[4], P[4];
+        init_state_uv<D>(c, sD, a == 0 ? D::seed(u0, 0) : D(u0), a == 1 ? D::seed(v0, 0) : D(v0), X, P);
+        eval<D, FAM>(sD, X, P, dA[a]);
+        for (int i = 0; i < 4; ++i) {
+            x[i] = X[i].v;
+            p[i] = P[i].v;
+            t[a][i] = X[i].d[0];
+            t[a][4 + i] = P[i].d[0];
         }
     }
-    Der<double> d;
-    eval<double, FAM>(s, x, p, d);
+    // the pass of axis a from the current state: state with that tangent, one DP45 step of h
+    auto pass = [&](int a, double hh, D X1[4], D P1[4], Der<D>& d1, D err[4], bool want_err) {
+        D X[4], P[4];
+        for (int i = 0; i < 4; ++i) {
+            X[i] = D(x[i]);
+            X[i].d[0] = t[a][i];
+            P[i] = D(p[i]);
+            P[i].d[0] = t[a][4 + i];
+        }
+        dp45_step<D, FAM, true>(sD, X, P, dA[a], D(hh), X1, P1, d1, err, want_err);
+    };
     double h = tp.h_init;
     if (tp.far_slope > 0) h = fmax(h, fmax(tp.h_max, event_radius(s, x) * tp.far_slope));
     int cls = SIRIUS_RAY_MAX_STEPS;
@@ -151,13 +138,13 @@ __global__ __launch_bounds__(64) void k_trace_bundle(const CamConst* __restrict_
     while (attempts < (uint32_t)tp.max_steps) {
         double hmax = tp.h_max;
         if (tp.far_slope > 0) hmax = fmax(hmax, event_radius(s, x) * tp.far_slope);
-        double x1[4], p1[4], err[4];
-        Der<double> d1;
-        dp45_step<double, FAM, true>(s, x, p, d, h, x1, p1, d1, err, true);
+        D X1[4], P1[4], err[4];
+        Der<D> d1;
+        pass(0, h, X1, P1, d1, err, true);
         ++attempts;
         double e = 0;
         for (int i = 0; i < 4; ++i) {
-            const double q = err[i] / (tp.atol + tp.rtol * fabs(x1[i]));
+            const double q = err[i].v / (tp.atol + tp.rtol * fabs(X1[i].v));
             e += q * q;
         }
         e = sqrt(0.25 * e);
@@ -166,31 +153,34 @@ __global__ __launch_bounds__(64) void k_trace_bundle(const CamConst* __restrict_
             if (!sfinite(e) || h <= tp.h_min) { cls = SIRIUS_RAY_FAILED; break; }
             continue;
         }
+        double x1[4], p1[4], k0[4], k1[4];
         bool bad = false;
-        for (int i = 0; i < 4; ++i) bad |= !sfinite(x1[i]) || !sfinite(p1[i]);
+        for (int i = 0; i < 4; ++i) {
+            x1[i] = X1[i].v;
+            p1[i] = P1[i].v;
+            k0[i] = dA[0].k[i].v;
+            k1[i] = d1.k[i].v;
+            bad |= !sfinite(x1[i]) || !sfinite(p1[i]);
+        }
         if (bad) { cls = SIRIUS_RAY_FAILED; break; }
         // thin disk: first crossing inside [r_in, r_out], landed with a DP45 step to the plane
         if constexpr (FAM == 0 && EMIT == SIRIUS_EMISSION_THIN_DISK) {
             if (x[3] * x1[3] < 0) {
-                const double th = locate_event(s, EV_Z, 0.0, x, d.k, x1, d1.k, h);
+                const double th = locate_event(s, EV_Z, 0.0, x, k0, x1, k1, h);
                 double xc[4];
-                hermite(x, d.k, x1, d1.k, h, th, xc);
+                hermite(x, k0, x1, k1, h, th, xc);
                 const double rc = kerr_r(s, xc[1], xc[2], xc[3]);
                 if (rc >= c.r_in && rc <= c.r_out) {
-                    const double hl = th * h;
-                    double xl[4], pl[4], el[4];
-                    Der<double> dl;
-                    dp45_step<double, FAM, true>(s, x, p, d, hl, xl, pl, dl, el, false);
-                    step_deviation<FAM>(s, x, p, hl, t);
                     // deviation of the hit point: along the ray onto the plane z = 0
-                    const double kz = dl.k[3];
-                    double m[2][2];
                     for (int a = 0; a < 2; ++a) {
-                        const double dz = fabs(kz) > 1e-300 ? t[a][3] / kz : 0.0;
-                        m[0][a] = t[a][1] - dz * dl.k[1];
-                        m[1][a] = t[a][2] - dz * dl.k[2];
+                        D XL[4], PL[4], el[4];
+                        Der<D> dl;
+                        pass(a, th * h, XL, PL, dl, el, false);
+                        const double kz = dl.k[3].v;
+                        const double dz = fabs(kz) > 1e-300 ? XL[3].d[0] / kz : 0.0;
+                        jac[a] = XL[1].d[0] - dz * dl.k[1].v;
+                        jac[2 + a] = XL[2].d[0] - dz * dl.k[2].v;
                     }
-                    jac[0] = m[0][0]; jac[1] = m[0][1]; jac[2] = m[1][0]; jac[3] = m[1][1];
                     cls = SIRIUS_RAY_DISK;
                     break;
                 }
@@ -198,24 +188,37 @@ __global__ __launch_bounds__(64) void k_trace_bundle(const CamConst* __restrict_
         }
         const double r1 = event_radius(s, x1);
         if (FAM == 0 && r1 < c.r_cap) { cls = SIRIUS_RAY_CAPTURED; break; }
-        step_deviation<FAM>(s, x, p, h, t);
+        // accepted: the second axis's pass, then both tangents and FSAL derivatives advance
+        {
+            D X2[4], P2[4], e2[4];
+            Der<D> d2;
+            pass(1, h, X2, P2, d2, e2, false);
+            for (int i = 0; i < 4; ++i) {
+                t[0][i] = X1[i].d[0];
+                t[0][4 + i] = P1[i].d[0];
+                t[1][i] = X2[i].d[0];
+                t[1][4 + i] = P2[i].d[0];
+            }
+            dA[0] = d1;
+            dA[1] = d2;
+        }
         for (int i = 0; i < 4; ++i) { x[i] = x1[i]; p[i] = p1[i]; }
-        d = d1;
         if (r1 > tp.r_escape) {
-            const double radial = (FAM == 1) ? x1[1] * d1.k[1] : x1[1] * d1.k[1] + x1[2] * d1.k[2] + x1[3] * d1.k[3];
+            const double radial = (FAM == 1) ? x1[1] * k1[1] : x1[1] * k1[1] + x1[2] * k1[2] + x1[3] * k1[3];
             if (radial > 0) {
-                // the escape direction differentiated along the two deviations
+                // the escape direction (a closed-form function of the final state and, for the
+                // wormhole, of the image point) differentiated along the two deviations
                 for (int a = 0; a < 2; ++a) {
                     const double eps = fd_eps(x, p, t[a]);
                     double ang[2][2];
                     for (int sg = 0; sg < 2; ++sg) {
-                        const double e2 = sg ? -eps : eps;
+                        const double e3 = sg ? -eps : eps;
                         double xs[4], ps[4];
                         for (int i = 0; i < 4; ++i) {
-                            xs[i] = x[i] + e2 * t[a][i];
-                            ps[i] = p[i] + e2 * t[a][4 + i];
+                            xs[i] = x[i] + e3 * t[a][i];
+                            ps[i] = p[i] + e3 * t[a][4 + i];
                         }
-                        escape_angles<FAM>(c, s, u0 + (a == 0 ? e2 : 0.0), v0 + (a == 1 ? e2 : 0.0), xs, ps,
+                        escape_angles<FAM>(c, s, u0 + (a == 0 ? e3 : 0.0), v0 + (a == 1 ? e3 : 0.0), xs, ps,
                                            ang[sg][0], ang[sg][1]);
                     }
                     double dph = ang[0][1] - ang[1][1];

@@ -16,6 +16,7 @@
 // closed form eta - H l l (plus a Sherman-Morrison rank-1 term for Lambda), so no 4x4
 // inverse is ever formed on the device.
 #pragma once
+#include <type_traits>
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -102,46 +103,104 @@ struct DevOut {
     int64_t n_rays;
 };
 
-// ------------------------------------------------------------------------------------------
-// Dual numbers: value + gradient w.r.t. (x, y, z)
-// ------------------------------------------------------------------------------------------
-template <class T>
-struct D3 {
-    T v, g0, g1, g2;
-};
 // a * b + c with one rounding in the working precision (fmaf for FP32, also in host builds of
 // this header, where a plain fma call on floats would resolve to the double overload).
 template <class T> SIRIUS_HD __forceinline__ T tfma(T a, T b, T c) {
     if constexpr (sizeof(T) == 4) return fmaf(a, b, c);
     else return fma(a, b, c);
 }
-template <class T> SIRIUS_HD __forceinline__ D3<T> dcst(T v) { return {v, T(0), T(0), T(0)}; }
-template <class T> SIRIUS_HD __forceinline__ D3<T> operator+(D3<T> a, D3<T> b) { return {a.v + b.v, a.g0 + b.g0, a.g1 + b.g1, a.g2 + b.g2}; }
-template <class T> SIRIUS_HD __forceinline__ D3<T> operator-(D3<T> a, D3<T> b) { return {a.v - b.v, a.g0 - b.g0, a.g1 - b.g1, a.g2 - b.g2}; }
-template <class T> SIRIUS_HD __forceinline__ D3<T> operator*(D3<T> a, D3<T> b) {
-    return {a.v * b.v, a.g0 * b.v + a.v * b.g0, a.g1 * b.v + a.v * b.g1, a.g2 * b.v + a.v * b.g2};
+
+// ------------------------------------------------------------------------------------------
+// Dual numbers (forward-mode derivatives, MTDL001A.h's Dual<T> with N tangents): a value and its
+// derivatives along N seed directions, all held in registers. The geodesic code is written over
+// a scalar type T; instantiated with Dual<double, N> it carries the derivatives of every quantity
+// with respect to N parameters through the same arithmetic (the ray bundles differentiate the
+// camera, the initial state and each DP45 step with respect to the image point this way).
+// Comparisons and the clamps fmax / fmin act on values.
+// ------------------------------------------------------------------------------------------
+template <class T, int N>
+struct Dual {
+    T v;
+    T d[N];
+    Dual() = default;
+    template <class S, std::enable_if_t<std::is_arithmetic_v<S>, int> = 0>
+    SIRIUS_HD constexpr Dual(S x) : v(T(x)), d{} {}
+    SIRIUS_HD static Dual seed(T x, int axis) {
+        Dual r(x);
+        r.d[axis] = T(1);
+        return r;
+    }
+    SIRIUS_HD Dual& operator+=(const Dual& b) { *this = *this + b; return *this; }
+    SIRIUS_HD Dual& operator-=(const Dual& b) { *this = *this - b; return *this; }
+    SIRIUS_HD Dual& operator*=(const Dual& b) { *this = *this * b; return *this; }
+    SIRIUS_HD Dual& operator/=(const Dual& b) { *this = *this / b; return *this; }
+};
+template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> operator+(const Dual<T, N>& a, const Dual<T, N>& b) {
+    Dual<T, N> r;
+    r.v = a.v + b.v;
+    for (int i = 0; i < N; ++i) r.d[i] = a.d[i] + b.d[i];
+    return r;
 }
-template <class T> SIRIUS_HD __forceinline__ D3<T> operator*(D3<T> a, T s) { return {a.v * s, a.g0 * s, a.g1 * s, a.g2 * s}; }
-template <class T> SIRIUS_HD __forceinline__ D3<T> operator+(D3<T> a, T s) { return {a.v + s, a.g0, a.g1, a.g2}; }
-template <class T> SIRIUS_HD __forceinline__ D3<T> operator-(D3<T> a, T s) { return {a.v - s, a.g0, a.g1, a.g2}; }
-// a * S and a + c S for a coordinate seed S = (s; unit tangent along AXIS): only the nonzero
-// tangent terms are formed. The compiler cannot drop a product with a zero tangent itself
-// (0 * t is not 0 for an infinite or NaN t), and the seeds' zero tangents would otherwise cost a
-// multiply or an add each in every evaluation.
-template <int AXIS, class T> SIRIUS_HD __forceinline__ D3<T> dmul_seed(D3<T> a, T s) {
-    return {a.v * s, AXIS == 0 ? tfma(a.g0, s, a.v) : a.g0 * s, AXIS == 1 ? tfma(a.g1, s, a.v) : a.g1 * s,
-            AXIS == 2 ? tfma(a.g2, s, a.v) : a.g2 * s};
+template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> operator-(const Dual<T, N>& a, const Dual<T, N>& b) {
+    Dual<T, N> r;
+    r.v = a.v - b.v;
+    for (int i = 0; i < N; ++i) r.d[i] = a.d[i] - b.d[i];
+    return r;
 }
-template <int AXIS, class T> SIRIUS_HD __forceinline__ D3<T> dadd_seed(D3<T> a, T c, T s) {
-    return {tfma(c, s, a.v), AXIS == 0 ? a.g0 + c : a.g0, AXIS == 1 ? a.g1 + c : a.g1, AXIS == 2 ? a.g2 + c : a.g2};
+template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> operator-(const Dual<T, N>& a) {
+    Dual<T, N> r;
+    r.v = -a.v;
+    for (int i = 0; i < N; ++i) r.d[i] = -a.d[i];
+    return r;
 }
-template <class T> SIRIUS_HD __forceinline__ D3<T> dinv(D3<T> a) {
-    T r = T(1) / a.v, r2 = -r * r;
-    return {r, a.g0 * r2, a.g1 * r2, a.g2 * r2};
+template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> operator*(const Dual<T, N>& a, const Dual<T, N>& b) {
+    Dual<T, N> r;
+    r.v = a.v * b.v;
+    for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+    return r;
 }
-template <class T> SIRIUS_HD __forceinline__ D3<T> dsqrt(D3<T> a) {
-    T s = sqrt(a.v), h = T(0.5) / s;
-    return {s, a.g0 * h, a.g1 * h, a.g2 * h};
+template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> operator/(const Dual<T, N>& a, const Dual<T, N>& b) {
+    Dual<T, N> r;
+    const T ib = T(1) / b.v;
+    r.v = a.v * ib;
+    for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) * ib;
+    return r;
+}
+template <class T, int N> SIRIUS_HD __forceinline__ bool operator<(const Dual<T, N>& a, const Dual<T, N>& b) { return a.v < b.v; }
+template <class T, int N> SIRIUS_HD __forceinline__ bool operator>(const Dual<T, N>& a, const Dual<T, N>& b) { return a.v > b.v; }
+template <class T, int N> SIRIUS_HD __forceinline__ bool operator<=(const Dual<T, N>& a, const Dual<T, N>& b) { return a.v <= b.v; }
+template <class T, int N> SIRIUS_HD __forceinline__ bool operator>=(const Dual<T, N>& a, const Dual<T, N>& b) { return a.v >= b.v; }
+template <class T, int N> SIRIUS_HD __forceinline__ bool operator==(const Dual<T, N>& a, const Dual<T, N>& b) { return a.v == b.v; }
+template <class T, int N> SIRIUS_HD __forceinline__ bool operator!=(const Dual<T, N>& a, const Dual<T, N>& b) { return a.v != b.v; }
+template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> chain(const Dual<T, N>& a, T f, T df) {
+    Dual<T, N> r;
+    r.v = f;
+    for (int i = 0; i < N; ++i) r.d[i] = df * a.d[i];
+    return r;
+}
+// the scalar overloads stay visible beside the Dual ones inside this namespace
+using ::fabs;
+using ::fma;
+using ::fmax;
+using ::fmin;
+using ::sqrt;
+using ::tanh;
+template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> sqrt(const Dual<T, N>& a) {
+    const T s = sqrt(a.v);
+    return chain(a, s, T(0.5) / s);
+}
+template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> fabs(const Dual<T, N>& a) { return a.v < T(0) ? -a : a; }
+template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> fmax(const Dual<T, N>& a, const Dual<T, N>& b) { return a.v < b.v ? b : a; }
+template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> fmin(const Dual<T, N>& a, const Dual<T, N>& b) { return b.v < a.v ? b : a; }
+template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> tanh(const Dual<T, N>& a) {
+    const T t = tanh(a.v);
+    return chain(a, t, T(1) - t * t);
+}
+template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> fma(const Dual<T, N>& a, const Dual<T, N>& b, const Dual<T, N>& c) {
+    Dual<T, N> r;
+    r.v = fma(a.v, b.v, c.v);
+    for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i] + c.d[i];
+    return r;
 }
 
 // Scalar parameters of a spacetime in the working precision.
