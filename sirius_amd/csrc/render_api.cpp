@@ -173,7 +173,7 @@ int sirius_hip_launch(SiriusHipHandle hh, const SiriusHipCamera* cam, const Siri
                           H->star_n[1], H->star_brightness[1], bo.jac, bo.cls, bo.flags, H->bundle.initial_size,
                           H->fb_dev, H->stream);
     if (e != hipSuccess) return H->hip_fail(e, "star shade");
-    e = hipMemcpyAsync(H->fb_host.data(), H->fb_dev, H->fb_host.size() * sizeof(float), hipMemcpyDeviceToHost,
+    e = hipMemcpyAsync(H->fb_host, H->fb_dev, H->fb_n * sizeof(float), hipMemcpyDeviceToHost,
                        H->stream);
     if (e != hipSuccess) return H->hip_fail(e, "framebuffer copy");
     e = hipStreamSynchronize(H->stream);  // launch is synchronous (RDOP001A.cu:1044)
@@ -185,7 +185,7 @@ int sirius_hip_launch(SiriusHipHandle hh, const SiriusHipCamera* cam, const Siri
 float* sirius_hip_get_frame_buffer(SiriusHipHandle hh) {
     if (!hh) return nullptr;
     Handle* H = static_cast<Handle*>(hh);
-    return H->initialised && !H->fb_host.empty() ? H->fb_host.data() : nullptr;
+    return H->initialised && H->fb_host ? H->fb_host : nullptr;
 }
 
 size_t sirius_hip_get_frame_buffer_size(SiriusHipHandle hh) {

@@ -217,6 +217,9 @@ void Handle::release() {
     }
     if (fb_dev) (void)hipFree(fb_dev);
     fb_dev = nullptr;
+    if (fb_host) (void)hipHostFree(fb_host);
+    fb_host = nullptr;
+    fb_n = 0;
     if (bg_dev) (void)hipFree(bg_dev);
     bg_dev = nullptr;
     if (pinned) (void)hipHostFree(pinned);
@@ -821,7 +824,15 @@ int sirius_hip_resize(SiriusHipHandle hh, int width, int height) {
     size_t bytes = (size_t)width * height * 4 * sizeof(float);
     hipError_t e = hipMalloc(&H->fb_dev, bytes);
     if (e != hipSuccess) { H->fb_dev = nullptr; return H->hip_fail(e, "framebuffer"); }
-    H->fb_host.assign((size_t)width * height * 4, 0.f);
+    // the host copy is pinned: launch's copy of the frame runs at the link's full rate
+    if (H->fb_host) (void)hipHostFree(H->fb_host);
+    H->fb_host = nullptr;
+    H->fb_n = 0;
+    const size_t n = (size_t)width * height * 4;
+    e = hipHostMalloc((void**)&H->fb_host, n * sizeof(float), hipHostMallocDefault);
+    if (e != hipSuccess) { H->fb_host = nullptr; return H->hip_fail(e, "framebuffer host copy"); }
+    std::memset(H->fb_host, 0, n * sizeof(float));
+    H->fb_n = n;
     H->fb_w = width;
     H->fb_h = height;
     H->initialised = true;
@@ -835,7 +846,6 @@ void sirius_hip_cleanup(SiriusHipHandle hh) {
     try {
         H->release();
         H->initialised = false;
-        H->fb_host.clear();
     } catch (...) {
     }
 }

@@ -74,7 +74,8 @@ struct Handle {
     const void* cam_dev_held = nullptr;
     // framebuffer (IAccelerator::initialise / getFrameBuffer)
     float* fb_dev = nullptr;
-    std::vector<float> fb_host;
+    float* fb_host = nullptr;  // pinned host copy (getFrameBuffer), fb_n floats
+    size_t fb_n = 0;
     int fb_w = 0, fb_h = 0;
     uint8_t* bg_dev = nullptr;
     int bg_w = 0, bg_h = 0;
