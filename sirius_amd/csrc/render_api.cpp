@@ -276,11 +276,12 @@ int sirius_hip_autotune(SiriusHipHandle hh, const SiriusHipCamera* cams, int n_c
     SiriusHipTraceOpts bo = *opts;
     const int blocks[3] = {64, 128, 256};
     const int refills[4] = {1, 8, 16, 32};
+    // every block size at full occupancy: on a sample, lower occupancies can time faster (its
+    // short trace has little latency to hide) while on whole batches they ran up to 1.8x slower
     for (int bs : blocks) {
         int occ = persistent_blocks_per_cu(false, B.family, B.tp.emission, B.tp.n_cam > 1, bs);
-        int bpcs[3] = {occ, std::max(1, occ / 2), std::max(1, occ / 4)};
-        for (int j = 0; j < 3; ++j) {
-            if (j > 0 && bpcs[j] == bpcs[j - 1]) continue;
+        int bpcs[1] = {occ};
+        for (int j = 0; j < 1; ++j) {
             for (int rf : refills) {
                 SiriusHipTraceOpts t = *opts;
                 t.block_size = bs;

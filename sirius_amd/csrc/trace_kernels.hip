@@ -73,6 +73,10 @@ SIRIUS_HD __forceinline__ void renorm_energy64(const Spacetime<double>& s, const
 // carries the code (and registers) of its own model. TIME (FP64 only) integrates the
 // coordinate time, which enters the reference's error norm (PHGD001A.cpp:508-517).
 // ------------------------------------------------------------------------------------------
+// Volumetric emission: up to this many spectral bins accumulate in registers (FP64) for the
+// ray's whole trace and are stored once when it ends; more bins accumulate in device memory.
+constexpr int SIRIUS_VOL_REG = 8;
+
 template <class T, int FAM>
 struct Ray {
     T x[4], p[4];
@@ -86,6 +90,7 @@ struct Ray {
     int cam;
     int cls;        // termination class of a finished ray
     uint32_t attempts, rejected, landings, flags;
+    double vacc[SIRIUS_VOL_REG];  // volumetric emission accumulators (n_bins <= SIRIUS_VOL_REG)
 };
 
 template <class T, int FAM, int EMIT>
@@ -124,16 +129,27 @@ SIRIUS_HD __forceinline__ void ray_start(Ray<T, FAM>& R, const CamConst* cams, i
     R.vr_prev = T(-1);
     R.attempts = R.rejected = R.landings = 0;
     R.flags = tp.gate_flags ? (o.flags[id] & SIRIUS_FLAG_CRITICAL) : 0;
-    if constexpr (EMIT == SIRIUS_EMISSION_VOLUMETRIC)
-        for (int k = 0; k < tp.n_bins; ++k) o.vol_acc[(int64_t)k * o.n_rays + id] = 0.0;
+    if constexpr (EMIT == SIRIUS_EMISSION_VOLUMETRIC) {
+#pragma unroll
+        for (int k = 0; k < SIRIUS_VOL_REG; ++k) R.vacc[k] = 0.0;
+        if (tp.n_bins > SIRIUS_VOL_REG)
+            for (int k = 0; k < tp.n_bins; ++k) o.vol_acc[(int64_t)k * o.n_rays + id] = 0.0;
+    }
 }
 
 // A finished ray leaves its class, counters and final state (x^mu, p_mu in FP64) in the
 // result arrays; k_finish turns them into directions, emission and null drift afterwards, so
 // the tracing kernels carry none of that FP64 code.
 template <class T, int FAM, int EMIT>
-SIRIUS_HD __forceinline__ void ray_store(const Ray<T, FAM>& R, const DevOut& o) {
+SIRIUS_HD __forceinline__ void ray_store(const Ray<T, FAM>& R, const DevOut& o, const TraceParams& tp) {
     int64_t id = R.id;
+    if constexpr (EMIT == SIRIUS_EMISSION_VOLUMETRIC) {
+        if (tp.n_bins <= SIRIUS_VOL_REG) {
+#pragma unroll
+            for (int k = 0; k < SIRIUS_VOL_REG; ++k)
+                if (k < tp.n_bins) o.vol_acc[(int64_t)k * o.n_rays + id] = R.vacc[k];
+        }
+    }
     o.cls[id] = (uint8_t)R.cls;
     o.flags[id] = (uint8_t)R.flags;
     o.lam[id] = (float)R.lam;
@@ -238,8 +254,10 @@ SIRIUS_HD __forceinline__ void vol_accumulate(Ray<T, FAM>& R, const CamConst& c,
         const T r_lo = T(c.r_in > tp.vol_r_min ? c.r_in : tp.vol_r_min);  // no emitters inside the ISCO
         if (r < r_lo || r > T(tp.vol_r_max)) continue;
         T sig = T(tp.vol_h_over_r) * r;
-        T rho = T(tp.vol_j0) / (r * r) * exp(-xq[3] * xq[3] / (T(2) * sig * sig)) *
-                vol_window(r, r_lo, T(tp.vol_r_max));
+        // beyond 10 scale heights the density factor is below e^-50: nothing to add
+        T zarg = xq[3] * xq[3] / (T(2) * sig * sig);
+        if (zarg > T(50)) continue;
+        T rho = T(tp.vol_j0) / (r * r) * exp(-zarg) * vol_window(r, r_lo, T(tp.vol_r_max));
         // emitter velocity, frequencies and the spectrum in the path's precision (each node's
         // contribution then joins an FP64 accumulator); p by cubic Hermite with dp/dlam
         T th = gn[q], t2 = th * th, t3 = t2 * th, hh = R.h;
@@ -261,9 +279,20 @@ SIRIUS_HD __forceinline__ void vol_accumulate(Ray<T, FAM>& R, const CamConst& c,
         T w = rho * (nuo * nuo * nuo / (nue * nue)) * dl * exp(-(R.tau + T(0.5) * dtau));
         T tv = pow_pos(T(6) / r, T(0.75));
         T xk = T(xk0), qk = T(qk0);
-        for (int k = 0; k < tp.n_bins; ++k, xk *= qk) {
-            T xe = xk / gf;
-            o.vol_acc[(int64_t)k * o.n_rays + R.id] += (double)(w * xe * xe * xe / expm1(xe / tv));
+        if (tp.n_bins <= SIRIUS_VOL_REG) {
+#pragma unroll
+            for (int k = 0; k < SIRIUS_VOL_REG; ++k) {
+                if (k < tp.n_bins) {
+                    T xe = xk / gf;
+                    R.vacc[k] += (double)(w * xe * xe * xe / expm1(xe / tv));
+                }
+                xk *= qk;
+            }
+        } else {
+            for (int k = 0; k < tp.n_bins; ++k, xk *= qk) {
+                T xe = xk / gf;
+                o.vol_acc[(int64_t)k * o.n_rays + R.id] += (double)(w * xe * xe * xe / expm1(xe / tv));
+            }
         }
         R.tau += dtau;
     }
@@ -564,7 +593,7 @@ __device__ __forceinline__ void persistent_body(const CamConst* __restrict__ cam
                 uint32_t k = atomicAdd(esc_count, 1u);
                 esc_list[k] = (int32_t)R.id;
             } else if (st == ST_DONE) {
-                ray_store<T, FAM, EMIT>(R, o);
+                ray_store<T, FAM, EMIT>(R, o, tp);
             }
             if (st != ST_RUN) {
                 my_attempts += R.attempts;
@@ -732,7 +761,7 @@ __global__ __launch_bounds__(256) void k_symplectic_grid(const CamConst* __restr
     R.attempts = n;
     R.cls = cls;
     R.rdisk = rd;
-    ray_store<T, FAM, SIRIUS_EMISSION_NONE>(R, o);
+    ray_store<T, FAM, SIRIUS_EMISSION_NONE>(R, o, tp);
 }
 
 }  // namespace sirius
