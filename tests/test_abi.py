@@ -77,7 +77,19 @@ def test_no_device_fails_loudly():
     with pytest.raises(SiriusHipError) as e:
         acc.initialise(8, 8)
     assert e.value.rc == abi.E_DEVICE
-    assert acc.getFrameBuffer is not None
+    # the image writers, ray bundles, star spheres and ray dump need the device too
+    from oracle import oracle_py as O
+    frame = np.zeros((4, 4, 4), np.float32)
+    for call in (lambda: acc.encode_image(abi.IMAGE_PNG, frame),
+                 lambda: acc.trace_bundle([abi.camera(30, 80, 30, 4, 4)], [abi.metric_ks()],
+                                          O.default_opts(abi.PRECISION_FP64)),
+                 lambda: acc.sample_starfield(0, np.array([[0.0, 0.0, 1.0]], np.float32))):
+        with pytest.raises(SiriusHipError) as e:
+            call()
+        assert e.value.rc == abi.E_DEVICE
+    with pytest.raises(SiriusHipError) as e:
+        acc.write_rays("/nonexistent/rays.bin")
+    assert e.value.rc == abi.E_STATE  # no trace ran
     acc.close()
 
 
