@@ -170,3 +170,23 @@ def test_beam_star_light_is_the_pixel_average(acc):
     assert np.median(e_b[mid]) < 0.2 * np.median(e_p[mid])
     tot = lambda f, sel: f[sel].sum()
     assert abs(tot(beam, far) - tot(ss, far)) < 0.01 * tot(ss, far)
+
+
+def test_bundle_batch_of_cameras_equals_single_cameras(acc):
+    """A batch of two cameras (different spins, sizes and metrics) gives each camera's bundles as
+    tracing it alone does, bit for bit."""
+    m0, m1 = abi.metric_ks(1.0, 0.5), abi.metric_ks(1.0, 0.95, 0.2)
+    c0 = abi.camera(40.0, 60.0, 30.0, 20, 12, disk=(0.0, 20.0), metric_index=0)
+    c1 = abi.camera(25.0, 85.0, 35.0, 16, 16, disk=(0.0, 20.0), metric_index=1)
+    o = O.default_opts(abi.PRECISION_FP64)
+    o.emission, o.n_bins = abi.EMISSION_THIN_DISK, 0
+    print("[case] bundle batch of two cameras", flush=True)
+    both = acc.trace_bundle([c0, c1], [m0, m1], o)
+    c1s = abi.camera(25.0, 85.0, 35.0, 16, 16, disk=(0.0, 20.0), metric_index=0)
+    one = acc.trace_bundle([c0], [m0], o)
+    two = acc.trace_bundle([c1s], [m1], o)
+    n0 = 20 * 12
+    for k in ("cls", "flags"):
+        assert np.array_equal(both[k][:n0], one[k]) and np.array_equal(both[k][n0:], two[k]), k
+    for k in ("jac", "beam"):
+        assert np.array_equal(both[k][:, :n0], one[k]) and np.array_equal(both[k][:, n0:], two[k]), k
