@@ -59,6 +59,8 @@ struct LaunchConfig {
     float warpVelocity = 0.5f;
     float bubbleRadius = 1.0f;
     float bubbleSigma = 0.5f;
+    bool enableStarfield = false;      // ACIB001A.h:110-112
+    float starfieldBrightness = 1.0f;
 };
 
 // Same pure-virtual method set as ACIB001A.h:148-189.
@@ -107,15 +109,25 @@ public:
     int traceBatch(const SiriusHipCamera* cams, int n_cam, const SiriusHipMetric* metrics, int n_metric,
                    const SiriusHipTraceOpts& opts, SiriusHipRayOutput& out, SiriusHipStats* stats);
     void setPrecision(int precision) { m_precision = precision; }
+    // Star light averaged over each pixel's ray bundle in later launches (point-sampled when off).
+    bool setRayBundle(bool enabled);
+    // Writes the framebuffer of the last launch (SIRIUS_IMAGE_*; exposure applies to the PPM),
+    // converted on the device: what SREP001A.cpp:252-276 does with EXRWriter / BufferWriter.
+    bool writeImage(const std::string& path, int format, float exposure = 1.0f);
 
     // LaunchConfig -> C-ABI descriptors (units of the hole's mass).
     static void convertConfig(const LaunchConfig& cfg, SiriusHipCamera& cam, SiriusHipMetric& metric,
                               SiriusHipTraceOpts& opts, int precision);
 
 private:
+    // LaunchConfig::enableStarfield -> the default catalogue on both celestial spheres
+    void syncStarfield(const LaunchConfig& config);
+
     SiriusHipHandle m_handle = nullptr;
     int m_precision = SIRIUS_PRECISION_FP32_ESCALATE;
     std::string m_error;
+    bool m_stars = false;
+    float m_starBrightness = 0.0f;
 };
 
 // Factory hook for ACBF001A.cpp's createAccelerator(BackendType::HIP).

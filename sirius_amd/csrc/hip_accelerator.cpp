@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 namespace Sirius::Acceleration::HIP {
 
@@ -80,8 +81,65 @@ void HIPAccelerator::convertConfig(const LaunchConfig& cfg, SiriusHipCamera& cam
     if (cfg.maxSteps > 0) opts.max_steps = cfg.maxSteps;
 }
 
+// The OptiX backend turns LaunchConfig::enableStarfield into its kernel's procedural star field
+// with starfieldBrightness (RDOX001A.cpp:269-274). Here it installs the reference's default
+// catalogue (StarfieldConfig defaults, seed 42) as the sky, and the next seed's as the far side
+// of a wormhole throat; both are regenerated only when the settings change.
+void HIPAccelerator::syncStarfield(const LaunchConfig& config) {
+    if (!config.enableStarfield) {
+        if (m_stars) {
+            sirius_hip_set_starfield(m_handle, 0, nullptr, nullptr, 0);
+            sirius_hip_set_starfield(m_handle, 1, nullptr, nullptr, 0);
+            m_stars = false;
+        }
+        return;
+    }
+    if (m_stars && m_starBrightness == config.starfieldBrightness) return;
+    SiriusHipStarfield sf;
+    sirius_hip_starfield_default(&sf);
+    sf.brightness_scale = config.starfieldBrightness;
+    for (int side = 0; side < 2; ++side) {
+        sf.seed = 42u + (uint32_t)side;
+        int n = 0;
+        if (sirius_hip_starfield_generate(&sf, nullptr, 0, &n) != SIRIUS_HIP_OK) return;
+        std::vector<SiriusHipStar> stars((size_t)std::max(n, 1));
+        if (sirius_hip_starfield_generate(&sf, stars.data(), n, &n) != SIRIUS_HIP_OK ||
+            sirius_hip_set_starfield(m_handle, side, &sf, stars.data(), n) != SIRIUS_HIP_OK) {
+            m_error = sirius_hip_last_error(m_handle);
+            return;
+        }
+    }
+    m_stars = true;
+    m_starBrightness = config.starfieldBrightness;
+}
+
+bool HIPAccelerator::setRayBundle(bool enabled) {
+    if (!m_handle) return false;
+    SiriusHipRayBundle b;
+    sirius_hip_default_ray_bundle(&b);
+    b.enabled = enabled ? 1 : 0;
+    if (sirius_hip_set_ray_bundle(m_handle, &b) != SIRIUS_HIP_OK) {
+        m_error = sirius_hip_last_error(m_handle);
+        return false;
+    }
+    return true;
+}
+
+bool HIPAccelerator::writeImage(const std::string& path, int format, float exposure) {
+    if (!m_handle) return false;
+    SiriusHipImageOpts io;
+    sirius_hip_default_image_opts(&io, format);
+    io.exposure = exposure;
+    if (sirius_hip_write_image(m_handle, path.c_str(), nullptr, 0, 0, 0, &io, nullptr) != SIRIUS_HIP_OK) {
+        m_error = sirius_hip_last_error(m_handle);
+        return false;
+    }
+    return true;
+}
+
 void HIPAccelerator::launch(const LaunchConfig& config) {
     if (!m_handle) return;
+    syncStarfield(config);
     SiriusHipCamera cam;
     SiriusHipMetric metric;
     SiriusHipTraceOpts opts;
