@@ -176,3 +176,24 @@ def test_ray_dump_needs_results(tmp_path):
         assert e.value.rc == abi.E_STATE
     finally:
         a.close()
+
+
+@pytest.mark.parametrize("h,w", [(1, 1), (1, 17), (17, 1), (16, 3), (33, 5)])
+def test_edge_sizes_decode_to_the_restatement(acc, h, w):
+    """Frames of one pixel, one row or column, and EXR heights on and across the 16-line block
+    boundary (PNGWriterTest.GammaCorrection, TSOF009A.cpp:155-178, writes a 1x1 frame of 0.5)."""
+    rng = np.random.default_rng(h * 100 + w)
+    f = rng.uniform(-0.2, 1.3, (h, w, 4)).astype(np.float32)
+    if (h, w) == (1, 1):
+        f[:] = 0.5
+    print(f"[case] edge size {h}x{w}", flush=True)
+    png, _ = acc.encode_image(abi.IMAGE_PNG, f)
+    want = np.concatenate([I.srgb8(f[..., :3]), I.alpha8(f[..., 3:])], axis=-1)
+    assert np.array_equal(I.png_decode(png), want)
+    if (h, w) == (1, 1):
+        assert list(I.png_decode(png)[0, 0, :3]) == [188, 188, 188] and len(png) < 1000
+    d = I.exr_decode(acc.encode_image(abi.IMAGE_EXR, f)[0])
+    for k, c in enumerate("RGBA"):
+        assert np.array_equal(d["channels"][c], I.half_bits(f[..., k])), c
+    assert np.array_equal(I.ppm_decode(acc.encode_image(abi.IMAGE_PPM, f)[0]), I.aces8(f[..., :3]))
+    assert np.array_equal(I.pfm_decode(acc.encode_image(abi.IMAGE_PFM, f)[0]), f[..., :3])
