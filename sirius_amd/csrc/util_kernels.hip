@@ -192,6 +192,18 @@ hipError_t launch_gate(const CamConst* cams, int n_cam, const float* tabs, int64
 // RGBA shading of one camera's results: disk intensity bins split into red/green/blue thirds
 // by frequency, escaped rays sample the equirectangular background (black without one),
 // captured rays are black; x / (1 + x) tone curve.
+namespace {
+// c00 w00 + c10 w10 + c01 w01 + c11 w11 on texel values / 255, each product and sum rounded on its own
+__device__ __forceinline__ float bilinear(uint8_t a, uint8_t b, uint8_t c, uint8_t d, float w00, float w10, float w01,
+                                          float w11) {
+#pragma clang fp contract(off)
+    // the IEEE float quotient (this file's FP32 division is the approximate one): in FP64, rounded
+    // once, which is the correctly rounded float for every byte value
+    const float fa = (float)(a / 255.0), fb = (float)(b / 255.0), fc = (float)(c / 255.0), fd = (float)(d / 255.0);
+    return fa * w00 + fb * w10 + fc * w01 + fd * w11;
+}
+}  // namespace
+
 __global__ __launch_bounds__(256) void k_shade(DevOut o, int64_t n, int width, int height, int n_bins,
                                                const uint8_t* __restrict__ bg, int bgw, int bgh,
                                                float* __restrict__ fb) {
@@ -206,12 +218,24 @@ __global__ __launch_bounds__(256) void k_shade(DevOut o, int64_t n, int width, i
         }
         for (int ch = 0; ch < 3; ++ch) rgb[ch] = rgb[ch] / (1.f + rgb[ch]);
     } else if (cls == SIRIUS_RAY_ESCAPED && bg) {
-        float u = (o.phi[id] + 3.14159265f) / 6.2831853f, v = o.theta[id] / 3.14159265f;
-        int ix = min(bgw - 1, max(0, (int)(u * bgw))), iy = min(bgh - 1, max(0, (int)(v * bgh)));
-        const uint8_t* px = bg + 4 * ((int64_t)iy * bgw + ix);
-        rgb[0] = px[0] / 255.f;
-        rgb[1] = px[1] / 255.f;
-        rgb[2] = px[2] / 255.f;
+        // RenderSession::sampleStarfield (SNRS001A.cpp:631-703): phi in [0, 2 pi), u = phi / 2 pi,
+        // v = theta / pi, bilinear between the four texels around (u (W - 1), v (H - 1)); the
+        // coordinates in FP64 and the weights and sums in float, in the reference's order
+        const double TWO_PI = 6.283185307179586, PI = 3.141592653589793;
+        double ph = o.phi[id];
+        if (ph < 0) ph += TWO_PI;
+        const double u = ph / TWO_PI, v = (double)o.theta[id] / PI;
+        const double fxp = u * (bgw - 1), fyp = v * (bgh - 1);
+        const int x0 = min(bgw - 1, max(0, (int)floor(fxp))), y0 = min(bgh - 1, max(0, (int)floor(fyp)));
+        const int x1 = min(x0 + 1, bgw - 1), y1 = min(y0 + 1, bgh - 1);
+        const double fx = fxp - x0, fy = fyp - y0;
+        const float w00 = (float)((1.0 - fx) * (1.0 - fy)), w10 = (float)(fx * (1.0 - fy));
+        const float w01 = (float)((1.0 - fx) * fy), w11 = (float)(fx * fy);
+        const uint8_t* c00 = bg + 4 * ((int64_t)y0 * bgw + x0);
+        const uint8_t* c10 = bg + 4 * ((int64_t)y0 * bgw + x1);
+        const uint8_t* c01 = bg + 4 * ((int64_t)y1 * bgw + x0);
+        const uint8_t* c11 = bg + 4 * ((int64_t)y1 * bgw + x1);
+        for (int ch = 0; ch < 3; ++ch) rgb[ch] = bilinear(c00[ch], c10[ch], c01[ch], c11[ch], w00, w10, w01, w11);
     }
     fb[4 * id + 0] = rgb[0];
     fb[4 * id + 1] = rgb[1];

@@ -260,3 +260,48 @@ def test_accelerator_launch_shades_framebuffer(acc):
         assert np.allclose(fb2[cls == abi.RAY_ESCAPED][:, 1], 200 / 255)
     with pytest.raises(Exception):
         acc.launch(abi.camera(50.0, 75.0, 20.0, w + 1, h), m, o)
+
+
+def _reference_background(theta, phi, tex):
+    """RenderSession::sampleStarfield (SNRS001A.cpp:631-703) at the escape angles: phi into
+    [0, 2 pi), u = phi / 2 pi, v = theta / pi, bilinear between the texels around
+    (u (W - 1), v (H - 1)); coordinates in FP64, weights and sums in float."""
+    h, w = tex.shape[:2]
+    ph = phi.astype(np.float64)
+    ph = np.where(ph < 0, ph + 2 * np.pi, ph)
+    u, v = ph / (2 * np.pi), theta.astype(np.float64) / np.pi
+    px, py = u * (w - 1), v * (h - 1)
+    x0, y0 = np.floor(px).astype(int), np.floor(py).astype(int)
+    x1, y1 = np.minimum(x0 + 1, w - 1), np.minimum(y0 + 1, h - 1)
+    fx, fy = px - x0, py - y0
+    w00 = ((1 - fx) * (1 - fy)).astype(np.float32)
+    w10 = (fx * (1 - fy)).astype(np.float32)
+    w01 = ((1 - fx) * fy).astype(np.float32)
+    w11 = (fx * fy).astype(np.float32)
+    c = tex[..., :3].astype(np.float32) / np.float32(255)
+    out = c[y0, x0] * w00[:, None] + c[y0, x1] * w10[:, None]
+    out = out + c[y1, x0] * w01[:, None]
+    return out + c[y1, x1] * w11[:, None]
+
+
+def test_background_is_the_reference_sampler(acc):
+    """Escaped pixels of launch take the equirectangular background as the reference's session
+    samples it, bilinear, to float rounding (the same float steps in the same order)."""
+    w, h = 80, 45
+    acc.initialise(w, h)
+    tex = np.random.default_rng(9).integers(0, 256, (64, 128, 4), dtype=np.uint8)
+    acc.uploadBackground(tex)
+    m = abi.metric_ks(1.0, 0.6)
+    cam = abi.camera(30.0, 70.0, 60.0, w, h, disk=(0.0, 0.5))
+    o = O.default_opts(abi.PRECISION_FP32_ESCALATE)
+    o.emission, o.n_bins = abi.EMISSION_NONE, 0
+    print("[case] background sampler", flush=True)
+    acc.launch(cam, m, o)
+    fb = acc.getFrameBuffer().reshape(-1, 4)[:, :3]
+    rays, _ = acc.trace_batch([cam], [m], o)
+    esc = rays.cls == abi.RAY_ESCAPED
+    assert esc.sum() > 0.5 * w * h
+    want = _reference_background(rays.theta[esc], rays.phi[esc], tex)
+    assert np.abs(fb[esc] - want).max() <= 2e-7, np.abs(fb[esc] - want).max()
+    assert np.all(fb[~esc] == 0)
+    acc.uploadBackground(np.zeros((1, 1, 4), np.uint8))
