@@ -85,6 +85,7 @@ struct Ray {
     T h, lam;
     T tau;          // volumetric optical depth
     T vr_prev;      // radial velocity sign tracking (FP32 escalation)
+    T r_cur;        // event radius of the current state (kept from the step that reached it)
     T rdisk;        // disk-hit radius of a finished ray
     int64_t id;
     int cam;
@@ -116,11 +117,12 @@ SIRIUS_HD __forceinline__ void ray_start(Ray<T, FAM>& R, const CamConst* cams, i
         init_state<T>(c, s, (int)(pix % c.width), (int)(pix / c.width), R.x, R.p);
     }
     eval<T, FAM>(s, R.x, R.p, R.d);
+    R.r_cur = event_radius(s, R.x);
     R.h = T(tp.h_init);
     if (tp.far_slope > 0) {
         // the first step starts at the far-field cap (with h0 = 0.01 a ray would spend ~10
         // doubling steps reaching it from the observer)
-        T cap = event_radius(s, R.x) * T(tp.far_slope);
+        T cap = R.r_cur * T(tp.far_slope);
         if (cap < T(tp.h_max)) cap = T(tp.h_max);
         if (cap > R.h) R.h = cap;
     }
@@ -370,7 +372,7 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
     constexpr bool vol = (FAM == 0) && EMIT == SIRIUS_EMISSION_VOLUMETRIC;
     T hmax = T(tp.h_max);
     T r0 = T(0);
-    if (tp.far_slope > 0 || tp.step_rexp > 0) r0 = event_radius(s, R.x);
+    if (tp.far_slope > 0 || tp.step_rexp > 0) r0 = R.r_cur;
     if (tp.far_slope > 0) {
         T rr = r0 * T(tp.far_slope);
         hmax = rr > hmax ? rr : hmax;
@@ -485,6 +487,7 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
     // given length rises steeply as r falls, so the proposal scales with (r1 / r0)^exp
     if (tp.step_rexp > 0 && r1 < r0) hprop *= pow_pos(r1 / r0, T(tp.step_rexp));
     R.h = next_step(hprop, e, T(tp.h_min), hmax2);
+    R.r_cur = r1;
     for (int i = 0; i < 4; ++i) { R.x[i] = x1[i]; R.p[i] = p1[i]; }
     if constexpr (fp32) commit_lo(R.lo, lo1);
     R.d = d1;
