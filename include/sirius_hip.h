@@ -232,6 +232,9 @@ int sirius_hip_synchronize(SiriusHipHandle h);
  * into host buffers (host pointers; null ones skipped). SIRIUS_HIP_E_STATE once a later call
  * has replaced the library's result arrays. */
 int sirius_hip_fetch_results(SiriusHipHandle h, SiriusHipRayOutput* host_out);
+/* Size of those results: rays and spectral bins (what fetch_results' host arrays must hold).
+ * SIRIUS_HIP_E_STATE when there are none. Has no counterpart in the reference. */
+int sirius_hip_last_result_size(SiriusHipHandle h, int64_t* n_rays, int32_t* n_bins);
 
 /* ---- physics: batched trace (new; no reference counterpart) --------------------------- */
 /* Fills opts with the defaults of the given precision (FP64 reference contract or FP32). */

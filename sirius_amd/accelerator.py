@@ -78,6 +78,7 @@ def lib():
     L.sirius_hip_starfield_generate.argtypes = [C.POINTER(abi.Starfield), C.c_void_p, i, C.POINTER(C.c_int)]
     L.sirius_hip_set_starfield.argtypes = [H, i, C.POINTER(abi.Starfield), C.c_void_p, i]
     L.sirius_hip_sample_starfield.argtypes = [H, i, C.c_void_p, i, C.c_void_p]
+    L.sirius_hip_last_result_size.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
     L.sirius_hip_default_ray_bundle.argtypes = [C.POINTER(abi.RayBundle)]
     L.sirius_hip_default_ray_bundle.restype = None
     L.sirius_hip_set_ray_bundle.argtypes = [H, C.POINTER(abi.RayBundle)]
@@ -324,8 +325,11 @@ class HipAccelerator:
         return st.as_dict() if stats else None
 
     def fetch_results(self, with_state=False):
-        """Host copy of the results of the last trace_resident call."""
-        out = abi.HostRays(self._resident_n, self._resident_bins, with_state)
+        """Host copy of the results the library holds: those of the last trace_resident,
+        trace_batch or launch, sized by the library's own count."""
+        n, nb = C.c_int64(), C.c_int32()
+        self._check(self._L.sirius_hip_last_result_size(self._h, C.byref(n), C.byref(nb)))
+        out = abi.HostRays(n.value, nb.value, with_state)
         ro = out.output()
         self._check(self._L.sirius_hip_fetch_results(self._h, C.byref(ro)))
         return out

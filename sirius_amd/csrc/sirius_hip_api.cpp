@@ -443,6 +443,22 @@ int bind_outputs(Handle& H, const SiriusHipRayOutput* out, int64_t n, int n_bins
     o.state = (double*)get("o_state", out ? out->state : nullptr, 64 * n);
     // volumetric emission accumulates in FP64 (the finish kernel rounds it into intensity)
     o.vol_acc = volumetric ? (double*)H.scratch("o_vacc", 8 * n * std::max(n_bins, 1), &rc) : nullptr;
+    // the arrays about to be written no longer hold the last results: callers that report
+    // their rays as the last results (trace_batch, launch) record them again afterwards
+    if (H.last) {
+        const void* mine[] = {o.cls, o.flags, o.theta, o.phi, o.lam, o.disk_r, o.redshift, o.null_drift,
+                              o.steps, o.intensity, o.state};
+        const void* held[] = {H.last->cls, H.last->flags, H.last->theta, H.last->phi, H.last->lam,
+                              H.last->disk_r, H.last->redshift, H.last->null_drift, H.last->steps,
+                              H.last->intensity, H.last->state};
+        bool shared = false;
+        for (const void* a : mine)
+            for (const void* b : held) shared |= a && a == b;
+        if (shared) {
+            delete H.last;
+            H.last = nullptr;
+        }
+    }
     return rc;
 }
 
@@ -961,6 +977,16 @@ int sirius_hip_fetch_results(SiriusHipHandle hh, SiriusHipRayOutput* host_out) {
     o.on_device = 0;
     return copy_outputs_to_host(*H, &o, *H->last, H->last_bins);
     GUARD_END(H)
+}
+
+int sirius_hip_last_result_size(SiriusHipHandle hh, int64_t* n_rays, int32_t* n_bins) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    if (!n_rays || !n_bins) return H->fail(SIRIUS_HIP_E_INVALID, "n_rays and n_bins are required");
+    if (!H->last) return H->fail(SIRIUS_HIP_E_STATE, "no trace results held");
+    *n_rays = H->last->n_rays;
+    *n_bins = H->last_bins;
+    return SIRIUS_HIP_OK;
 }
 
 int sirius_hip_synchronize(SiriusHipHandle hh) {

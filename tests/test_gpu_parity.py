@@ -15,6 +15,7 @@ import pytest
 
 from oracle import oracle_py as O
 from sirius_amd import abi
+from sirius_amd.accelerator import SiriusHipError
 
 pytestmark = pytest.mark.gpu
 
@@ -305,3 +306,30 @@ def test_background_is_the_reference_sampler(acc):
     assert np.abs(fb[esc] - want).max() <= 2e-7, np.abs(fb[esc] - want).max()
     assert np.all(fb[~esc] == 0)
     acc.uploadBackground(np.zeros((1, 1, 4), np.uint8))
+
+
+def test_last_results_follow_the_library_buffers(acc):
+    """fetch_results returns the last trace's results while the library's result arrays hold
+    them: after launch they are the frame's rays; once a later call has replaced the arrays
+    (here an autotune over a larger batch) fetching is an error, never a read of freed memory."""
+    m = abi.metric_ks(1.0, 0.9)
+    small = abi.camera(50.0, 75.0, 20.0, 32, 18, disk=(0.0, 30.0))
+    o = O.default_opts(abi.PRECISION_FP32_ESCALATE)
+    print("[case] last results and buffer replacement", flush=True)
+    acc.trace_resident([small], [m], o)
+    got = acc.fetch_results()
+    ref, _ = acc.trace_batch([small], [m], o)
+    assert np.array_equal(got.cls, ref.cls) and np.array_equal(got.theta, ref.theta)
+    w, h = 40, 24
+    acc.initialise(w, h)
+    frame = abi.camera(50.0, 75.0, 20.0, w, h, disk=(0.0, 30.0))
+    acc.launch(frame, m, o)
+    after_launch = acc.fetch_results()
+    want, _ = acc.trace_batch([frame], [m], o)
+    assert after_launch.n == w * h and np.array_equal(after_launch.cls, want.cls)
+    acc.trace_resident([small], [m], o)
+    big = abi.camera(50.0, 75.0, 20.0, 640, 360, disk=(0.0, 30.0))
+    acc.autotune([big], [m], O.default_opts(abi.PRECISION_FP32_ESCALATE))
+    with pytest.raises(SiriusHipError) as e:
+        acc.fetch_results()
+    assert e.value.rc == abi.E_STATE
