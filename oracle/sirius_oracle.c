@@ -298,7 +298,8 @@ void oracle_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->precision = precision;
     o->emission = SIRIUS_EMISSION_THIN_DISK;
     o->schedule = precision == SIRIUS_PRECISION_FP64 ? SIRIUS_SCHEDULE_GRID : SIRIUS_SCHEDULE_PERSISTENT;
-    /* BASELINE configs 1-2: rtol 1e-8, atol 1e-10, h0 = 0.01, h in [1e-8, 2] */
+    /* h0 = 0.01 and h_min = 1e-8 as PHCN001A.h:74,77; rtol 1e-8, atol 1e-10 and h_max 2 are the
+       oracle's FP64 settings */
     o->rtol = 1e-8;
     o->atol = 1e-10;
     o->h_init = 0.01;
