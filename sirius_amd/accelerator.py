@@ -320,8 +320,6 @@ class HipAccelerator:
         self._check(self._L.sirius_hip_trace_batch(self._h, ca, len(cams), ma, len(metrics),
                                                    C.byref(opts), C.byref(ro),
                                                    C.byref(st) if stats else None))
-        self._resident_n = sum(c.width * c.height for c in cams)
-        self._resident_bins = opts.n_bins
         return st.as_dict() if stats else None
 
     def fetch_results(self, with_state=False):
