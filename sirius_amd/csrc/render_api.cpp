@@ -243,9 +243,10 @@ int sirius_hip_critical_distance(SiriusHipHandle hh, const SiriusHipCamera* cams
     GUARD_END(H)
 }
 
-// Autotuner: times the FP32 persistent trace of a strided sample of the batch (every k-th ray,
-// about 1/16 of it, at least 64k rays) over block sizes {64, 128, 256} x blocks/CU
-// {occupancy, 1/2, 1/4 of it} x refill threshold {1, 8, 16, 32}, and keeps the fastest.
+// Autotuner: times the FP32 persistent trace over block sizes {64, 128, 256} x refill threshold
+// {1, 8, 16, 32}, each at full occupancy (blocks per CU from the occupancy API), and keeps the
+// fastest. Batches of up to 4 M rays are timed whole; larger ones on a strided sample (every
+// k-th ray, about 1/16 of the batch, at least 64k rays).
 int sirius_hip_autotune(SiriusHipHandle hh, const SiriusHipCamera* cams, int n_cam, const SiriusHipMetric* metrics,
                         int n_metric, SiriusHipTraceOpts* opts, double* best_ms) {
     if (!hh) return SIRIUS_HIP_E_INVALID;
