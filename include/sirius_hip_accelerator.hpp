@@ -128,6 +128,10 @@ private:
     std::string m_error;
     bool m_stars = false;
     float m_starBrightness = 0.0f;
+    // launch shape from sirius_hip_autotune, kept per frame size and spacetime
+    bool m_tuned = false;
+    double m_tuneKey[7] = {};
+    int m_tune[3] = {0, 0, 0};
 };
 
 // Factory hook for ACBF001A.cpp's createAccelerator(BackendType::HIP).

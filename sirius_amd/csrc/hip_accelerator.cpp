@@ -144,6 +144,29 @@ void HIPAccelerator::launch(const LaunchConfig& config) {
     SiriusHipMetric metric;
     SiriusHipTraceOpts opts;
     convertConfig(config, cam, metric, opts, m_precision);
+    // The FP32 path's launch shape is tuned on the first frame of each size and spacetime (shapes
+    // never change results); if tuning fails, the default shape is kept.
+    if (opts.precision == SIRIUS_PRECISION_FP32_ESCALATE && opts.integrator == SIRIUS_INTEGRATOR_DP45) {
+        const double key[7] = {double(cam.width), double(cam.height), double(metric.family), metric.M, metric.a,
+                               metric.Q, metric.Lambda};
+        bool same = m_tuned;
+        for (int i = 0; i < 7; ++i) same = same && key[i] == m_tuneKey[i];
+        if (!same) {
+            SiriusHipTraceOpts t = opts;
+            if (sirius_hip_autotune(m_handle, &cam, 1, &metric, 1, &t, nullptr) == SIRIUS_HIP_OK) {
+                m_tune[0] = t.block_size;
+                m_tune[1] = t.blocks_per_cu;
+                m_tune[2] = t.refill_min;
+            } else {
+                m_tune[0] = m_tune[1] = m_tune[2] = 0;
+            }
+            m_tuned = true;
+            for (int i = 0; i < 7; ++i) m_tuneKey[i] = key[i];
+        }
+        opts.block_size = m_tune[0];
+        opts.blocks_per_cu = m_tune[1];
+        opts.refill_min = m_tune[2];
+    }
     int rc = sirius_hip_launch(m_handle, &cam, &metric, &opts);
     if (rc != SIRIUS_HIP_OK) m_error = sirius_hip_last_error(m_handle);
 }
