@@ -230,7 +230,9 @@ int sirius_hip_upload_background(SiriusHipHandle h, const unsigned char* rgba, i
 int sirius_hip_synchronize(SiriusHipHandle h);
 /* Copies the results of the most recent trace_batch or launch held in library device memory
  * into host buffers (host pointers; null ones skipped). SIRIUS_HIP_E_STATE once a later call
- * has replaced the library's result arrays. */
+ * has replaced the library's result arrays. Only library-owned arrays are held: a result that
+ * trace_batch wrote into the caller's device array (on_device) is not copied here (its host
+ * array is left untouched), and sirius_hip_write_rays leaves it out of the file. */
 int sirius_hip_fetch_results(SiriusHipHandle h, SiriusHipRayOutput* host_out);
 /* Size of those results: rays and spectral bins (what fetch_results' host arrays must hold).
  * SIRIUS_HIP_E_STATE when there are none. Has no counterpart in the reference. */

@@ -528,6 +528,7 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
     // work[16..18] gate counts (A, B, L)
     uint32_t* work = (uint32_t*)H.scratch("work", 128, &rc);
     if (rc) return rc;
+    unsigned int* audit_acc = nullptr;  // surrogate audit: largest direction error, class mismatches
     if ((e = launch_fill(work, 32, 0u, H.stream)) != hipSuccess) return H.hip_fail(e, "fill");
     H.last_band = false;
     if (time_it) (void)hipEventRecord(H.ev0, H.stream);
@@ -671,6 +672,7 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
             if ((rc = bind_audit_outputs(H, B.n, o.n_bins, oa))) return rc;
             unsigned int* aacc = (unsigned int*)H.scratch("audit_acc", 64, &rc);
             if (rc) return rc;
+            audit_acc = aacc;
             if ((e = launch_fill(aacc, 2, 0u, H.stream)) != hipSuccess) return H.hip_fail(e, "fill");
             unsigned long long* aatt = (unsigned long long*)H.scratch("audit_att", 8, &rc);
             if (rc) return rc;
@@ -710,7 +712,7 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
             return H.hip_fail(e, "stats copy");
         uint32_t wk[32] = {0};
         if (persistent) {
-            if ((e = hipMemcpyAsync(wk, H.bufs["work"].ptr, sizeof(wk), hipMemcpyDeviceToHost, H.stream)) != hipSuccess)
+            if ((e = hipMemcpyAsync(wk, work, sizeof(wk), hipMemcpyDeviceToHost, H.stream)) != hipSuccess)
                 return H.hip_fail(e, "stats copy");
         }
         if ((e = hipStreamSynchronize(H.stream)) != hipSuccess) return H.hip_fail(e, "synchronize");
@@ -748,7 +750,8 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
         stats->flops += (double)bulk * flops_per_step(B.family);
         if (H.last_audit) {
             unsigned int aa[2] = {0, 0};
-            if ((e = hipMemcpy(aa, H.bufs["audit_acc"].ptr, sizeof(aa), hipMemcpyDeviceToHost)) != hipSuccess)
+            if ((e = hipMemcpyAsync(aa, audit_acc, sizeof(aa), hipMemcpyDeviceToHost, H.stream)) != hipSuccess ||
+                (e = hipStreamSynchronize(H.stream)) != hipSuccess)
                 return H.hip_fail(e, "audit copy");
             float err;
             std::memcpy(&err, &aa[0], 4);
@@ -950,6 +953,22 @@ int sirius_hip_trace_batch(SiriusHipHandle hh, const SiriusHipCamera* cams, int 
     if ((rc = bind_outputs(*H, out, B.n, opts->n_bins, dout, opts->emission == SIRIUS_EMISSION_VOLUMETRIC))) return rc;
     if (!H->last) H->last = new DevOut;
     *H->last = dout;
+    if (out->on_device) {
+        // the caller's device arrays are not held: the caller may free them before a later
+        // fetch_results / write_rays, which then skip those fields
+        DevOut& L = *H->last;
+        if (out->cls) L.cls = nullptr;
+        if (out->flags) L.flags = nullptr;
+        if (out->theta) L.theta = nullptr;
+        if (out->phi) L.phi = nullptr;
+        if (out->lambda) L.lam = nullptr;
+        if (out->disk_r) L.disk_r = nullptr;
+        if (out->redshift) L.redshift = nullptr;
+        if (out->null_drift) L.null_drift = nullptr;
+        if (out->steps) L.steps = nullptr;
+        if (out->intensity) L.intensity = nullptr;
+        if (out->state) L.state = nullptr;
+    }
     H->last_bins = opts->n_bins;
     if (stats) std::memset(stats, 0, sizeof(*stats));
     (void)hipEventRecord(H->ev2, H->stream);

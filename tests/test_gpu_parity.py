@@ -360,3 +360,29 @@ def test_last_results_follow_the_library_buffers(acc):
     with pytest.raises(SiriusHipError) as e:
         acc.fetch_results()
     assert e.value.rc == abi.E_STATE
+
+
+@pytest.mark.gpu
+def test_results_in_caller_device_arrays_are_not_held(acc):
+    """A trace into the caller's device arrays leaves only the library-owned fields as the held
+    results: after the caller frees its arrays, fetch_results copies the library's own fields
+    (here the state, not given by the caller) and leaves the caller's fields untouched, never
+    reading the freed memory."""
+    from devmem import DeviceArray
+    m = abi.metric_ks(1.0, 0.9)
+    cam = abi.camera(50.0, 75.0, 20.0, 48, 27, disk=(0.0, 30.0))
+    n = 48 * 27
+    o = O.default_opts(abi.PRECISION_FP32_ESCALATE)
+    print("[case] caller-owned device outputs", flush=True)
+    cls_d = DeviceArray(shape=(n,), dtype=np.uint8)
+    theta_d = DeviceArray(shape=(n,), dtype=np.float32)
+    ro = abi.RayOutput(on_device=1)
+    ro.cls, ro.theta = cls_d.data_ptr(), theta_d.data_ptr()
+    acc.trace_batch([cam], [m], o, device_out=ro)
+    want_cls = cls_d.get()
+    del cls_d, theta_d  # the caller frees its arrays
+    got = acc.fetch_results(with_state=True)
+    assert (got.cls == 0).all() and (got.theta == 0).all()  # not copied: the caller's fields
+    ref, _ = acc.trace_batch([cam], [m], o, with_state=True)
+    assert np.array_equal(want_cls, ref.cls)
+    assert np.array_equal(got.state, ref.state) and np.array_equal(got.phi, ref.phi)
