@@ -5,14 +5,21 @@ One step = one full frame traced on the GPU with every result left in device mem
   value          the FP32 path (compensated FP32 state, FP64 null-constraint re-normalisation,
                  escalation criteria at their defaults)
   fp64_plain     the plain FP64 integrator on the same frame (the comparison baseline)
-  accuracy       the FP32 frame against the FP64 frame, pixel by pixel
+  fp64_same_step_policy
+                 the FP64 integrator at the FP32 path's step policy (rtol, far-field cap,
+                 inward step proposal): the part of the speed-up that precision alone buys
+  symplectic     the frame by the fixed-step symplectic integrator (h = 0.01 M, FP64) with its
+                 null-constraint drift
+  accuracy       the FP32 frame against the FP64 frame, pixel by pixel, outside the critical band
+                 and (reported apart) inside it; per-pixel and per-bin intensity errors
   with_edge_escalation
                  the same frame with inner-edge disk crossings re-traced in FP64
   roofline       dominant kernel (the FP32 persistent tracer): algorithmic flops per launch
                  (1,326 per Kerr-Schild DP45 attempt) over its HIP-event time, against the
                  FP32 vector peak; traffic from the profile's HBM byte counters
   cpu_baseline   the reference's own CPU integrator (oracle/_ref, compiled from its sources)
-                 on every pixel of the frame, on this host's cores (about 15 s)
+                 on every pixel of the frame, on this host's cores (about 15 s), and the
+                 reference's image writers on the shaded frame
   other_workloads
                  the 64-camera random batch (volumetric, autotuned launch) and the wormhole and
                  warp scenes at 1024x1024: Mrays/s, geodesic steps/s, null-constraint drift
@@ -22,7 +29,7 @@ One step = one full frame traced on the GPU with every result left in device mem
                  the rays it answered
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-accuracy]
-                [--no-surrogate] [--no-extra] [--copy-calibration MIB]
+                [--no-surrogate] [--no-symplectic] [--no-extra] [--copy-calibration MIB]
 """
 import argparse
 import json
@@ -98,25 +105,48 @@ def tune_fp64(acc, cam, metric, o):
     return o
 
 
+def _angles(u, v):
+    return np.arccos(np.clip((u * v).sum(1) / np.linalg.norm(u, axis=1) / np.linalg.norm(v, axis=1), -1, 1))
+
+
+def _intensity_errors(If, Ir, scale, prefix, out):
+    """Intensity error of disk pixels (bins x pixels): per pixel, the largest bin error over the
+    pixel's brightest bin (floored at 1e-2 of the frame's median brightest bin); and per bin, the
+    error relative to that bin's own value, over bins above 1e-6 of the same median."""
+    rel = np.abs(If - Ir).max(0) / np.maximum(Ir.max(0), 1e-2 * scale)
+    out[prefix + "intensity_rel_err_max"] = float(rel.max())
+    out[prefix + "intensity_rel_err_p999"] = float(np.percentile(rel, 99.9))
+    out[prefix + "intensity_l1_rel_err"] = float(np.abs(If - Ir).sum() / Ir.sum())
+    live = Ir > 1e-6 * scale
+    if live.any():
+        pb = np.abs(If - Ir)[live] / Ir[live]
+        out[prefix + "intensity_per_bin_rel_err_max"] = float(pb.max())
+        out[prefix + "intensity_per_bin_rel_err_p999"] = float(np.percentile(pb, 99.9))
+        out[prefix + "intensity_per_bin_rel_err_median"] = float(np.median(pb))
+
+
 def accuracy(fast, ref, crit, band=0.5):
+    """The FP32 frame against the FP64 frame, pixel by pixel: outside the critical band (the
+    contract's classes) and, reported apart, inside it (rays within `band` M of the critical
+    curve: their direction and intensity errors where both frames agree on the class)."""
     out = {}
     keep = np.abs(crit) > band
     out["class_agreement_outside_band"] = float((fast.cls == ref.cls)[keep].mean())
     out["band_fraction"] = float((~keep).mean())
-    esc = keep & (fast.cls == 0) & (ref.cls == 0)
-    if esc.any():
-        u, v = fast.direction()[esc], ref.direction()[esc]
-        ang = np.arccos(np.clip((u * v).sum(1) / np.linalg.norm(u, axis=1) / np.linalg.norm(v, axis=1), -1, 1))
-        out["escape_dir_err_max_rad"] = float(ang.max())
-        out["escape_dir_err_p99_rad"] = float(np.percentile(ang, 99))
-    dk = keep & (fast.cls == 2) & (ref.cls == 2)
-    if dk.any():
-        Ir, If = ref.intensity[:, dk], fast.intensity[:, dk]
-        scale = np.median(Ir.max(0))
-        rel = np.abs(If - Ir).max(0) / np.maximum(Ir.max(0), 1e-2 * scale)
-        out["intensity_rel_err_max"] = float(rel.max())
-        out["intensity_rel_err_p999"] = float(np.percentile(rel, 99.9))
-        out["intensity_l1_rel_err"] = float(np.abs(If - Ir).sum() / Ir.sum())
+    out["class_agreement_in_band"] = float((fast.cls == ref.cls)[~keep].mean()) if (~keep).any() else 1.0
+    dk_all = (fast.cls == 2) & (ref.cls == 2)
+    scale = float(np.median(ref.intensity[:, dk_all].max(0))) if dk_all.any() else 1.0
+    for prefix, region in (("", keep), ("band_", ~keep)):
+        esc = region & (fast.cls == 0) & (ref.cls == 0)
+        if esc.any():
+            ang = _angles(fast.direction()[esc], ref.direction()[esc])
+            out[prefix + "escape_dir_err_max_rad"] = float(ang.max())
+            out[prefix + "escape_dir_err_p99_rad"] = float(np.percentile(ang, 99))
+            out[prefix + "escaped_rays"] = int(esc.sum())
+        dk = region & dk_all
+        if dk.any():
+            _intensity_errors(fast.intensity[:, dk], ref.intensity[:, dk], scale, prefix, out)
+            out[prefix + "disk_rays"] = int(dk.sum())
     return out
 
 
@@ -208,16 +238,13 @@ IMAGE_BYTES_PER_PIXEL = {"png": 16 + 4 + 4 + 4, "exr": 16 + 8, "ppm": 16 + 3}
 def image_writers(acc, steps):
     """The 1080p Kerr frame written to files as PNG, EXR and PPM from the framebuffer in device
     memory: the device conversion (HIP events), the copy of the payload to pinned host memory, the
-    host compression on up to 16 threads and the whole call, per format; beside it the reference's
-    own writers
-    (PNGWriter::writeRGBA, EXRWriter::writeEXR, BufferWriter::writePPM, compiled in place) on the
-    same frame on one host core."""
+    host compression on up to 16 threads and the whole call, per format (the reference's own
+    writers are timed on the same frame by the CPU-baseline leg)."""
     from sirius_amd import abi
     from sirius_amd.accelerator import default_opts
     cam, metric = workload()
     acc.initialise(W, H)
     acc.launch(cam, metric, default_opts(abi.PRECISION_FP32_ESCALATE))
-    frame = acc.getFrameBuffer().reshape(H, W, 4).copy()
     out = {"frame": "1920x1080 RGBA float (the Kerr a=0.9 benchmark frame, shaded)"}
     fmts = (("png", abi.IMAGE_PNG), ("exr", abi.IMAGE_EXR), ("ppm", abi.IMAGE_PPM))
     import tempfile
@@ -232,25 +259,36 @@ def image_writers(acc, steps):
         out[name] = {"total_ms": mean("total_ms"), "convert_ms": conv, "copy_ms": mean("copy_ms"),
                      "pack_ms": mean("pack_ms"), "file_bytes": int(sts[-1].file_bytes),
                      "convert_GBps": IMAGE_BYTES_PER_PIXEL[name] * W * H / (conv * 1e-3) / 1e9}
-    try:
-        from oracle import ref_py as RF
-        if os.path.exists(RF.IMAGE_LIB_PATH):
-            for name, fn in (("png", RF.write_png), ("exr", RF.write_exr),
-                             ("ppm", lambda f: RF.write_ppm_aces(f, 1.0))):
-                t0 = time.perf_counter()
-                fn(frame)
-                ms = (time.perf_counter() - t0) * 1e3
-                out[name]["reference_cpu_ms"] = ms
-                out[name]["speedup_vs_reference_cpu"] = ms / out[name]["total_ms"]
-    except Exception as e:  # the reference build is optional on a machine
-        out["reference_cpu"] = f"unavailable: {e}"
+    return out
+
+
+def reference_writers(acc, writers):
+    """CPU baseline of the image writers: the reference's own PNGWriter::writeRGBA,
+    EXRWriter::writeEXR and BufferWriter::writePPM (oracle/_ref, compiled in place) on the shaded
+    1080p frame, one host core each, beside this backend's whole call."""
+    from oracle import ref_py as RF
+    from sirius_amd import abi
+    from sirius_amd.accelerator import default_opts
+    if not os.path.exists(RF.IMAGE_LIB_PATH):
+        return {"unavailable": "oracle/_ref image library not built"}
+    cam, metric = workload()
+    acc.initialise(W, H)
+    acc.launch(cam, metric, default_opts(abi.PRECISION_FP32_ESCALATE))
+    frame = acc.getFrameBuffer().reshape(H, W, 4).copy()
+    out = {}
+    for name, fn in (("png", RF.write_png), ("exr", RF.write_exr), ("ppm", lambda f: RF.write_ppm_aces(f, 1.0))):
+        t0 = time.perf_counter()
+        fn(frame)
+        ms = (time.perf_counter() - t0) * 1e3
+        out[name] = {"reference_cpu_ms": ms}
+        if name in writers:
+            out[name]["speedup_of_backend"] = ms / writers[name]["total_ms"]
     return out
 
 
 def surrogate_ablation(acc, cam, metric, ref, steps, warmup):
     """Train the network once, then time the frame with the surrogate on and measure the
     answered rays against the FP64 frame."""
-    from oracle import oracle_py as O
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from workloads import MT19937_64
     from sirius_amd import abi
@@ -261,7 +299,7 @@ def surrogate_ablation(acc, cam, metric, ref, steps, warmup):
         a, incl, r_obs = g.uniform(0.0, 0.99), g.uniform(5.0, 89.0), g.uniform(20.0, 100.0)
         metrics.append(abi.metric_ks(1.0, a))
         cams.append(abi.camera(r_obs, incl, 20.0, 64, 64, disk=(0.0, 30.0), metric_index=i))
-    lab = O.default_opts(abi.PRECISION_FP64)
+    lab = default_opts(abi.PRECISION_FP64)
     lab.far_slope = 0.25
     acc.mlp_init(0x51121006)
     t0 = time.perf_counter()
@@ -286,6 +324,45 @@ def surrogate_ablation(acc, cam, metric, ref, steps, warmup):
             out["escape_dir_err_median_rad"] = float(np.median(ang))
             out["escape_dir_err_max_rad"] = float(ang.max())
     return out
+
+
+def symplectic_leg(acc, cam, metric):
+    """The frame by the fixed-step symplectic integrator (Tao's extended phase space, Strang
+    split, Yoshida-4 composition, h = 0.01 M, FP64), one frame after one warm-up: Mrays/s,
+    geodesic steps/s and the null-constraint drift of the final states (max and median). Escaped
+    rays run to r = 1000 M, about 1e5 fixed steps each."""
+    from sirius_amd import abi
+    from sirius_amd.accelerator import default_opts
+    o = default_opts(abi.PRECISION_FP64)
+    o.integrator = abi.INTEGRATOR_SYMPLECTIC
+    o.symplectic_h = 0.01
+    o.symplectic_order = 4
+    o.max_steps = 400000
+    acc.trace_resident([cam], [metric], o, stats=False)
+    acc.synchronise()
+    t0 = time.perf_counter()
+    acc.trace_resident([cam], [metric], o, stats=False)
+    acc.synchronise()
+    dt = time.perf_counter() - t0
+    st = acc.trace_resident([cam], [metric], o)
+    got = acc.fetch_results()
+    n = W * H
+    return {"integrator": "Tao extended phase space (omega 20), Strang split, Yoshida-4, fixed h = 0.01 M, FP64",
+            "ms_per_step": dt * 1e3, "mrays_per_s": n / dt / 1e6,
+            "geodesic_steps_per_s": st["steps_bulk"] / dt, "steps_per_ray": st["steps_bulk"] / n,
+            "kernel_ms": st["kernel_ms"], "achieved_tflops": st["flops"] / (st["kernel_ms"] * 1e-3) / 1e12,
+            "flops_per_step": st["flops"] / max(st["steps_bulk"], 1),
+            "null_drift_max": st["null_drift_max"], "null_drift_median": st["null_drift_median"],
+            "class_counts": {k: int((got.cls == v).sum()) for k, v in
+                             (("escaped", 0), ("captured", 1), ("disk", 2), ("max_steps", 3), ("failed", 5))}}
+
+
+def cpu_threads():
+    """Host threads for the CPU baseline: the cores this process may run on, within the
+    OMP_NUM_THREADS share the machine allots (a GPU box gives one GPU 16 host cores)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(share))) if share.isdigit() and int(share) > 0 else n
 
 
 def cpu_baseline(n_threads):
@@ -322,6 +399,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-symplectic", action="store_true", help="skip the fixed-step symplectic frame")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and non-Kerr workloads")
     ap.add_argument("--no-tune", action="store_true",
                     help="default launch shapes instead of tuned ones for the headline frame")
@@ -393,12 +471,27 @@ def main():
                        "achieved_tflops": s64["flops"] / (k64 * 1e-3) / 1e12,
                        "frac_of_fp64_peak": s64["flops"] / (k64 * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS},
         "speedup_vs_fp64_plain": ms64 / ms32,
+        "null_drift_max": s["null_drift_max"],
+        "null_drift_median": s["null_drift_median"],
         "launch": {"fp32": {"block_size": o32.block_size, "blocks_per_cu": o32.blocks_per_cu,
                             "refill_min": o32.refill_min},
                    "fp64": {"block_size": o64.block_size},
                    "tuned": not args.no_tune},
         "steps_per_ray": s["steps_bulk"] / n_rays,
     }
+    # the plain FP64 tracer at the FP32 path's step policy (its rtol, atol, far-field cap and
+    # inward step proposal): the share of the speed-up that comes from precision alone
+    o64p = default_opts(abi.PRECISION_FP64)
+    for k in ("rtol", "atol", "far_slope", "step_radial_exp", "block_size"):
+        setattr(o64p, k, getattr(o32 if k != "block_size" else o64, k))
+    ms64p, st64p = time_frames(acc, cam, metric, o64p, max(1, args.steps // 2), 1)
+    result["fp64_same_step_policy"] = {
+        "value": n_rays / ms64p / 1e6, "ms_per_step": ms64p * 1e3,
+        "steps_per_ray": st64p[-1]["steps_bulk"] / n_rays, "speedup_of_fp32": ms64p / ms32,
+        "policy": {"rtol": o64p.rtol, "atol": o64p.atol, "far_slope": o64p.far_slope,
+                   "step_radial_exp": o64p.step_radial_exp}}
+    if not args.no_symplectic:
+        result["symplectic"] = symplectic_leg(acc, cam, metric)
     if not args.no_accuracy:
         crit = acc.critical_distance([cam], [metric])
         result["accuracy_vs_fp64"] = accuracy(fast, ref, crit)
@@ -417,7 +510,10 @@ def main():
     if not args.no_surrogate and ref is not None:
         result["surrogate"] = surrogate_ablation(acc, cam, metric, ref, args.steps, args.warmup)
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(min(16, os.cpu_count() or 1))
+        result["cpu_baseline"] = cpu_baseline(cpu_threads())
+        if "other_workloads" in result:
+            result["cpu_baseline"]["image_writers"] = reference_writers(
+                acc, result["other_workloads"]["image_writers_1080p"])
     acc.close()
     print(json.dumps(result))
 

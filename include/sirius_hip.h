@@ -152,6 +152,16 @@ typedef struct SiriusHipTraceOpts {
                               * (r1 < r0): h_next = h fac(err) (r1 / r0)^exp, anticipating the
                               * local error's growth toward the hole. 0 = off (the reference's
                               * controller, the FP64 default); FP32 default 2              */
+    double surrogate_err_max;/* surrogate: answer a ray only where the escape-direction error the
+                              * calibration measured on held-out rays of its bin is at most this
+                              * (rad); default 5e-5                                            */
+    double surrogate_margin; /* surrogate: the analytic candidate test refuses a ray whose
+                              * reference orbit crosses the equatorial plane within this distance
+                              * (M) of the thin disk's [r_in, r_out]; default 2.5              */
+    double precision_audit;  /* FP32 path: this fraction of the traced rays (seeded by audit_seed)
+                              * is re-traced after the batch by the plain FP64 integrator at the
+                              * FP64 defaults' tolerances and compared: the stats' precision_*
+                              * fields estimate the call's error against FP64. 0 = off (default) */
 } SiriusHipTraceOpts;
 
 /* Structure-of-arrays result buffers. Every array holds n_rays entries, except intensity
@@ -200,6 +210,15 @@ typedef struct SiriusHipStats {
     double audit_dir_err_max;    /* largest escape-direction error of audited answers (rad) */
     uint64_t audit_class_mismatch; /* audited answers whose class the integrator disputes    */
     double surrogate_flops;      /* algorithmic flops of the network inference               */
+    uint64_t rays_candidates;    /* rays the surrogate's analytic test passed to the network  */
+    uint64_t rays_gated;         /* candidates the calibrated error gate left to the integrator */
+    uint64_t precision_audited;  /* rays of the FP64 precision audit (precision_audit)        */
+    uint64_t precision_class_mismatch; /* audited rays whose FP32 class differs from FP64's    */
+    double precision_dir_err_max;      /* escape-direction error of the audited rays (rad):   */
+    double precision_dir_err_median;   /* largest and median, both classes escaped            */
+    double precision_int_err_max;      /* relative intensity error of audited disk rays (the  */
+    double precision_int_err_median;   /* largest bin error over the pixel's brightest bin)   */
+    double precision_audit_ms;   /* HIP-event time of the precision audit                    */
 } SiriusHipStats;
 
 typedef void* SiriusHipHandle;
@@ -264,19 +283,31 @@ int sirius_hip_autotune(SiriusHipHandle h, const SiriusHipCamera* cams, int n_ca
 
 /* ---- MLP far-field surrogate (new) ------------------------------------------------------ */
 /* Network: 5 inputs -> 4 x 256 GELU (tanh form) -> 6 outputs: 3 class logits (escaped,
- * captured, disk) and the escape direction residual d_esc - n_0 (n_0: the ray's initial
- * Cartesian direction at the camera). Inputs: a, theta_obs / (pi/2), (r_obs - 60) / 40,
- * alpha / 15, beta / 15 (alpha, beta: the ray's impact parameters on the observer's sky, M).
+ * captured, disk) and the escape direction's residual over an analytic reference,
+ * (d_esc - d_ref) / SIRIUS_MLP_DIR_SCALE. d_ref is the exact exit direction (at r_escape) of the
+ * Schwarzschild ray with the same initial coordinate direction starting from the initial
+ * position displaced by 1.1 a e_phi (e_phi: the observer's azimuthal unit vector; to first order
+ * in a, frame dragging acts as such a displacement of the lens): at a = Q = Lambda = 0 the
+ * integrator's own answer, so the network learns what spin beyond that, charge and Lambda add. Inputs: a, theta_obs / (pi/2), (r_obs - 60) / 40, alpha / 15, beta / 15 (alpha,
+ * beta: the ray's impact parameters on the observer's sky, M).
+ * The network is asked only about candidates: rays outside surrogate_band of the critical curve
+ * and outside the shadow whose reference orbit scatters and crosses the equatorial plane only
+ * outside the thin disk widened by surrogate_margin (the reference's own test, analytic). A
+ * candidate is answered when its most probable class is escaped with probability >=
+ * surrogate_pmin and the calibration of its bin (spin x critical distance x observer radius)
+ * measured, on held-out rays, no misclassification and a largest direction error <=
+ * surrogate_err_max; every other ray is integrated.
  * Parameters are FP32 masters on the device, laid out layer by layer as W_l (out x in,
- * row-major) then b_l; forward and backward run on BF16 MFMA with FP32 accumulation (GEMM
- * operands rounded to BF16: activations, weights, and the output/pre-activation gradients).
- * Loss per sample: softmax cross-entropy + SIRIUS_MLP_DIR_WEIGHT * |residual error|^2 for
- * escaped rays; the batch loss is the mean. */
+ * row-major) then b_l. The hidden layers run on BF16 MFMA with FP32 accumulation (operands
+ * rounded to BF16: activations, weights and pre-activation gradients); the output head (256 ->
+ * 6) is FP32. Loss per sample: softmax cross-entropy + SIRIUS_MLP_DIR_WEIGHT * |residual
+ * error|^2 for escaped rays; the batch loss is the mean. */
 #define SIRIUS_MLP_IN 5
 #define SIRIUS_MLP_HIDDEN 256
 #define SIRIUS_MLP_LAYERS 4
 #define SIRIUS_MLP_OUT 6
 #define SIRIUS_MLP_DIR_WEIGHT 100.0
+#define SIRIUS_MLP_DIR_SCALE 0.005
 size_t sirius_hip_mlp_param_count(void);
 /* Deterministic initialisation from a seed (uniform +-sqrt(6 / fan_in) weights, zero biases). */
 int sirius_hip_mlp_init(SiriusHipHandle h, uint64_t seed);
@@ -293,19 +324,29 @@ int sirius_hip_mlp_loss_grad(SiriusHipHandle h, const float* x, const int32_t* c
  * receives the batch loss before the update. */
 int sirius_hip_mlp_train_step(SiriusHipHandle h, const float* x, const int32_t* cls,
                               const float* dir, int n, float lr, float* loss);
-/* Network inputs x[n*5] and initial Cartesian directions n0[n*3] of every ray of the
- * cameras (device buffers; Kerr-Schild cameras). */
+/* Network inputs x[n*5] and reference exit directions ref[n*3] of every ray of the cameras
+ * (device buffers; Kerr-Schild cameras; ref is zero for rays the candidate test refuses, with
+ * the FP32 defaults' surrogate_band, surrogate_margin and r_escape). */
 int sirius_hip_surrogate_features(SiriusHipHandle h, const SiriusHipCamera* cams, int n_cam,
-                                  const SiriusHipMetric* metrics, int n_metric, float* x, float* n0);
-/* Trains the network on every ray of the cameras, labelled by the integrator with label_opts
- * (FP64): inputs and targets are built on the device, then `epochs` passes of Adam over
- * batches of `batch` rays in a seeded random order, at learning rate lr with a cosine decay
- * to lr/20. *loss (host) receives the mean batch loss of the last epoch. Uses the current
- * parameters (sirius_hip_mlp_init first for a fresh network). */
+                                  const SiriusHipMetric* metrics, int n_metric, float* x, float* ref);
+/* Trains the network on the candidate rays of the cameras, labelled by the integrator with
+ * label_opts (FP64; its surrogate_band, surrogate_margin and r_escape define the candidates):
+ * inputs and targets are built on the device; a seeded 1 in 16 of the candidates is held out,
+ * the rest trained on for `epochs` passes of Adam over batches of `batch` rays in a seeded
+ * random order, at learning rate lr with a cosine decay to lr/20; then the held-out rays
+ * calibrate the error gate (label_opts' surrogate_pmin). *loss (host) receives the mean batch
+ * loss of the last epoch. Uses the current parameters (sirius_hip_mlp_init first for a fresh
+ * network). */
 int sirius_hip_surrogate_train(SiriusHipHandle h, const SiriusHipCamera* cams, int n_cam,
                                const SiriusHipMetric* metrics, int n_metric,
                                const SiriusHipTraceOpts* label_opts, int epochs, int batch,
                                float lr, uint64_t seed, float* loss);
+/* The error gate's calibration, per bin (*n_bins of them; capacity must be at least that):
+ * held-out rays the network would answer, how many of them it misclassified, and the largest
+ * escape-direction error among the rest (rad). All zero before sirius_hip_surrogate_train, and
+ * after new parameters (an uncalibrated network answers nothing). */
+int sirius_hip_surrogate_calibration(SiriusHipHandle h, float* max_err, uint32_t* answered,
+                                     uint32_t* misclassified, int capacity, int* n_bins);
 
 /* ---- procedural star field (celestial spheres) ---------------------------------------- */
 /* One catalogue star, field for field the reference's StarEntry

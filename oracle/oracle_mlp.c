@@ -3,9 +3,11 @@
  *
  * TEST INFRASTRUCTURE ONLY (see sirius_oracle.h). The reference has no network: the contract
  * is sirius_hip.h's (5 -> 4 x 256 GELU -> 6, layer-by-layer W_l then b_l, cross-entropy plus
- * weighted residual error). With round_bf16 the GEMM operands are rounded to BF16 where the
- * device rounds them: the activations h_l, the weights W_l, the output gradient and the
- * pre-activation gradients; biases, pre-activations and bias gradients stay unrounded.
+ * weighted residual error). With round_bf16 the GEMM operands of the hidden layers are rounded
+ * to BF16 where the device rounds them: their input activations, their weights and the
+ * pre-activation gradients; biases, pre-activations and bias gradients stay unrounded, and so
+ * does the output head (the last hidden activation, the head's weights and the output
+ * gradient), which the device keeps in FP32.
  * Pins: tests/test_oracle_mlp.py (finite differences of the FP64 loss).
  */
 #include <math.h>
@@ -74,11 +76,12 @@ static void forward1(const layout* L, const float* x, double h[NL][SIRIUS_MLP_HI
         for (int o = 0; o < DIM_OUT[l]; ++o) {
             double acc = 0;
             const float* w = L->W[l] + (size_t)o * DIM_IN[l];
-            for (int k = 0; k < DIM_IN[l]; ++k) acc += bf(w[k], rb) * h[l][k];
+            const int rl = rb && l < NL - 1;
+            for (int k = 0; k < DIM_IN[l]; ++k) acc += bf(w[k], rl) * h[l][k];
             acc += L->b[l][o];
             if (l < NL - 1) {
                 z[l][o] = acc;
-                h[l + 1][o] = bf(gelu(acc), rb);
+                h[l + 1][o] = bf(gelu(acc), rb && l + 1 < NL - 1);
             } else {
                 y[o] = acc;
             }
@@ -134,7 +137,8 @@ double oracle_mlp_loss_grad(const float* params, const float* x, const int32_t* 
             for (int o = 0; o < SIRIUS_MLP_OUT; ++o) delta[o] = gy[o] / n;
             for (int l = NL - 1; l >= 0; --l) {
                 int no = DIM_OUT[l], ni = DIM_IN[l];
-                for (int o = 0; o < no; ++o) dr[o] = bf(delta[o], rb);
+                const int rl = rb && l < NL - 1;
+                for (int o = 0; o < no; ++o) dr[o] = bf(delta[o], rl);
                 for (int o = 0; o < no; ++o) {
                     double* gw = g + L.offW[l] + (size_t)o * ni;
                     for (int k = 0; k < ni; ++k) gw[k] += dr[o] * h[l][k];
@@ -143,7 +147,7 @@ double oracle_mlp_loss_grad(const float* params, const float* x, const int32_t* 
                 if (l == 0) break;
                 for (int k = 0; k < ni; ++k) {
                     double acc = 0;
-                    for (int o = 0; o < no; ++o) acc += dr[o] * bf(L.W[l][(size_t)o * ni + k], rb);
+                    for (int o = 0; o < no; ++o) acc += dr[o] * bf(L.W[l][(size_t)o * ni + k], rl);
                     gh[k] = acc;
                 }
                 for (int k = 0; k < ni; ++k) delta[k] = gh[k] * gelu_d(z[l - 1][k]);

@@ -330,6 +330,9 @@ void oracle_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->escalate_rtol = 0.0;
     o->critical_band = 0.5;
     o->step_radial_exp = 0.0;
+    o->surrogate_err_max = 5e-5;
+    o->surrogate_margin = 2.5;
+    o->precision_audit = 0.0;
     if (precision == SIRIUS_PRECISION_FP32_ESCALATE) {
         o->rtol = 1e-6;
         o->atol = 1e-8;
@@ -1080,15 +1083,23 @@ static void trace_symplectic(const cam_ctx* cx, const SiriusHipTraceOpts* o, con
         double r = oracle_kerr_radius(m, q[1], q[2], q[3]);
         if (m->family == SIRIUS_METRIC_KERR_SCHILD && o->emission == SIRIUS_EMISSION_THIN_DISK &&
             zprev * q[3] < 0) {
-            double a = -zprev / (q[3] - zprev);
-            double xc = q0[1] + a * (q[1] - q0[1]), yc = q0[2] + a * (q[2] - q0[2]);
-            double rd = sqrt(fmax(xc * xc + yc * yc - m->a * m->a, 0.0));
+            /* the crossing is located on the cubic Hermite interpolant of the whole state over
+             * the step (the derivatives of H at both ends), and the ray ends there: a linear
+             * interpolant would leave an O(h^2) null-constraint error of its own at the hit */
+            st8 s0, s1, sc;
+            der8 d0, d1;
+            for (int i = 0; i < 4; ++i) { s0.y[i] = q0[i]; s0.y[4 + i] = p0[i]; s1.y[i] = q[i]; s1.y[4 + i] = p[i]; }
+            eval_der(m, &s0, &d0);
+            eval_der(m, &s1, &d1);
+            double th = locate_event(m, EV_Z, 0, &s0, &d0, &s1, &d1, h);
+            hermite8(&s0, &d0, &s1, &d1, h, th, sc.y);
+            double rd = sqrt(fmax(sc.y[1] * sc.y[1] + sc.y[2] * sc.y[2] - m->a * m->a, 0.0));
             if (rd >= cx->r_in && rd <= cx->r_out) {
-                st8 s;
-                for (int i = 0; i < 4; ++i) { s.y[i] = q0[i] + a * (q[i] - q0[i]); s.y[4 + i] = p0[i] + a * (p[i] - p0[i]); }
+                for (int i = 0; i < 4; ++i) { q[i] = sc.y[i]; p[i] = sc.y[4 + i]; }
+                lam += (th - 1) * h;
                 R->cls = SIRIUS_RAY_DISK;
                 R->disk_r = rd;
-                disk_emission(cx, o, &s, rd, &R->g, R->I);
+                disk_emission(cx, o, &sc, rd, &R->g, R->I);
                 break;
             }
         }

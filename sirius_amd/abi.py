@@ -66,6 +66,7 @@ class TraceOpts(C.Structure):
         ("vol_r_min", C.c_double), ("vol_r_max", C.c_double), ("vol_tau_max", C.c_double),
         ("escalate_disk_edge", C.c_double), ("escalate_rtol", C.c_double), ("critical_band", C.c_double),
         ("step_radial_exp", C.c_double),
+        ("surrogate_err_max", C.c_double), ("surrogate_margin", C.c_double), ("precision_audit", C.c_double),
     ]
 
 
@@ -90,6 +91,11 @@ class Stats(C.Structure):
         ("null_drift_max", C.c_double), ("null_drift_median", C.c_double),
         ("audit_ms", C.c_double), ("audit_dir_err_max", C.c_double), ("audit_class_mismatch", C.c_uint64),
         ("surrogate_flops", C.c_double),
+        ("rays_candidates", C.c_uint64), ("rays_gated", C.c_uint64),
+        ("precision_audited", C.c_uint64), ("precision_class_mismatch", C.c_uint64),
+        ("precision_dir_err_max", C.c_double), ("precision_dir_err_median", C.c_double),
+        ("precision_int_err_max", C.c_double), ("precision_int_err_median", C.c_double),
+        ("precision_audit_ms", C.c_double),
     ]
 
     def as_dict(self):

@@ -62,6 +62,42 @@ hipError_t launch_adam(float* params, const float* grad, float* m, float* v, int
                        hipStream_t st);
 hipError_t launch_pack(const float* params, uint16_t* const* w, float* const* b, hipStream_t st);
 
+// Fused inference (k_mlp_fused): the whole network over rows of x in one kernel.
+struct MlpFusedArgs {
+    const float* x;        // inputs: row r at x[r * x_stride] (x[idx[r] * x_stride] when idx is set)
+    int64_t n;             // rows (when n_dev is null)
+    const uint32_t* n_dev; // rows, read on the device (null: n)
+    const int32_t* idx;    // optional gather list
+    int64_t x_stride;      // floats per input record (5 for packed inputs)
+    const uint16_t* w0;    // [256][16] BF16, input columns padded to 16
+    const uint16_t* w[3];  // layers 1..3, [256][256] BF16
+    const float* b[4];     // biases of layers 0..3, [256]
+    const uint16_t* w4h;   // head weights [32][256], BF16 high parts (rows 6..31 zero)
+    const uint16_t* w4l;   // and low parts
+    const float* b4;       // head bias [6]
+    float* y;              // n x 6 outputs
+};
+hipError_t launch_mlp_fused(const MlpFusedArgs& a, int64_t cap, int max_blocks, hipStream_t st);
+
+// Surrogate records (one per candidate ray, by ray id): network inputs (5), signed distance from
+// the critical curve, the reference orbit's impact parameter, padding, the reference exit
+// direction (3), padding. Calibration bins of the error gate, and the held-out rays a bin needs.
+constexpr int SUR_REC = 12;
+// The reference orbit of a ray starts at x0 + SUR_SPIN_SHIFT a e_phi (e_phi: the observer's
+// azimuthal direction): the lens displaced as frame dragging displaces it to first order in a.
+constexpr double SUR_SPIN_SHIFT = 1.1;
+constexpr int SUR_BINS = 270;
+constexpr uint32_t SUR_CAL_MIN = 20;
+// FP32 output head of the training passes (z3: the last hidden pre-activations [n_pad][256])
+hipError_t launch_head_fwd(const float* z3, const float* W4, const float* b4, int64_t n_pad, float* y64,
+                           hipStream_t st);
+hipError_t launch_head_bwd(const float* gy64, const float* W4, const float* z3, int64_t n_pad, float* gzf,
+                           uint16_t* gzb, hipStream_t st);
+hipError_t launch_head_wgrad(const float* gy64, const float* z3, int64_t n_pad, float* part, int nsplit, float* dW4,
+                             hipStream_t st);
+hipError_t launch_pack_fused(const float* params, uint16_t* w0, uint16_t* w4h, uint16_t* w4l, size_t off_w0,
+                             size_t off_w4, hipStream_t st);
+
 // Device state of the network of one handle.
 struct MlpState {
     float* params = nullptr;  // FP32 masters, public layout
@@ -71,6 +107,15 @@ struct MlpState {
     int64_t step = 0;
     uint16_t* w[MLP_NL] = {};  // padded BF16 weights, W_l[NP][KP]
     float* b[MLP_NL] = {};     // padded biases [NP]
+    uint16_t* fw0 = nullptr;   // fused inference: W0 [256][16], head W_hi / W_lo [32][256]
+    uint16_t* fw4h = nullptr;
+    uint16_t* fw4l = nullptr;
+    // calibration of the error gate (SUR_BINS each): held-out rays the network would answer,
+    // misclassified ones, and the largest direction error (rad); all zero until calibrated, so
+    // an uncalibrated network answers nothing
+    float* cal_err = nullptr;
+    uint32_t* cal_n = nullptr;
+    uint32_t* cal_bad = nullptr;
     bool ready = false;
 };
 

@@ -24,9 +24,12 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
     return (uint16_t)(u >> 16);
 }
 
+// GELU, tanh form: 0.5 z (1 + tanh(u)) = z / (1 + e^{-2u}), u = 0.7978845608 (z + 0.044715 z^3),
+// with the hardware base-2 exponential and reciprocal (the same function in training and
+// inference)
 __device__ __forceinline__ float gelu_f(float z) {
-    float t = tanhf(0.7978845608028654f * (z + 0.044715f * z * z * z));
-    return 0.5f * z * (1.f + t);
+    const float u2 = z * (2.3022081983f + 0.1029432617f * z * z);  // 2u log2(e)
+    return z * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-u2));
 }
 __device__ __forceinline__ float gelu_df(float z) {
     float t = tanhf(0.7978845608028654f * (z + 0.044715f * z * z * z));
@@ -109,6 +112,283 @@ hipError_t launch_gemm(int epi, const GemmArgs& g, int nsplit, hipStream_t st) {
         case EPI_BWD: hipLaunchKernelGGL(k_gemm<EPI_BWD>, grid, block, 0, st, g); break;
         default: hipLaunchKernelGGL(k_gemm<EPI_PART>, grid, block, 0, st, g); break;
     }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
+// Fused inference: the whole network for a tile of 128 rows in one kernel, with gfx950's
+// v_mfma_f32_32x32x16_bf16 (A 32 x 16 and B 16 x 32 BF16 operands, 8 values per lane: lane
+// l holds row / column l % 32 and k = 8 (l / 32) .. +7; C 32 x 32 FP32, 16 values per lane:
+// column l % 32, rows 8 (r / 4) + 4 (l / 32) + r % 4).
+//  - The tile's activations stay in LDS ([128][264] BF16, rows padded by 8 values so that the
+//    16-byte row reads of eight consecutive lanes cover every bank once): each layer's MFMAs read
+//    their A operand (rows x k) there, and the bias + GELU epilogue writes the layer's BF16
+//    output back in place, between two barriers.
+//  - Wave w computes output neurons 64 w .. 64 w + 63 of all 128 rows (4 x 2 tiles of 32 x 32,
+//    128 accumulator registers); its B operand (weights W[j][k], 16 contiguous bytes per lane)
+//    streams from L2 through the vector cache, one k-step ahead of the MFMAs.
+//  - The output head (256 -> 6) keeps FP32 precision with three BF16 products: the last hidden
+//    activation h = hi + lo and the head's weights W = W_hi + W_lo, each split into BF16 parts,
+//    and y = hi W_hi + hi W_lo + lo W_hi (the lo W_lo term is below FP32's resolution of y).
+//    hi, then lo, pass through the same LDS buffer.
+// ------------------------------------------------------------------------------------------
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+constexpr int MF_ROWS = 128;
+constexpr int MF_PITCH = 264;
+
+__device__ __forceinline__ v8bf ld8(const uint16_t* p) { return *(const v8bf*)p; }
+
+__global__ __launch_bounds__(256, 2) void k_mlp_fused(MlpFusedArgs a) {
+    __shared__ __attribute__((aligned(16))) uint16_t act[MF_ROWS * MF_PITCH];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int li = lane & 31, lh = lane >> 5;
+    const int64_t n = a.n_dev ? (int64_t)*a.n_dev : a.n;
+    // tiles of 128 rows, grid-stride (the row count may live on the device)
+    for (int64_t row0 = (int64_t)blockIdx.x * MF_ROWS; row0 < n; row0 += (int64_t)gridDim.x * MF_ROWS) {
+    __syncthreads();  // the previous tile's last reads of act are done
+    // layer-0 input: x padded to 16 columns (row r of the tile is x[idx[r]] when gathered)
+    for (int e = t; e < MF_ROWS * 16; e += 256) {
+        const int r = e >> 4, k = e & 15;
+        const int64_t gr = row0 + r;
+        float v = 0.f;
+        if (k < SIRIUS_MLP_IN && gr < n) v = a.x[(a.idx ? (int64_t)a.idx[gr] : gr) * a.x_stride + k];
+        act[r * MF_PITCH + k] = f2bf(v);
+    }
+    __syncthreads();
+    const int col0 = wave * 64;
+    v16f acc[4][2];
+    uint32_t lo_pk[4][2][8];  // low BF16 parts of the last hidden activation
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[mt][nt][r] = 0.f;
+    {
+        v8bf b[2];
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) b[nt] = ld8(a.w0 + (col0 + nt * 32 + li) * 16 + 8 * lh);
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            const v8bf av = ld8(&act[(mt * 32 + li) * MF_PITCH + 8 * lh]);
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b[nt], acc[mt][nt], 0, 0, 0);
+        }
+    }
+    for (int l = 0; l < 4; ++l) {
+        if (l > 0) {
+            const uint16_t* W = a.w[l - 1];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) acc[mt][nt][r] = 0.f;
+            v8bf b[2], bn[2];
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) b[nt] = ld8(W + (col0 + nt * 32 + li) * MLP_H + 8 * lh);
+#pragma unroll
+            for (int ks = 0; ks < MLP_H / 16; ++ks) {
+                const int k0 = ks * 16;
+                if (ks + 1 < MLP_H / 16) {
+#pragma unroll
+                    for (int nt = 0; nt < 2; ++nt) bn[nt] = ld8(W + (col0 + nt * 32 + li) * MLP_H + k0 + 16 + 8 * lh);
+                }
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt) {
+                    const v8bf av = ld8(&act[(mt * 32 + li) * MF_PITCH + k0 + 8 * lh]);
+#pragma unroll
+                    for (int nt = 0; nt < 2; ++nt)
+                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b[nt], acc[mt][nt], 0, 0, 0);
+                }
+                if (ks + 1 < MLP_H / 16) {
+#pragma unroll
+                    for (int nt = 0; nt < 2; ++nt) b[nt] = bn[nt];
+                }
+            }
+        }
+        __syncthreads();  // every wave has read this layer's input
+        // bias + GELU; the hidden layers' BF16 outputs (the last one's high parts) back into LDS,
+        // the last one's low parts kept in registers, two BF16 values to a register
+        const float* bias = a.b[l];
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const int j = col0 + nt * 32 + li;
+            const float bj = bias[j];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const float h = gelu_f(acc[mt][nt][r] + bj);
+                    const uint16_t hi = f2bf(h);
+                    act[(mt * 32 + 8 * (r >> 2) + 4 * lh + (r & 3)) * MF_PITCH + j] = hi;
+                    if (l == 3) {
+                        const uint32_t lo = f2bf(h - __uint_as_float((uint32_t)hi << 16));
+                        lo_pk[mt][nt][r >> 1] = (r & 1) ? (lo_pk[mt][nt][r >> 1] | (lo << 16)) : lo;
+                    }
+                }
+        }
+        __syncthreads();
+    }
+    // output head, rows 32 w .. 32 w + 31 of the tile: hi W_hi + hi W_lo, then lo W_hi
+    v16f y;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) y[r] = 0.f;
+#pragma unroll
+    for (int ks = 0; ks < MLP_H / 16; ++ks) {
+        const int k0 = ks * 16;
+        const v8bf av = ld8(&act[(wave * 32 + li) * MF_PITCH + k0 + 8 * lh]);
+        y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, ld8(a.w4h + li * MLP_H + k0 + 8 * lh), y, 0, 0, 0);
+        y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, ld8(a.w4l + li * MLP_H + k0 + 8 * lh), y, 0, 0, 0);
+    }
+    __syncthreads();  // every wave has read the high parts
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        const int j = col0 + nt * 32 + li;
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                act[(mt * 32 + 8 * (r >> 2) + 4 * lh + (r & 3)) * MF_PITCH + j] =
+                    (uint16_t)(lo_pk[mt][nt][r >> 1] >> (16 * (r & 1)));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < MLP_H / 16; ++ks) {
+        const int k0 = ks * 16;
+        const v8bf av = ld8(&act[(wave * 32 + li) * MF_PITCH + k0 + 8 * lh]);
+        y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, ld8(a.w4h + li * MLP_H + k0 + 8 * lh), y, 0, 0, 0);
+    }
+    if (li < SIRIUS_MLP_OUT) {
+        const float bo = a.b4[li];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int64_t gr = row0 + wave * 32 + 8 * (r >> 2) + 4 * lh + (r & 3);
+            if (gr < n) a.y[gr * SIRIUS_MLP_OUT + li] = y[r] + bo;
+        }
+    }
+    }
+}
+
+// Grid: one block per tile up to `max_blocks` (two blocks fit a CU), each looping over tiles.
+hipError_t launch_mlp_fused(const MlpFusedArgs& a, int64_t cap, int max_blocks, hipStream_t st) {
+    if (cap <= 0) return hipSuccess;
+    int64_t blocks = (cap + MF_ROWS - 1) / MF_ROWS;
+    if (blocks > max_blocks) blocks = max_blocks;
+    hipLaunchKernelGGL(k_mlp_fused, dim3((unsigned)blocks), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
+// The output head in FP32 for training (the fused inference keeps FP32 precision with split
+// BF16 products): h4 = gelu(z3) from the stored pre-activations, W4 the FP32 masters.
+// ------------------------------------------------------------------------------------------
+// y64[i][o] = b4[o] + sum_k h4[i][k] W4[o][k] (o < 6; columns 6..63 zero). One wave per row
+// group: each lane takes four k, the six partial sums are reduced across the wave.
+__global__ __launch_bounds__(256) void k_head_fwd(const float* __restrict__ z3, const float* __restrict__ W4,
+                                                  const float* __restrict__ b4, int64_t n_pad, float* __restrict__ y64) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n_pad) return;
+    float part[SIRIUS_MLP_OUT] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int k = lane * 4 + q;
+        const float h = gelu_f(z3[i * MLP_H + k]);
+#pragma unroll
+        for (int o = 0; o < SIRIUS_MLP_OUT; ++o) part[o] = fmaf(h, W4[o * MLP_H + k], part[o]);
+    }
+#pragma unroll
+    for (int o = 0; o < SIRIUS_MLP_OUT; ++o)
+        for (int off = 32; off > 0; off >>= 1) part[o] += __shfl_xor(part[o], off);
+    if (lane < MLP_N4) {
+        float v = 0.f;
+#pragma unroll
+        for (int o = 0; o < SIRIUS_MLP_OUT; ++o) v = lane == o ? part[o] + b4[o] : v;
+        y64[i * MLP_N4 + lane] = v;
+    }
+}
+
+// gz3[i][k] = (sum_o gy[i][o] W4[o][k]) gelu'(z3[i][k]), in FP32 and BF16.
+__global__ void k_head_bwd(const float* __restrict__ gy64, const float* __restrict__ W4, const float* __restrict__ z3,
+                           int64_t n_pad, float* __restrict__ gzf, uint16_t* __restrict__ gzb) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_pad * MLP_H) return;
+    const int64_t i = e / MLP_H;
+    const int k = (int)(e % MLP_H);
+    float dh = 0.f;
+#pragma unroll
+    for (int o = 0; o < SIRIUS_MLP_OUT; ++o) dh = fmaf(gy64[i * MLP_N4 + o], W4[o * MLP_H + k], dh);
+    const float g = dh * gelu_df(z3[e]);
+    gzf[e] = g;
+    gzb[e] = f2bf(g);
+}
+
+// dW4[o][k] = sum_i gy[i][o] gelu(z3[i][k]): row chunks into part[split][6][256], then a fixed-
+// order sum over the chunks (deterministic).
+__global__ __launch_bounds__(256) void k_head_wgrad_part(const float* __restrict__ gy64, const float* __restrict__ z3,
+                                                         int64_t n_pad, int64_t chunk, float* __restrict__ part) {
+    const int k = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * chunk, r1 = min(n_pad, r0 + chunk);
+    float acc[SIRIUS_MLP_OUT] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int64_t i = r0; i < r1; ++i) {
+        const float h = gelu_f(z3[i * MLP_H + k]);
+#pragma unroll
+        for (int o = 0; o < SIRIUS_MLP_OUT; ++o) acc[o] = fmaf(gy64[i * MLP_N4 + o], h, acc[o]);
+    }
+#pragma unroll
+    for (int o = 0; o < SIRIUS_MLP_OUT; ++o) part[((int64_t)blockIdx.x * SIRIUS_MLP_OUT + o) * MLP_H + k] = acc[o];
+}
+__global__ void k_head_wgrad_final(const float* __restrict__ part, int nsplit, float* __restrict__ dW4) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= SIRIUS_MLP_OUT * MLP_H) return;
+    float s = 0.f;
+    for (int q = 0; q < nsplit; ++q) s += part[(int64_t)q * SIRIUS_MLP_OUT * MLP_H + e];
+    dW4[e] = s;
+}
+
+hipError_t launch_head_fwd(const float* z3, const float* W4, const float* b4, int64_t n_pad, float* y64,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(k_head_fwd, dim3((unsigned)((n_pad + 3) / 4)), dim3(256), 0, st, z3, W4, b4, n_pad, y64);
+    return hipGetLastError();
+}
+hipError_t launch_head_bwd(const float* gy64, const float* W4, const float* z3, int64_t n_pad, float* gzf,
+                           uint16_t* gzb, hipStream_t st) {
+    const int64_t e = n_pad * MLP_H;
+    hipLaunchKernelGGL(k_head_bwd, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, st, gy64, W4, z3, n_pad, gzf, gzb);
+    return hipGetLastError();
+}
+hipError_t launch_head_wgrad(const float* gy64, const float* z3, int64_t n_pad, float* part, int nsplit, float* dW4,
+                             hipStream_t st) {
+    const int64_t chunk = (n_pad + nsplit - 1) / nsplit;
+    hipLaunchKernelGGL(k_head_wgrad_part, dim3((unsigned)nsplit), dim3(MLP_H), 0, st, gy64, z3, n_pad, chunk, part);
+    hipLaunchKernelGGL(k_head_wgrad_final, dim3((SIRIUS_MLP_OUT * MLP_H + 255) / 256), dim3(256), 0, st, part, nsplit,
+                       dW4);
+    return hipGetLastError();
+}
+
+// Operands of the fused kernel from the FP32 masters: W0 padded to 16 input columns, W1..W3 as
+// they are, the head's weights split into BF16 high and low parts (rows 6..31 zero).
+__global__ void k_pack_fused(const float* __restrict__ P, uint16_t* __restrict__ w0, uint16_t* __restrict__ w4h,
+                             uint16_t* __restrict__ w4l, size_t off_w0, size_t off_w4) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < MLP_H * 16) {
+        const int j = e / 16, k = e % 16;
+        w0[e] = k < SIRIUS_MLP_IN ? f2bf(P[off_w0 + (size_t)j * SIRIUS_MLP_IN + k]) : (uint16_t)0;
+    }
+    if (e < 32 * MLP_H) {
+        const int j = e / MLP_H, k = e % MLP_H;
+        const float w = j < SIRIUS_MLP_OUT ? P[off_w4 + (size_t)j * MLP_H + k] : 0.f;
+        const uint16_t hi = f2bf(w);
+        w4h[e] = hi;
+        w4l[e] = f2bf(w - __uint_as_float((uint32_t)hi << 16));
+    }
+}
+hipError_t launch_pack_fused(const float* params, uint16_t* w0, uint16_t* w4h, uint16_t* w4l, size_t off_w0,
+                             size_t off_w4, hipStream_t st) {
+    hipLaunchKernelGGL(k_pack_fused, dim3((32 * MLP_H + 255) / 256), dim3(256), 0, st, params, w0, w4h, w4l, off_w0,
+                       off_w4);
     return hipGetLastError();
 }
 

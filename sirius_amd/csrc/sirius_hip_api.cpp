@@ -48,6 +48,33 @@ hipError_t launch_shade(const DevOut& o, int64_t n, int width, int height, int n
 hipError_t launch_sur_audit(const CamConst* cams, const TraceParams& tp, const DevOut& oa, const DevOut& o,
                             const int32_t* list, const uint32_t* count, int64_t cap, unsigned int* acc,
                             hipStream_t st);
+hipError_t launch_precision_select(const DevOut& o, int64_t n, double fraction, uint64_t seed, int32_t* list,
+                                   uint32_t* count, hipStream_t st);
+hipError_t launch_precision_compare(const CamConst* cams, const TraceParams& tp, const DevOut& oa, const DevOut& o,
+                                    const int32_t* list, const uint32_t* count, int64_t cap, unsigned int* acc,
+                                    unsigned int* hist, hipStream_t st);
+
+// Trace parameters of the plain FP64 integrator at the FP64 defaults' tolerances and step limits
+// (the contract the audits compare against), for the batch of tp.
+static TraceParams fp64_reference_params(const TraceParams& tp) {
+    SiriusHipTraceOpts d;
+    sirius_hip_default_opts(&d, SIRIUS_PRECISION_FP64);
+    TraceParams t = tp;
+    t.rtol = d.rtol;
+    t.atol = d.atol;
+    t.h_init = d.h_init;
+    t.h_min = d.h_min;
+    t.h_max = d.h_max;
+    t.far_slope = d.far_slope;
+    t.step_rexp = d.step_radial_exp;
+    t.max_steps = d.max_steps;
+    t.escalate = 0;
+    t.gate_flags = 0;
+    t.spectrum32 = 0;
+    t.first_list = nullptr;
+    t.first_count = nullptr;
+    return t;
+}
 
 // ---------------------------------------------------------------------------------------
 // Host-side closed forms (FP64), shared with the camera constant setup.
@@ -236,7 +263,7 @@ void Handle::release() {
     ev0 = ev1 = ev2 = ev3 = ev4 = nullptr;
     delete last;
     last = nullptr;
-    for (hipEvent_t* ev : {&ev_gate, &ev_band0, &ev_band1, &ev_s0, &ev_s1, &ev_a0, &ev_a1}) {
+    for (hipEvent_t* ev : {&ev_gate, &ev_band0, &ev_band1, &ev_s0, &ev_s1, &ev_a0, &ev_a1, &ev_p0, &ev_p1}) {
         if (*ev) (void)hipEventDestroy(*ev);
         *ev = nullptr;
     }
@@ -275,7 +302,7 @@ int Handle::ensure_device() {
     total_mem = prop.totalGlobalMem;
     if ((e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "stream");
     if ((e = hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "stream");
-    for (hipEvent_t* ev : {&ev_gate, &ev_band0, &ev_band1, &ev_s0, &ev_s1, &ev_a0, &ev_a1})
+    for (hipEvent_t* ev : {&ev_gate, &ev_band0, &ev_band1, &ev_s0, &ev_s1, &ev_a0, &ev_a1, &ev_p0, &ev_p1})
         if ((e = hipEventCreate(ev)) != hipSuccess) return hip_fail(e, "event");
     if ((e = hipEventCreate(&ev0)) != hipSuccess) return hip_fail(e, "event");
     if ((e = hipEventCreate(&ev1)) != hipSuccess) return hip_fail(e, "event");
@@ -349,6 +376,10 @@ static int validate_opts(Handle& H, const SiriusHipTraceOpts& o) {
         return H.fail(SIRIUS_HIP_E_UNSUPPORTED, "the surrogate runs on the FP32 persistent DP45 path");
     if (o.use_surrogate && !(o.audit_fraction >= 0 && o.audit_fraction <= 1))
         return H.fail(SIRIUS_HIP_E_INVALID, "audit_fraction must be in [0, 1]");
+    if (o.use_surrogate && !(o.surrogate_err_max >= 0 && o.surrogate_margin >= 0 && o.surrogate_band >= 0))
+        return H.fail(SIRIUS_HIP_E_INVALID, "surrogate_err_max, surrogate_margin and surrogate_band must be >= 0");
+    if (!(o.precision_audit >= 0 && o.precision_audit <= 1))
+        return H.fail(SIRIUS_HIP_E_INVALID, "precision_audit must be in [0, 1]");
     return SIRIUS_HIP_OK;
 }
 
@@ -513,6 +544,14 @@ double flops_per_step(int family) {
     }
 }
 
+// Algorithmic flops of one fixed step of the symplectic integrator: per Strang substep of Tao's
+// extended phase space, 4 right-hand-side evaluations, 4 half-step updates of 7 components
+// (56) and the rotation of the 4 coordinate pairs (72); 3 substeps for Yoshida-4, 1 for order 2.
+double flops_per_symplectic_step(int family, int order) {
+    double rhs = family == 1 ? 25.0 : (family == 2 ? 42.0 : 150.0);
+    return (order >= 4 ? 3.0 : 1.0) * (4.0 * rhs + 56.0 + 72.0);
+}
+
 int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, const int32_t* list_dev,
               const uint32_t* list_count_dev, int64_t list_cap, SiriusHipStats* stats, bool time_it) {
     hipError_t e;
@@ -525,12 +564,14 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
     // work[4] escalation queue head, work[5] unused, work[6..7] escalation attempts (u64),
     // work[8..10] surrogate list counts (A, B, audit), work[11] audit queue head,
     // work[12] band queue head, work[13] unused, work[14..15] band attempts (u64),
-    // work[16..18] gate counts (A, B, L)
+    // work[16..18] gate counts (A, B, L), work[19] surrogate candidates, work[20] candidates the
+    // calibrated error gate refused
     uint32_t* work = (uint32_t*)H.scratch("work", 128, &rc);
     if (rc) return rc;
     unsigned int* audit_acc = nullptr;  // surrogate audit: largest direction error, class mismatches
     if ((e = launch_fill(work, 32, 0u, H.stream)) != hipSuccess) return H.hip_fail(e, "fill");
     H.last_band = false;
+    H.last_audit = false;  // set again below only when this call audits surrogate answers
     if (time_it) (void)hipEventRecord(H.ev0, H.stream);
     if (o.integrator == SIRIUS_INTEGRATOR_SYMPLECTIC) {
         if (list_dev) return H.fail(SIRIUS_HIP_E_UNSUPPORTED, "symplectic integrator with a ray list");
@@ -590,14 +631,17 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
         if (sur) {
             int32_t* la = (int32_t*)H.scratch("gate_a", 4 * (size_t)B.n, &rc);
             list_b = (int32_t*)H.scratch("gate_b", 4 * (size_t)B.n, &rc);
+            int32_t* ll = (int32_t*)H.scratch("gate_l", 4 * (size_t)B.n, &rc);
             list_audit = (int32_t*)H.scratch("sur_audit", 4 * (size_t)B.n, &rc);
             if (rc) return rc;
             if (time_it) (void)hipEventRecord(H.ev_s0, H.stream);
-            if ((rc = surrogate_stage(H, B, o, dout, la, list_b, list_audit, work + 8))) return rc;
+            if ((rc = surrogate_stage(H, B, o, dout, la, list_b, ll, list_audit, work))) return rc;
             if (time_it) (void)hipEventRecord(H.ev_s1, H.stream);
             list_a = la;
-            count_a = work + 8;
+            count_a = work + 16;
             cap_a = B.n;
+            t32.first_list = ll;
+            t32.first_count = work + 18;
         }
         if (gate) {
             float* tabs = upload_critical_tables(H, B, &rc);
@@ -638,7 +682,7 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
             int bpc64 = persistent_blocks_per_cu(true, B.family, B.tp.emission, B.tp.n_cam > 1, 64);
             (void)hipEventRecord(H.ev_band0, H.stream2);
             if ((e = launch_trace_persistent(true, B.family, B.tp.emission, B.tp.n_cam > 1, B.cams_dev, t64, dout, list_b,
-                                             sur ? work + 9 : work + 17, B.n, work + 12, work + 13, nullptr,
+                                             work + 17, B.n, work + 12, work + 13, nullptr,
                                              (unsigned long long*)(work + 14), std::max(1, bpc64 * H.cu_count), 64, 1,
                                              SIRIUS_FLAG_ESCALATED, H.stream2)) != hipSuccess)
                 return H.hip_fail(e, "band launch");
@@ -676,8 +720,8 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
             if ((e = launch_fill(aacc, 2, 0u, H.stream)) != hipSuccess) return H.hip_fail(e, "fill");
             unsigned long long* aatt = (unsigned long long*)H.scratch("audit_att", 8, &rc);
             if (rc) return rc;
-            TraceParams ta = t64;
-            ta.gate_flags = 0;
+            TraceParams ta = fp64_reference_params(B.tp);
+            if ((e = launch_fill((uint32_t*)aatt, 2, 0u, H.stream)) != hipSuccess) return H.hip_fail(e, "fill");
             int bpc64 = persistent_blocks_per_cu(true, B.family, B.tp.emission, B.tp.n_cam > 1, 64);
             if ((e = launch_trace_persistent(true, B.family, B.tp.emission, B.tp.n_cam > 1, B.cams_dev, ta, oa,
                                              list_audit, work + 10, B.n, work + 11, work + 5, nullptr, aatt,
@@ -696,6 +740,39 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
     }
     if ((e = launch_finish(B.family, B.tp.emission, B.cams_dev, B.tp, dout, H.stream)) != hipSuccess)
         return H.hip_fail(e, "finish launch");
+    // precision audit of the FP32 path (timed apart): a seeded fraction of the traced rays again by
+    // the plain FP64 integrator, compared with the batch's own results
+    unsigned int* pacc = nullptr;
+    unsigned int* phist = nullptr;
+    if (!fp64 && o.precision_audit > 0 && o.integrator == SIRIUS_INTEGRATOR_DP45 && !list_dev) {
+        if (B.tp.emission == SIRIUS_EMISSION_VOLUMETRIC)
+            return H.fail(SIRIUS_HIP_E_UNSUPPORTED, "the precision audit serves thin-disk and emission-free batches");
+        if (time_it) (void)hipEventRecord(H.ev_p0, H.stream);
+        DevOut oa;
+        if ((rc = bind_audit_outputs(H, B.n, o.n_bins, oa))) return rc;
+        int32_t* plist = (int32_t*)H.scratch("prec_list", 4 * (size_t)B.n, &rc);
+        pacc = (unsigned int*)H.scratch("prec_acc", 4 * 8, &rc);
+        phist = (unsigned int*)H.scratch("prec_hist", 4 * 128, &rc);
+        unsigned long long* patt = (unsigned long long*)H.scratch("prec_att", 8, &rc);
+        if (rc) return rc;
+        // work[21] audit list length, work[22] its queue head, work[23] the FP64 kernel's escalation count
+        if ((e = launch_fill(pacc, 8, 0u, H.stream)) != hipSuccess || (e = launch_fill(phist, 128, 0u, H.stream)) != hipSuccess ||
+            (e = launch_fill((uint32_t*)patt, 2, 0u, H.stream)) != hipSuccess ||
+            (e = launch_fill(work + 21, 3, 0u, H.stream)) != hipSuccess)
+            return H.hip_fail(e, "fill");
+        TraceParams tr = fp64_reference_params(B.tp);
+        int bpc64 = persistent_blocks_per_cu(true, B.family, B.tp.emission, B.tp.n_cam > 1, 64);
+        if ((e = launch_precision_select(dout, B.n, o.precision_audit, o.audit_seed, plist, work + 21, H.stream)) !=
+                hipSuccess ||
+            (e = launch_trace_persistent(true, B.family, B.tp.emission, B.tp.n_cam > 1, B.cams_dev, tr, oa, plist,
+                                         work + 21, B.n, work + 22, work + 23, nullptr, patt,
+                                         std::max(1, bpc64 * H.cu_count), 64, 1, SIRIUS_FLAG_AUDITED, H.stream)) !=
+                hipSuccess ||
+            (e = launch_precision_compare(B.cams_dev, tr, oa, dout, plist, work + 21, B.n, pacc, phist, H.stream)) !=
+                hipSuccess)
+            return H.hip_fail(e, "precision audit");
+        if (time_it) (void)hipEventRecord(H.ev_p1, H.stream);
+    }
     if (stats) {
         unsigned long long* acc = (unsigned long long*)H.scratch("stat_acc", 8 * 8, &rc);
         if (rc) return rc;
@@ -747,8 +824,10 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
             stats->flops_escalated += (double)(esc_att + band_att) * flops_per_step(B.family);
         }
         stats->steps_bulk += bulk;
-        stats->flops += (double)bulk * flops_per_step(B.family);
-        if (H.last_audit) {
+        stats->flops += (double)bulk * (o.integrator == SIRIUS_INTEGRATOR_SYMPLECTIC
+                                            ? flops_per_symplectic_step(B.family, o.symplectic_order)
+                                            : flops_per_step(B.family));
+        if (H.last_audit && audit_acc) {
             unsigned int aa[2] = {0, 0};
             if ((e = hipMemcpyAsync(aa, audit_acc, sizeof(aa), hipMemcpyDeviceToHost, H.stream)) != hipSuccess ||
                 (e = hipStreamSynchronize(H.stream)) != hipSuccess)
@@ -758,10 +837,46 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
             stats->audit_dir_err_max = std::max(stats->audit_dir_err_max, (double)err);
             stats->audit_class_mismatch += aa[1];
         }
+        if (pacc) {
+            unsigned int pa[4], ph[128];
+            if ((e = hipMemcpyAsync(pa, pacc, sizeof(pa), hipMemcpyDeviceToHost, H.stream)) != hipSuccess ||
+                (e = hipMemcpyAsync(ph, phist, sizeof(ph), hipMemcpyDeviceToHost, H.stream)) != hipSuccess ||
+                (e = hipStreamSynchronize(H.stream)) != hipSuccess)
+                return H.hip_fail(e, "precision audit copy");
+            float dm, im;
+            std::memcpy(&dm, &pa[2], 4);
+            std::memcpy(&im, &pa[3], 4);
+            stats->precision_class_mismatch += pa[0];
+            stats->precision_audited += pa[1];
+            stats->precision_dir_err_max = std::max(stats->precision_dir_err_max, (double)dm);
+            stats->precision_int_err_max = std::max(stats->precision_int_err_max, (double)im);
+            // medians from the 1/4-decade histograms (bin centres)
+            auto median = [](const unsigned int* hb) {
+                uint64_t tot = 0, run = 0;
+                for (int b = 0; b < 64; ++b) tot += hb[b];
+                if (!tot) return 0.0;
+                int b = 0;
+                for (; b < 64; ++b) {
+                    run += hb[b];
+                    if (2 * run >= tot) break;
+                }
+                return std::pow(10.0, (b + 0.5) / 4.0 - 12.0);
+            };
+            stats->precision_dir_err_median = median(ph);
+            stats->precision_int_err_median = median(ph + 64);
+            if (time_it) {
+                float msp = 0;
+                (void)hipEventElapsedTime(&msp, H.ev_p0, H.ev_p1);
+                stats->precision_audit_ms += msp;
+            }
+        }
         if (o.use_surrogate) {
+            // the network runs on the candidates only
             stats->rays_surrogate += h[5];
-            stats->rays_fallback += (uint64_t)wk[8] + wk[9];
-            stats->surrogate_flops += (double)B.n * 398848.0;
+            stats->rays_candidates += wk[19];
+            stats->rays_gated += wk[20];
+            stats->rays_fallback += (uint64_t)wk[19] - std::min<uint64_t>(wk[19], h[5]);
+            stats->surrogate_flops += (double)wk[19] * 398848.0;
         }
         if (time_it) {
             float ms = 0, ms2 = 0, ms3 = 0;
@@ -930,6 +1045,9 @@ void sirius_hip_default_opts(SiriusHipTraceOpts* o, int precision) {
     o->escalate_rtol = 0.0;
     o->critical_band = 0.5;
     o->step_radial_exp = 0.0;
+    o->surrogate_err_max = 5e-5;
+    o->surrogate_margin = 2.5;
+    o->precision_audit = 0.0;
     if (precision == SIRIUS_PRECISION_FP32_ESCALATE) {
         o->rtol = 1e-6;
         o->atol = 1e-8;

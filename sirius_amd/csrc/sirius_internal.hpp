@@ -61,6 +61,7 @@ struct Handle {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr, ev4 = nullptr;
     hipEvent_t ev_gate = nullptr, ev_band0 = nullptr, ev_band1 = nullptr;
     hipEvent_t ev_s0 = nullptr, ev_s1 = nullptr, ev_a0 = nullptr, ev_a1 = nullptr;  // surrogate, audit
+    hipEvent_t ev_p0 = nullptr, ev_p1 = nullptr;  // precision audit
     int cu_count = 0;
     int max_threads = 0;
     size_t total_mem = 0;
@@ -129,10 +130,11 @@ float* upload_critical_tables(Handle& H, Batch& B, int* rc);
 
 // surrogate (mlp_kernels.hip / surrogate.cpp)
 void mlp_release(Handle& H);
-// Network stage of the FP32 path: inputs, inference, per-ray decision. Fills list_a (FP32
-// integrator), list_b (FP64 band) and list_audit; counts[0..2] are their lengths.
+// Network stage of the FP32 path: gate with the analytic candidate test, fused inference on the
+// candidates, per-candidate decision. Fills list_a (FP32 integrator), list_b (FP64 band), list_l
+// (taken first) and list_audit; their lengths are work[16], work[17], work[18] and work[10].
 int surrogate_stage(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, int32_t* list_a,
-                    int32_t* list_b, int32_t* list_audit, uint32_t* counts);
+                    int32_t* list_b, int32_t* list_l, int32_t* list_audit, uint32_t* work);
 int mlp_forward_dev(Handle& H, const float* x, float* y, int64_t n);
 
 }  // namespace sirius

@@ -30,6 +30,8 @@ void mlp_release(Handle& H) {
         if (S.w[l]) (void)hipFree(S.w[l]);
         if (S.b[l]) (void)hipFree(S.b[l]);
     }
+    for (void* p : {(void*)S.fw0, (void*)S.fw4h, (void*)S.fw4l, (void*)S.cal_err, (void*)S.cal_n, (void*)S.cal_bad})
+        if (p) (void)hipFree(p);
     delete H.mlp;
     H.mlp = nullptr;
 }
@@ -49,7 +51,33 @@ static int mlp_alloc(Handle& H) {
             (e = hipMalloc((void**)&S.b[l], (size_t)MLP_NP[l] * 4)) != hipSuccess)
             return H.hip_fail(e, "mlp alloc");
     }
+    if ((e = hipMalloc((void**)&S.fw0, (size_t)MLP_H * 16 * 2)) != hipSuccess ||
+        (e = hipMalloc((void**)&S.fw4h, (size_t)32 * MLP_H * 2)) != hipSuccess ||
+        (e = hipMalloc((void**)&S.fw4l, (size_t)32 * MLP_H * 2)) != hipSuccess ||
+        (e = hipMalloc((void**)&S.cal_err, SUR_BINS * 4)) != hipSuccess ||
+        (e = hipMalloc((void**)&S.cal_n, SUR_BINS * 4)) != hipSuccess ||
+        (e = hipMalloc((void**)&S.cal_bad, SUR_BINS * 4)) != hipSuccess)
+        return H.hip_fail(e, "mlp alloc");
     return SIRIUS_HIP_OK;
+}
+
+hipError_t launch_fill(uint32_t* p, int n, uint32_t v, hipStream_t st);
+
+// New parameters invalidate the calibration: the error gate answers nothing until the network
+// is calibrated again (sirius_hip_surrogate_train calibrates on held-out rays).
+static hipError_t mlp_clear_calibration(MlpState& S, hipStream_t st) {
+    hipError_t e;
+    if ((e = launch_fill((uint32_t*)S.cal_err, SUR_BINS, 0u, st)) != hipSuccess ||
+        (e = launch_fill(S.cal_n, SUR_BINS, 0u, st)) != hipSuccess)
+        return e;
+    return launch_fill(S.cal_bad, SUR_BINS, 0u, st);
+}
+
+// BF16 operands of every kernel from the FP32 masters (after each update).
+static hipError_t mlp_pack(MlpState& S, hipStream_t st) {
+    hipError_t e = launch_pack(S.params, S.w, S.b, st);
+    if (e != hipSuccess) return e;
+    return launch_pack_fused(S.params, S.fw0, S.fw4h, S.fw4l, mlp_param_offset_w(0), mlp_param_offset_w(4), st);
 }
 
 static int mlp_reset_adam(Handle& H) {
@@ -60,7 +88,8 @@ static int mlp_reset_adam(Handle& H) {
         (e = hipMemsetAsync(S.adam_v, 0, P * 4, H.stream)) != hipSuccess)
         return H.hip_fail(e, "adam reset");
     S.step = 0;
-    if ((e = launch_pack(S.params, S.w, S.b, H.stream)) != hipSuccess) return H.hip_fail(e, "pack");
+    if ((e = mlp_pack(S, H.stream)) != hipSuccess || (e = mlp_clear_calibration(S, H.stream)) != hipSuccess)
+        return H.hip_fail(e, "pack");
     if ((e = hipStreamSynchronize(H.stream)) != hipSuccess) return H.hip_fail(e, "synchronize");
     S.ready = true;
     return SIRIUS_HIP_OK;
@@ -122,7 +151,8 @@ static GemmArgs gemm(const uint16_t* A, int64_t sai, int64_t sak, const uint16_t
     return g;
 }
 
-// Forward of n_pad rows already in W.h[0]; keeps z (training) or only the last activations.
+// Training forward of n_pad rows already in W.h[0]: the hidden layers on BF16 MFMA (keeping
+// their pre-activations z), the output head in FP32 from z3.
 static hipError_t mlp_forward_rows(MlpState& S, MlpWork& W, int64_t n_pad, hipStream_t st) {
     hipError_t e;
     for (int l = 0; l < MLP_NL - 1; ++l) {
@@ -133,10 +163,8 @@ static hipError_t mlp_forward_rows(MlpState& S, MlpWork& W, int64_t n_pad, hipSt
         g.hout = W.h[l + 1];
         if ((e = launch_gemm(EPI_HIDDEN, g, 1, st)) != hipSuccess) return e;
     }
-    GemmArgs g = gemm(W.h[4], MLP_H, 1, S.w[4], 1, MLP_H, (int)n_pad, MLP_N4, MLP_H);
-    g.bias = S.b[4];
-    g.out = W.y64;
-    return launch_gemm(EPI_OUT, g, 1, st);
+    return launch_head_fwd(W.z[3], S.params + mlp_param_offset_w(4), S.params + mlp_param_offset_b(4), n_pad, W.y64,
+                           st);
 }
 
 // Split of the sample dimension for the weight-gradient GEMMs
@@ -164,23 +192,18 @@ static int mlp_loss_grad_dev(Handle& H, const float* x, const int32_t* cls, cons
         return H.hip_fail(e, "mlp loss");
     int nsplit, kchunk;
     split_k(n_pad, &nsplit, &kchunk);
-    // layer 4: dW4 = gy^T h4, db4 = colsum(gy), gz3 = (gy W4) * gelu'(z3)
+    // layer 4 (the FP32 head): dW4 = gy^T gelu(z3), db4 = colsum(gy), gz3 = (gy W4) * gelu'(z3)
     {
-        GemmArgs g = gemm(W.gyb, 1, MLP_N4, W.h[4], MLP_H, 1, MLP_N4, MLP_H, (int)n_pad);
-        g.kchunk = kchunk;
-        g.out = W.part;
-        if ((e = launch_gemm(EPI_PART, g, nsplit, st)) != hipSuccess) return H.hip_fail(e, "dW4");
-        if ((e = launch_reduce_partials(W.part, nsplit, MLP_N4, MLP_H, SIRIUS_MLP_OUT, MLP_H,
-                                        grad + mlp_param_offset_w(4), st)) != hipSuccess)
-            return H.hip_fail(e, "dW4 reduce");
+        const int hsplit = (int)std::min<int64_t>(128, std::max<int64_t>(1, n_pad / 64));
+        if ((e = launch_head_wgrad(W.gyf, W.z[3], n_pad, W.part, hsplit, grad + mlp_param_offset_w(4), st)) !=
+            hipSuccess)
+            return H.hip_fail(e, "dW4");
         if ((e = launch_colsum(W.gyf, n_pad, SIRIUS_MLP_OUT, MLP_N4, W.colpart, COLSPLIT, grad + mlp_param_offset_b(4),
                                st)) != hipSuccess)
             return H.hip_fail(e, "db4");
-        GemmArgs d = gemm(W.gyb, MLP_N4, 1, S.w[4], MLP_H, 1, (int)n_pad, MLP_H, MLP_N4);
-        d.zin = W.z[3];
-        d.gzf = W.gzf[0];
-        d.gzb = W.gzb[0];
-        if ((e = launch_gemm(EPI_BWD, d, 1, st)) != hipSuccess) return H.hip_fail(e, "dX4");
+        if ((e = launch_head_bwd(W.gyf, S.params + mlp_param_offset_w(4), W.z[3], n_pad, W.gzf[0], W.gzb[0], st)) !=
+            hipSuccess)
+            return H.hip_fail(e, "dX4");
     }
     int cur = 0;
     for (int l = MLP_NL - 2; l >= 0; --l) {
@@ -212,52 +235,76 @@ static int mlp_loss_grad_dev(Handle& H, const float* x, const int32_t* cls, cons
     return SIRIUS_HIP_OK;
 }
 
-// Forward pass of n samples x[n*5] -> y[n*6] (device), in chunks.
-int mlp_forward_dev(Handle& H, const float* x, float* y, int64_t n) {
-    MlpState& S = *H.mlp;
-    const int64_t CH = 1 << 18;
-    MlpWork W;
-    int rc = mlp_workspace(H, pad64(std::min(n, CH)), false, W);
-    if (rc) return rc;
-    hipError_t e;
-    for (int64_t s = 0; s < n; s += CH) {
-        int64_t m = std::min(CH, n - s), m_pad = pad64(m);
-        if ((e = launch_mlp_input(x + s * SIRIUS_MLP_IN, m, m_pad, W.h[0], H.stream)) != hipSuccess ||
-            (e = mlp_forward_rows(S, W, m_pad, H.stream)) != hipSuccess ||
-            (e = launch_mlp_extract(W.y64, m, y + s * SIRIUS_MLP_OUT, H.stream)) != hipSuccess)
-            return H.hip_fail(e, "mlp forward");
-    }
-    return SIRIUS_HIP_OK;
+MlpFusedArgs mlp_fused_args(const MlpState& S, const float* x, float* y, int64_t n) {
+    MlpFusedArgs a;
+    a.x = x;
+    a.n = n;
+    a.n_dev = nullptr;
+    a.idx = nullptr;
+    a.x_stride = SIRIUS_MLP_IN;
+    a.w0 = S.fw0;
+    for (int l = 1; l < 4; ++l) a.w[l - 1] = S.w[l];
+    for (int l = 0; l < 4; ++l) a.b[l] = S.b[l];
+    a.w4h = S.fw4h;
+    a.w4l = S.fw4l;
+    a.b4 = S.params + mlp_param_offset_b(4);
+    a.y = y;
+    return a;
 }
 
-hipError_t launch_sur_features(const CamConst* cams, int n_cam, const float* tabs, int64_t n, double crit_band,
-                               float* x, float* n0, float* dist, uint8_t* flags, hipStream_t st);
-hipError_t launch_sur_targets(const DevOut& o, const float* n0, int64_t n, int32_t* cls, float* dir, hipStream_t st);
-hipError_t launch_sur_apply(const float* y, const float* n0, const float* dist, int64_t n, double esc_band,
-                            double sur_band, double pmin, double audit_fraction, uint64_t audit_seed, const DevOut& o,
-                            int n_bins, int32_t* list_a, int32_t* list_b, int32_t* list_audit, uint32_t* counts,
-                            hipStream_t st);
+// Forward pass of n samples x[n*5] -> y[n*6] (device): the fused inference kernel.
+int mlp_forward_dev(Handle& H, const float* x, float* y, int64_t n) {
+    hipError_t e = launch_mlp_fused(mlp_fused_args(*H.mlp, x, y, n), n, 1 << 30, H.stream);
+    return e == hipSuccess ? SIRIUS_HIP_OK : H.hip_fail(e, "mlp forward");
+}
+
+hipError_t launch_sur_features(const CamConst* cams, int n_cam, const float* tabs, int64_t n, double sur_band,
+                               double margin, double r_exit, float* x, float* ref, uint8_t* cand, float* dist,
+                               float* rec, hipStream_t st);
+hipError_t launch_sur_targets(const DevOut& o, const float* ref, int64_t n, int32_t* cls, float* dir, hipStream_t st);
+hipError_t launch_sur_gate(const CamConst* cams, int n_cam, const float* tabs, int64_t n, double band, double crit_band,
+                           double long_band, double sur_band, double margin, double r_exit, int32_t* list_a,
+                           int32_t* list_b, int32_t* list_l, int32_t* list_s, uint32_t* counts, uint8_t* flags,
+                           float* rec, hipStream_t st);
+hipError_t launch_sur_decide(const int32_t* list_s, const uint32_t* count_s, int64_t cap, const float* y,
+                             const float* rec, const float* cal_err, const uint32_t* cal_n, const uint32_t* cal_bad,
+                             double pmin, double err_max, double audit_fraction, uint64_t audit_seed, const DevOut& o,
+                             int n_bins, int32_t* list_a, uint32_t* count_a, int32_t* list_audit,
+                             uint32_t* count_audit, uint32_t* n_gated, hipStream_t st);
+hipError_t launch_sur_calibrate(const int32_t* list, int64_t m, const float* y, const float* rec, const DevOut& o,
+                                double pmin, float* cal_err, uint32_t* cal_n, uint32_t* cal_bad, hipStream_t st);
 hipError_t launch_sur_gather(const int32_t* perm, int64_t start, int64_t m, const float* x, const int32_t* cls,
                              const float* dir, float* xb, int32_t* cb, float* db, hipStream_t st);
 
+// Surrogate stage of the FP32 path: the gate sorts the batch into lists A (FP32 integrator),
+// B (FP64 band), L (taken first: near the critical curve) and S (the network's candidates, by the
+// analytic test); the fused network runs on S, and the decision answers candidates or appends
+// them to list A. work[16..19] = |A|, |B|, |L|, |S|, work[10] = |audit|, work[20] = candidates
+// the error gate refused. Every count stays on the device.
 int surrogate_stage(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, int32_t* list_a,
-                    int32_t* list_b, int32_t* list_audit, uint32_t* counts) {
+                    int32_t* list_b, int32_t* list_l, int32_t* list_audit, uint32_t* work) {
     int rc = SIRIUS_HIP_OK;
     float* tabs = upload_critical_tables(H, B, &rc);
     const int64_t n = B.n;
-    float* x = (float*)H.scratch("sur_x", (size_t)n * 5 * 4, &rc);
-    float* n0 = (float*)H.scratch("sur_n0", (size_t)n * 3 * 4, &rc);
-    float* dist = (float*)H.scratch("sur_dist", (size_t)n * 4, &rc);
+    int32_t* list_s = (int32_t*)H.scratch("sur_s", (size_t)n * 4, &rc);
+    float* rec = (float*)H.scratch("sur_rec", (size_t)n * SUR_REC * 4, &rc);
     float* y = (float*)H.scratch("sur_y", (size_t)n * SIRIUS_MLP_OUT * 4, &rc);
     if (rc) return rc;
+    MlpState& S = *H.mlp;
     hipError_t e;
-    if ((e = launch_sur_features(B.cams_dev, (int)B.cc.size(), tabs, n, std::max(o.critical_band, 0.0), x, n0, dist,
-                                 dout.flags, H.stream)) != hipSuccess)
-        return H.hip_fail(e, "surrogate inputs");
-    if ((rc = mlp_forward_dev(H, x, y, n))) return rc;
-    if ((e = launch_sur_apply(y, n0, dist, n, std::max(o.escalate_band, 0.0), o.surrogate_band, o.surrogate_pmin,
-                              o.audit_fraction, o.audit_seed, dout, o.n_bins, list_a, list_b, list_audit, counts,
-                              H.stream)) != hipSuccess)
+    if ((e = launch_sur_gate(B.cams_dev, (int)B.cc.size(), tabs, n, std::max(o.escalate_band, 0.0),
+                             std::max(o.critical_band, 0.0), 1.0, o.surrogate_band, o.surrogate_margin, o.r_escape,
+                             list_a, list_b, list_l, list_s, work + 16, dout.flags, rec, H.stream)) != hipSuccess)
+        return H.hip_fail(e, "surrogate gate");
+    MlpFusedArgs a = mlp_fused_args(S, rec, y, 0);
+    a.n_dev = work + 19;
+    a.idx = list_s;
+    a.x_stride = SUR_REC;
+    if ((e = launch_mlp_fused(a, n, 2 * std::max(H.cu_count, 1), H.stream)) != hipSuccess)
+        return H.hip_fail(e, "surrogate network");
+    if ((e = launch_sur_decide(list_s, work + 19, n, y, rec, S.cal_err, S.cal_n, S.cal_bad, o.surrogate_pmin,
+                               o.surrogate_err_max, o.audit_fraction, o.audit_seed, dout, o.n_bins, list_a, work + 16,
+                               list_audit, work + 10, work + 20, H.stream)) != hipSuccess)
         return H.hip_fail(e, "surrogate decision");
     return SIRIUS_HIP_OK;
 }
@@ -366,26 +413,28 @@ static int mlp_adam_step(Handle& H, const float* x, const int32_t* cls, const fl
     hipError_t e;
     if ((e = launch_adam(S.params, S.grad, S.adam_m, S.adam_v, (int64_t)mlp_param_count(), lr, c1, c2, H.stream)) !=
             hipSuccess ||
-        (e = launch_pack(S.params, S.w, S.b, H.stream)) != hipSuccess)
+        (e = mlp_pack(S, H.stream)) != hipSuccess)
         return H.hip_fail(e, "adam");
     return SIRIUS_HIP_OK;
 }
 
 int sirius_hip_surrogate_features(SiriusHipHandle hh, const SiriusHipCamera* cams, int n_cam,
-                                  const SiriusHipMetric* metrics, int n_metric, float* x, float* n0) {
+                                  const SiriusHipMetric* metrics, int n_metric, float* x, float* ref) {
     if (!hh) return SIRIUS_HIP_E_INVALID;
     Handle* H = static_cast<Handle*>(hh);
     MLP_GUARD_BEGIN
-    if (!x || !n0) return H->fail(SIRIUS_HIP_E_INVALID, "x and n0 are required");
+    if (!x || !ref) return H->fail(SIRIUS_HIP_E_INVALID, "x and ref are required");
     int rc = H->ensure_device();
     if (rc) return rc;
     SiriusHipTraceOpts o;
-    sirius_hip_default_opts(&o, SIRIUS_PRECISION_FP64);
+    sirius_hip_default_opts(&o, SIRIUS_PRECISION_FP32_ESCALATE);
     Batch B;
     if ((rc = prepare_batch(*H, cams, n_cam, metrics, n_metric, o, B))) return rc;
     if (B.family != SIRIUS_METRIC_KERR_SCHILD) return H->fail(SIRIUS_HIP_E_UNSUPPORTED, "Kerr-Schild cameras only");
-    hipError_t e = launch_sur_features(B.cams_dev, (int)B.cc.size(), nullptr, B.n, 0.0, x, n0, nullptr, nullptr,
-                                       H->stream);
+    float* tabs = upload_critical_tables(*H, B, &rc);
+    if (rc) return rc;
+    hipError_t e = launch_sur_features(B.cams_dev, (int)B.cc.size(), tabs, B.n, o.surrogate_band, o.surrogate_margin,
+                                       o.r_escape, x, ref, nullptr, nullptr, nullptr, H->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(H->stream);
     return e == hipSuccess ? SIRIUS_HIP_OK : H->hip_fail(e, "surrogate inputs");
     MLP_GUARD_END(H)
@@ -409,31 +458,53 @@ int sirius_hip_surrogate_train(SiriusHipHandle hh, const SiriusHipCamera* cams, 
     if ((rc = bind_outputs(*H, nullptr, B.n, label_opts->n_bins, dout, label_opts->emission == SIRIUS_EMISSION_VOLUMETRIC))) return rc;
     if ((rc = run_trace(*H, B, *label_opts, dout, nullptr, nullptr, 0, nullptr, false))) return rc;
     const int64_t n = B.n;
+    float* tabs = upload_critical_tables(*H, B, &rc);
     float* x = (float*)H->scratch("tr_x", (size_t)n * 5 * 4, &rc);
-    float* n0 = (float*)H->scratch("tr_n0", (size_t)n * 3 * 4, &rc);
+    float* ref = (float*)H->scratch("tr_ref", (size_t)n * 3 * 4, &rc);
+    uint8_t* cand = (uint8_t*)H->scratch("tr_cand", (size_t)n, &rc);
+    float* rec = (float*)H->scratch("tr_rec", (size_t)n * SUR_REC * 4, &rc);
     int32_t* cls = (int32_t*)H->scratch("tr_cls", (size_t)n * 4, &rc);
     float* dir = (float*)H->scratch("tr_dir", (size_t)n * 3 * 4, &rc);
-    int32_t* perm = (int32_t*)H->scratch("tr_perm", (size_t)n * 4, &rc);
-    const int64_t m = std::min<int64_t>(batch, n);
+    if (rc) return rc;
+    // the candidate test and the reference are the trace path's, with label_opts' band, margin and
+    // escape radius (the decision uses the tracing options' own)
+    hipError_t e;
+    if ((e = launch_sur_features(B.cams_dev, (int)B.cc.size(), tabs, n, label_opts->surrogate_band,
+                                 label_opts->surrogate_margin, label_opts->r_escape, x, ref, cand, nullptr, rec,
+                                 H->stream)) != hipSuccess ||
+        (e = launch_sur_targets(dout, ref, n, cls, dir, H->stream)) != hipSuccess)
+        return H->hip_fail(e, "training set");
+    // training and held-out candidates: a seeded 1 in 16 of them calibrates the error gate
+    std::vector<uint8_t> c((size_t)n);
+    if ((e = hipMemcpyAsync(c.data(), cand, (size_t)n, hipMemcpyDeviceToHost, H->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(H->stream)) != hipSuccess)
+        return H->hip_fail(e, "candidates");
+    std::vector<int32_t> tr, ho;
+    for (int64_t i = 0; i < n; ++i) {
+        if (!c[(size_t)i]) continue;
+        uint64_t z = seed ^ (0xD1B54A32D192ED03ULL * (uint64_t)(i + 1));
+        (((splitmix(z) >> 32) & 15u) == 0 ? ho : tr).push_back((int32_t)i);
+    }
+    if (tr.empty()) return H->fail(SIRIUS_HIP_E_INVALID, "no ray of these cameras is a surrogate candidate");
+    const int64_t nt = (int64_t)tr.size(), nh = (int64_t)ho.size();
+    int32_t* perm = (int32_t*)H->scratch("tr_perm", (size_t)nt * 4, &rc);
+    int32_t* hol = (int32_t*)H->scratch("tr_hold", (size_t)std::max<int64_t>(nh, 1) * 4, &rc);
+    float* yh = (float*)H->scratch("tr_yhold", (size_t)std::max<int64_t>(nh, 1) * SIRIUS_MLP_OUT * 4, &rc);
+    const int64_t m = std::min<int64_t>(batch, nt);
     float* xb = (float*)H->scratch("tr_xb", (size_t)m * 5 * 4, &rc);
     int32_t* cb = (int32_t*)H->scratch("tr_cb", (size_t)m * 4, &rc);
     float* db = (float*)H->scratch("tr_db", (size_t)m * 3 * 4, &rc);
     if (rc) return rc;
-    hipError_t e;
-    if ((e = launch_sur_features(B.cams_dev, (int)B.cc.size(), nullptr, n, 0.0, x, n0, nullptr, nullptr, H->stream)) !=
-            hipSuccess ||
-        (e = launch_sur_targets(dout, n0, n, cls, dir, H->stream)) != hipSuccess)
-        return H->hip_fail(e, "training set");
-    std::vector<int32_t> p((size_t)n);
+    if (nh && (e = hipMemcpyAsync(hol, ho.data(), (size_t)nh * 4, hipMemcpyHostToDevice, H->stream)) != hipSuccess)
+        return H->hip_fail(e, "held-out upload");
     uint64_t s = seed;
-    const int64_t steps_per_epoch = n / m;
+    const int64_t steps_per_epoch = std::max<int64_t>(1, nt / m);
     const int64_t total = steps_per_epoch * epochs;
     int64_t step = 0;
     double last = 0;
     for (int ep = 0; ep < epochs; ++ep) {
-        for (int64_t i = 0; i < n; ++i) p[(size_t)i] = (int32_t)i;
-        for (int64_t i = n - 1; i > 0; --i) std::swap(p[(size_t)i], p[(size_t)(splitmix(s) % (uint64_t)(i + 1))]);
-        if ((e = hipMemcpyAsync(perm, p.data(), (size_t)n * 4, hipMemcpyHostToDevice, H->stream)) != hipSuccess)
+        for (int64_t i = nt - 1; i > 0; --i) std::swap(tr[(size_t)i], tr[(size_t)(splitmix(s) % (uint64_t)(i + 1))]);
+        if ((e = hipMemcpyAsync(perm, tr.data(), (size_t)nt * 4, hipMemcpyHostToDevice, H->stream)) != hipSuccess)
             return H->hip_fail(e, "permutation upload");
         double sum = 0;
         for (int64_t b = 0; b < steps_per_epoch; ++b, ++step) {
@@ -445,10 +516,45 @@ int sirius_hip_surrogate_train(SiriusHipHandle hh, const SiriusHipCamera* cams, 
             if ((rc = mlp_adam_step(*H, xb, cb, db, (int)m, lr_t, &l))) return rc;
             sum += l;
         }
-        last = sum / (double)std::max<int64_t>(steps_per_epoch, 1);
+        last = sum / (double)steps_per_epoch;
+        // the permutation buffer is rewritten next epoch: its uploads read tr, which the next
+        // shuffle changes, so the stream finishes this epoch first
+        if ((e = hipStreamSynchronize(H->stream)) != hipSuccess) return H->hip_fail(e, "synchronize");
+    }
+    // calibration of the error gate on the held-out candidates
+    MlpState& S = *H->mlp;
+    if ((e = mlp_clear_calibration(S, H->stream)) != hipSuccess) return H->hip_fail(e, "calibration");
+    if (nh) {
+        MlpFusedArgs a = mlp_fused_args(S, rec, yh, nh);
+        a.idx = hol;
+        a.x_stride = SUR_REC;
+        if ((e = launch_mlp_fused(a, nh, 1 << 30, H->stream)) != hipSuccess ||
+            (e = launch_sur_calibrate(hol, nh, yh, rec, dout, label_opts->surrogate_pmin, S.cal_err, S.cal_n,
+                                      S.cal_bad, H->stream)) != hipSuccess)
+            return H->hip_fail(e, "calibration");
     }
     if ((e = hipStreamSynchronize(H->stream)) != hipSuccess) return H->hip_fail(e, "synchronize");
     if (loss) *loss = (float)last;
+    return SIRIUS_HIP_OK;
+    MLP_GUARD_END(H)
+}
+
+int sirius_hip_surrogate_calibration(SiriusHipHandle hh, float* max_err, uint32_t* answered, uint32_t* misclassified,
+                                     int capacity, int* n_bins) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    MLP_GUARD_BEGIN
+    if (n_bins) *n_bins = SUR_BINS;
+    if (!H->mlp || !H->mlp->ready) return H->fail(SIRIUS_HIP_E_STATE, "network not initialised");
+    if (!max_err || !answered || !misclassified || capacity < SUR_BINS)
+        return H->fail(SIRIUS_HIP_E_INVALID, "three arrays of at least n_bins entries are required");
+    MlpState& S = *H->mlp;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(max_err, S.cal_err, SUR_BINS * 4, hipMemcpyDeviceToHost, H->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(answered, S.cal_n, SUR_BINS * 4, hipMemcpyDeviceToHost, H->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(misclassified, S.cal_bad, SUR_BINS * 4, hipMemcpyDeviceToHost, H->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(H->stream)) != hipSuccess)
+        return H->hip_fail(e, "calibration copy");
     return SIRIUS_HIP_OK;
     MLP_GUARD_END(H)
 }

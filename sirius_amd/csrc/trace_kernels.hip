@@ -423,7 +423,7 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
         if (R.x[3] * x1[3] < T(0)) {
             T xc[4];
             if constexpr (fp32) {
-                th_disk = locate_event_dense(s, EV_Z, T(0), R.x, R.d.k, x1, d1.k, dn, R.h);
+                th_disk = locate_event_newton(s, EV_Z, T(0), R.x, R.d.k, x1, d1.k, dn, R.h, R.x[3], x1[3]);
                 dense_pos(R.x, R.d.k, x1, d1.k, dn, R.h, th_disk, xc);
             } else {
                 th_disk = locate_event(s, EV_Z, T(0), R.x, R.d.k, x1, d1.k, R.h);
@@ -459,7 +459,9 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
         bool plane = (ev == SIRIUS_RAY_DISK) && EMIT == SIRIUS_EMISSION_THIN_DISK;
         if (plane || locate) {
             if constexpr (fp32) {
-                T th = plane ? th_disk : locate_event_dense(s, kind, target, R.x, R.d.k, x1, d1.k, dn, R.h);
+                T th = plane ? th_disk
+                             : locate_event_newton(s, kind, target, R.x, R.d.k, x1, d1.k, dn, R.h, R.r_cur - target,
+                                                   r1 - target);
                 land_dense<T, FAM>(R, x1, p1, d1, dn, lo1, th);
             } else {
                 T th = plane ? th_disk : locate_event(s, kind, target, R.x, R.d.k, x1, d1.k, R.h);
@@ -643,6 +645,15 @@ __global__ __launch_bounds__(256) void k_finish(const CamConst* __restrict__ cam
     ray_outputs<FAM, EMIT>(cams[cam], tp, o, id);
 }
 
+// Angle between two directions given as (theta, phi), in FP64 by the chord (accurate down to
+// 1e-15 rad, where the arc cosine of a dot product loses everything below ~1e-8).
+__device__ __forceinline__ double dir_angle(float ta, float pa, float tb, float pb) {
+    const double a[3] = {sin((double)ta) * cos((double)pa), sin((double)ta) * sin((double)pa), cos((double)ta)};
+    const double b[3] = {sin((double)tb) * cos((double)pb), sin((double)tb) * sin((double)pb), cos((double)tb)};
+    const double d = sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+    return 2.0 * asin(fmin(1.0, 0.5 * d));
+}
+
 // Surrogate audit: outputs of the FP64 re-traces of audited rays (in oa), compared with the
 // surrogate's answers in o. acc[0]: largest escape-direction error (float bits), acc[1]:
 // class mismatches.
@@ -661,10 +672,74 @@ __global__ void k_sur_audit(const CamConst* __restrict__ cams, TraceParams tp, D
         return;
     }
     if (ca == SIRIUS_RAY_ESCAPED) {
-        float ta = oa.theta[id], pa = oa.phi[id], ts = o.theta[id], ps = o.phi[id];
-        float dot = sinf(ta) * sinf(ts) * cosf(pa - ps) + cosf(ta) * cosf(ts);
-        float err = acosf(fminf(1.f, fmaxf(-1.f, dot)));
+        const float err = (float)dir_angle(oa.theta[id], oa.phi[id], o.theta[id], o.phi[id]);
         atomicMax(&acc[0], __float_as_uint(err));
+    }
+}
+
+// Selection of the precision audit: rays of the batch with hash(seed ^ id) below the fraction's
+// threshold that the FP32 path traced (not answered by the surrogate), into list (count *count).
+__global__ void k_precision_select(const DevOut o, int64_t n, uint64_t thr, uint64_t seed, int32_t* __restrict__ list,
+                                   uint32_t* __restrict__ count) {
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool take = false;
+    if (id < n) {
+        uint64_t z = (seed ^ (uint64_t)id) + 0x9E3779B97F4A7C15ULL;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        take = z <= thr && !(o.flags[id] & SIRIUS_FLAG_SURROGATE);
+    }
+    const unsigned long long m = __ballot(take);
+    const int lane = __lane_id();
+    uint32_t base = 0;
+    if (lane == 0 && m) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, 0);
+    if (take) list[base + __popcll(m & (lane ? ((1ull << lane) - 1ull) : 0ull))] = (int32_t)id;
+}
+
+// Comparison of the precision audit: the FP64 re-traces (oa) against the batch's results (o).
+// acc[0] class mismatches, acc[1] audited rays, acc[2] largest direction error (float bits),
+// acc[3] largest relative intensity error (float bits); hist[0..63] and hist[64..127] bin the
+// direction and intensity errors by log10, 1/4 decade per bin from 1e-12, for their medians.
+__global__ void k_precision_compare(const CamConst* __restrict__ cams, TraceParams tp, DevOut oa, DevOut o,
+                                    const int32_t* __restrict__ list, const uint32_t* __restrict__ count, int64_t cap,
+                                    unsigned int* __restrict__ acc, unsigned int* __restrict__ hist) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t lim = min((int64_t)*count, cap);
+    if (i >= lim) return;
+    const int64_t id = list[i];
+    const int cam = tp.n_cam == 1 ? 0 : find_camera(cams, tp.n_cam, id);
+    if (cams[cam].family == 1) ray_outputs<1, SIRIUS_EMISSION_NONE>(cams[cam], tp, oa, id);
+    else if (cams[cam].family == 2) ray_outputs<2, SIRIUS_EMISSION_NONE>(cams[cam], tp, oa, id);
+    else if (tp.emission == SIRIUS_EMISSION_THIN_DISK) ray_outputs<0, SIRIUS_EMISSION_THIN_DISK>(cams[cam], tp, oa, id);
+    else ray_outputs<0, SIRIUS_EMISSION_NONE>(cams[cam], tp, oa, id);
+    atomicAdd(&acc[1], 1u);
+    const int ca = oa.cls[id], cs = o.cls[id];
+    if (ca != cs) {
+        atomicAdd(&acc[0], 1u);
+        return;
+    }
+    auto bin = [](double e) {
+        int b = e > 0.0 ? (int)((log10(e) + 12.0) * 4.0) : 0;
+        return b < 0 ? 0 : (b > 63 ? 63 : b);
+    };
+    if (ca == SIRIUS_RAY_ESCAPED) {
+        const double err = dir_angle(oa.theta[id], oa.phi[id], o.theta[id], o.phi[id]);
+        atomicMax(&acc[2], __float_as_uint((float)err));
+        atomicAdd(&hist[bin(err)], 1u);
+    } else if (ca == SIRIUS_RAY_DISK && tp.emission == SIRIUS_EMISSION_THIN_DISK && tp.n_bins > 0) {
+        double imax = 0.0, dmax = 0.0;
+        for (int b = 0; b < tp.n_bins; ++b) {
+            const double ia = oa.intensity[(int64_t)b * oa.n_rays + id], is = o.intensity[(int64_t)b * o.n_rays + id];
+            imax = fmax(imax, ia);
+            dmax = fmax(dmax, fabs(is - ia));
+        }
+        if (imax > 0.0) {
+            const double rel = dmax / imax;
+            atomicMax(&acc[3], __float_as_uint((float)rel));
+            atomicAdd(&hist[64 + bin(rel)], 1u);
+        }
     }
 }
 
@@ -743,12 +818,22 @@ __global__ __launch_bounds__(256) void k_symplectic_grid(const CamConst* __restr
         if (bad) { cls = SIRIUS_RAY_FAILED; break; }
         T r = event_radius(s, q);
         if (FAM == 0 && tp.emission == SIRIUS_EMISSION_THIN_DISK && q0[3] * q[3] < T(0)) {
-            T al = -q0[3] / (q[3] - q0[3]);
-            T xc = q0[1] + al * (q[1] - q0[1]), yc = q0[2] + al * (q[2] - q0[2]);
-            T rr = xc * xc + yc * yc - s.a2;
+            // the crossing is located on the cubic Hermite interpolant of the whole state over the
+            // step (H's derivatives at both ends) and the ray ends there: a linear interpolant
+            // would add an O(h^2) null-constraint error of its own at the hit
+            Der<T> D0, D1;
+            eval<T, FAM>(s, q0, p0, D0);
+            eval<T, FAM>(s, q, p, D1);
+            T th = locate_event(s, EV_Z, T(0), q0, D0.k, q, D1.k, h);
+            T xc[4], pc[4];
+            hermite(q0, D0.k, q, D1.k, h, th, xc);
+            T rr = xc[1] * xc[1] + xc[2] * xc[2] - s.a2;
             T rdisk = sqrt(rr > T(0) ? rr : T(0));
             if (rdisk >= T(c.r_in) && rdisk <= T(c.r_out)) {
-                for (int i = 0; i < 4; ++i) { q[i] = q0[i] + al * (q[i] - q0[i]); p[i] = p0[i] + al * (p[i] - p0[i]); }
+                const T f0[4] = {T(0), D0.f[0], D0.f[1], D0.f[2]}, f1[4] = {T(0), D1.f[0], D1.f[1], D1.f[2]};
+                hermite(p0, f0, p, f1, h, th, pc);
+                for (int i = 0; i < 4; ++i) { q[i] = xc[i]; p[i] = pc[i]; }
+                R.lam += (th - T(1)) * h;
                 rd = rdisk;
                 cls = SIRIUS_RAY_DISK;
                 break;
@@ -859,6 +944,21 @@ hipError_t launch_symplectic(bool fp64, int family, const CamConst* cams, const 
             default: hipLaunchKernelGGL((k_symplectic_grid<float, 0>), g, b, 0, st, cams, tp, o); break;
         }
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_precision_select(const DevOut& o, int64_t n, double fraction, uint64_t seed, int32_t* list,
+                                   uint32_t* count, hipStream_t st) {
+    const uint64_t thr = fraction >= 1.0 ? ~0ull : (uint64_t)(fraction * 18446744073709551615.0);
+    hipLaunchKernelGGL(k_precision_select, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, o, n, thr, seed, list,
+                       count);
+    return hipGetLastError();
+}
+hipError_t launch_precision_compare(const CamConst* cams, const TraceParams& tp, const DevOut& oa, const DevOut& o,
+                                    const int32_t* list, const uint32_t* count, int64_t cap, unsigned int* acc,
+                                    unsigned int* hist, hipStream_t st) {
+    hipLaunchKernelGGL(k_precision_compare, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, cams, tp, oa, o,
+                       list, count, cap, acc, hist);
     return hipGetLastError();
 }
 

@@ -83,18 +83,64 @@ def test_symplectic_matches_oracle_and_reports_null_drift(acc):
     dk = ref.cls == abi.RAY_DISK
     assert np.allclose(got.intensity[:, dk], ref.intensity[:, dk], rtol=1e-6, atol=1e-12)
     assert np.array_equal(got.steps, ref.steps)
-    # the null-constraint drift is reported per ray and stays bounded. Its value is sensitive to
-    # rounding: the device's reported drift runs ~10-30x the oracle's on some rays, while the
-    # same device code executed on the host reproduces the oracle's to a few per cent
+    # the null-constraint drift is reported per ray (disk rays at the Hermite-located crossing,
+    # escaped rays at the escape step) and matches the oracle's ray by ray: both run the same
+    # fixed-step arithmetic, and rounding moves a drift of ~1e-9 by a few per cent at most
     live = esc | dk
-    assert st["null_drift_max"] > 0 and np.median(got.null_drift[live]) < 1e-7
-    assert got.null_drift[live].max() < 1e-4 and ref.null_drift[live].max() < 1e-4
+    gd, rd = got.null_drift[live], ref.null_drift[live]
+    assert st["null_drift_max"] > 0 and np.median(gd) < 1e-8
+    assert gd.max() < 1e-5 and rd.max() < 1e-5
+    rel = np.abs(gd - rd) / np.maximum(rd, 1e-14)
+    assert np.percentile(rel, 95) < 0.05, np.percentile(rel, 95)
+    assert abs(np.median(gd) / np.median(rd) - 1) < 0.02 and abs(gd.max() / rd.max() - 1) < 0.05
     # the same rays by DP45: the symplectic result agrees to its step error
     od = O.default_opts(abi.PRECISION_FP64)
     od.r_escape = 60.0
     dp, _ = acc.trace_batch([cam], [m], od)
     both = esc & (dp.cls == abi.RAY_ESCAPED)
     assert np.percentile(_angle(got.direction()[both], dp.direction()[both]), 90) < 1e-4
+
+
+def test_symplectic_1080p_kerr_frame_matches_oracle_on_a_pixel_sample(acc):
+    """The 1920x1080 Kerr a = 0.9 camera (r_obs 50 M, inclination 75 deg, FOV 20 deg, disk from the
+    ISCO to 30 M) by the fixed-step symplectic integrator, h = 0.01 M, FP64, escaping at 1000 M
+    (about 1e5 steps per escaped ray). A fixed sample of pixels (a stride over the frame plus
+    pixels in its top corners, where rays escape) is traced by the oracle: classes
+    identical, escape directions, disk intensity and the per-ray null-constraint drift agree."""
+    m = abi.metric_ks(1.0, 0.9)
+    W, H = 1920, 1080
+    cam = abi.camera(50.0, 75.0, 20.0, W, H, disk=(0.0, 30.0))
+    o = O.default_opts(abi.PRECISION_FP64)
+    o.integrator = abi.INTEGRATOR_SYMPLECTIC
+    o.symplectic_h = 0.01
+    o.max_steps = 400000
+    print("[case] symplectic fp64 1920x1080 h=0.01", flush=True)
+    got, st = acc.trace_batch([cam], [m], o)
+    assert (got.cls == abi.RAY_MAX_STEPS).sum() == 0 and (got.cls == abi.RAY_FAILED).sum() == 0
+    # the frame's escaped rays lie in its top corners (1.5 % of the frame)
+    corners = [y * W + x for y in (0, 20, 40) for x in (0, 40, 80, 120, W - 121, W - 81, W - 41, W - 1)]
+    ids = np.unique(np.concatenate([np.arange(0, W * H, 20011), corners])).astype(np.int64)
+    ref, _ = O.trace([cam], [m], o, ray_ids=ids)
+    assert np.array_equal(got.cls[ids], ref.cls)
+    esc = ref.cls == abi.RAY_ESCAPED
+    dk = ref.cls == abi.RAY_DISK
+    assert esc.sum() >= 10 and dk.sum() >= 50
+    assert _angle(got.direction()[ids][esc], ref.direction()[esc]).max() < 1e-6
+    assert np.allclose(got.intensity[:, ids][:, dk], ref.intensity[:, dk], rtol=1e-6, atol=1e-12)
+    # disk rays (~5e3 steps): the drift is the integrator's truncation, the same on both sides;
+    # escaped rays (~1e5 steps to 1000 M, where the metric is nearly flat): it is accumulated
+    # rounding, which differs with every fused multiply-add, so both are only bounded
+    gd, rd = got.null_drift[ids][dk], ref.null_drift[dk]
+    rel = np.abs(gd - rd) / np.maximum(rd, 1e-14)
+    assert np.percentile(rel, 90) < 0.05, np.percentile(rel, 90)
+    assert abs(np.median(gd) / np.median(rd) - 1) < 0.05
+    ge, re_ = got.null_drift[ids][esc], ref.null_drift[esc]
+    print(f"[symplectic 1080p] escaped drift device max {ge.max():.2e}, oracle max {re_.max():.2e}", flush=True)
+    assert ge.max() < 1e-8 and re_.max() < 1e-8
+    # the frame's drift statistics: median over every ray, and the largest
+    assert 0 < st["null_drift_median"] < 1e-8 and st["null_drift_max"] < 1e-2
+    print(f"[symplectic 1080p] steps {st['steps_accepted']}, drift median {st['null_drift_median']:.2e}"
+          f" max {st['null_drift_max']:.2e}", flush=True)
 
 
 @pytest.mark.parametrize("prec", [abi.PRECISION_FP64, abi.PRECISION_FP32_ESCALATE])
