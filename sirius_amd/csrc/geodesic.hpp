@@ -691,22 +691,26 @@ SIRIUS_HD __forceinline__ T dense_slope(T y0, T y1, T h, T k0, T k1, T r5, T th)
 // Event location on the dense output by safeguarded Newton iteration: the root of the event
 // function over th in [0, 1], where it changes sign. Each iterate stays inside the bracket that
 // the signs seen so far keep (a step leaving it is replaced by the bracket's midpoint), starting
-// from the linear interpolant's root. The event functions are smooth across a step (the plane
-// z = 0, or a sphere of the event radius), so a few iterations reach the working precision
-// that a bisection reaches in 24 (FP32) halvings, for a fraction of the instructions: a wave
-// runs this whenever one of its lanes has an event in the step.
+// from the linear interpolant's root. The bracket is closed: a converged iterate has just become
+// one of its ends, and its Newton update (itself, to rounding) is kept, not bisected away. The
+// iteration ends when a Newton step moves th by less than tol, or after 24 rounds (each of them
+// at least as good as a halving: a step falling back to the midpoint halves the bracket, one
+// staying inside converges quadratically). The event functions are smooth across a step (the
+// plane z = 0, or a sphere of the event radius), so most crossings converge in two or three
+// rounds, for a fraction of the instructions of 24 halvings: a wave runs this whenever one of its
+// lanes has an event in the step, for as many rounds as its slowest lane needs.
 template <class T>
 SIRIUS_HD __forceinline__ T locate_event_newton(const Spacetime<T>& s, int kind, T target, const T x0[4],
                                                 const T k0[4], const T x1[4], const T k1[4], const T dn[6], T h,
                                                 T f0, T f1) {
-    constexpr int iters = 5;
+    constexpr int max_iters = 24;
+    const T tol = T(sizeof(T) == 4 ? 2e-7 : 1e-15);
     T lo = T(0), hi = T(1);
     const bool neg_lo = f0 < T(0);
     T den = f0 - f1;
     T th = den != T(0) ? f0 / den : T(0.5);
     th = th > T(0) && th < T(1) ? th : T(0.5);
-#pragma unroll
-    for (int it = 0; it < iters; ++it) {
+    for (int it = 0; it < max_iters; ++it) {
         T f, df;
         if (kind == EV_Z) {
             f = dense_eval(x0[3], x1[3], h, k0[3], k1[3], dn[2], th);
@@ -720,9 +724,16 @@ SIRIUS_HD __forceinline__ T locate_event_newton(const Spacetime<T>& s, int kind,
             f = event_radius_rate(s, xm, v, rate) - target;
             df = rate;
         }
+        if (f == T(0)) break;
         if ((f < T(0)) == neg_lo) lo = th; else hi = th;
-        T nt = df != T(0) ? th - f / df : T(-1);
-        th = (nt > lo && nt < hi) ? nt : T(0.5) * (lo + hi);
+        const T nt = df != T(0) ? th - f / df : T(-1);
+        if (nt >= lo && nt <= hi) {
+            const bool done = fabs(nt - th) <= tol;
+            th = nt;
+            if (done) break;
+        } else {
+            th = T(0.5) * (lo + hi);
+        }
     }
     return th;
 }
