@@ -266,6 +266,23 @@ int sirius_hip_trace_batch(SiriusHipHandle h, const SiriusHipCamera* cams, int n
                            const SiriusHipMetric* metrics, int n_metric,
                            const SiriusHipTraceOpts* opts, SiriusHipRayOutput* out,
                            SiriusHipStats* stats);
+/* sirius_hip_trace_batch without allocation: the same kernels, with every device scratch
+ * buffer carved from one fixed array of 256 MiB that the library holds per GPU until the process
+ * ends, so the call makes no hipMalloc / hipFree (nor any other device or pinned-host allocation).
+ * Has no counterpart in the reference; it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ *  - out->on_device must be 1: results go to the caller's device arrays. A null array is
+ *    computed in the scratch and discarded; sirius_hip_fetch_results then has nothing to fetch.
+ *  - Limit: the batch's scratch (88 bytes per ray with every result array supplied except state;
+ *    24 with state supplied too; more for volumetric emission and for arrays left null) must fit
+ *    in 256 MiB, else SIRIUS_HIP_E_INVALID before any work. A 1920x1080 frame fits.
+ *  - use_surrogate is not supported (SIRIUS_HIP_E_UNSUPPORTED).
+ *  - Every call on one GPU, from any handle, shares the array and leaves its contents behind
+ *    (each call initialises what it reads); calls that overlap on one GPU race. */
+int sirius_hip_trace_batch_noalloc(SiriusHipHandle h, const SiriusHipCamera* cams, int n_cam,
+                                   const SiriusHipMetric* metrics, int n_metric,
+                                   const SiriusHipTraceOpts* opts, SiriusHipRayOutput* out,
+                                   SiriusHipStats* stats);
 /* Distance (M) of each ray from the critical curve in the (alpha, beta) plane, measured
  * radially from the curve's centre (alpha_0, 0): rho - rho_c(psi), negative inside the shadow.
  * alpha = -L/(E sin th), beta = p_th/E at the ray's start. Rays are all pixels of the cameras,

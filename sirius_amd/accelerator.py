@@ -55,6 +55,7 @@ def lib():
     L.sirius_hip_trace_batch.argtypes = [H, C.POINTER(abi.Camera), i, C.POINTER(abi.Metric), i,
                                          C.POINTER(abi.TraceOpts), C.POINTER(abi.RayOutput),
                                          C.POINTER(abi.Stats)]
+    L.sirius_hip_trace_batch_noalloc.argtypes = L.sirius_hip_trace_batch.argtypes
     L.sirius_hip_critical_distance.argtypes = [H, C.POINTER(abi.Camera), i, C.POINTER(abi.Metric), i,
                                                C.c_void_p, i]
     L.sirius_hip_autotune.argtypes = [H, C.POINTER(abi.Camera), i, C.POINTER(abi.Metric), i,

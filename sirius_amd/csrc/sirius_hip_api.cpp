@@ -314,6 +314,24 @@ int Handle::ensure_device() {
 }
 
 void* Handle::scratch(const char* name, size_t bytes, int* rc) {
+    if (arena) {
+        // no-allocation mode: the same name within one call is the same sub-buffer
+        for (int i = 0; i < arena_n; ++i)
+            if (std::strcmp(arena_slots[i].name, name) == 0) {
+                if (arena_slots[i].bytes >= bytes) return arena + arena_slots[i].off;
+                *rc = fail(SIRIUS_HIP_E_INVALID, std::string("no-allocation scratch: buffer ") + name + " grew");
+                return nullptr;
+            }
+        size_t off = (arena_used + 255) & ~(size_t)255;
+        if (arena_n >= kArenaSlots || off + bytes > arena_cap) {
+            *rc = fail(SIRIUS_HIP_E_INVALID, "batch exceeds the no-allocation scratch (" +
+                                                 std::to_string(arena_cap >> 20) + " MiB)");
+            return nullptr;
+        }
+        arena_slots[arena_n++] = {name, off, bytes};
+        arena_used = off + bytes;
+        return arena + off;
+    }
     Buf& b = bufs[name];
     if (b.bytes >= bytes && b.ptr) return b.ptr;
     // a replaced buffer no longer holds an upload (a new allocation may reuse its address)

@@ -97,6 +97,17 @@ struct Handle {
     size_t pinned_bytes = 0;
     // surrogate network
     MlpState* mlp = nullptr;
+    // No-allocation mode (sirius_hip_trace_batch_noalloc): while `arena` is set, scratch()
+    // carves named sub-buffers from that fixed device region instead of allocating.
+    struct ArenaSlot {
+        const char* name;
+        size_t off, bytes;
+    };
+    static constexpr int kArenaSlots = 64;
+    unsigned char* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
+    ArenaSlot arena_slots[kArenaSlots];
+    int arena_n = 0;
 
     ~Handle();
     void release();
