@@ -120,19 +120,22 @@ hipError_t launch_gemm(int epi, const GemmArgs& g, int nsplit, hipStream_t st) {
 // v_mfma_f32_32x32x16_bf16 (A 32 x 16 and B 16 x 32 BF16 operands, 8 values per lane: lane
 // l holds row / column l % 32 and k = 8 (l / 32) .. +7; C 32 x 32 FP32, 16 values per lane:
 // column l % 32, rows 8 (r / 4) + 4 (l / 32) + r % 4).
-//  - The tile's activations stay in LDS ([128][264] BF16, rows padded by 8 values so that the
-//    16-byte row reads of eight consecutive lanes cover every bank once): each layer's MFMAs read
-//    their A operand (rows x k) there, and the bias + GELU epilogue writes the layer's BF16
-//    output back in place, between two barriers.
-//  - Wave w computes output neurons 64 w .. 64 w + 63 of all 128 rows (4 x 2 tiles of 32 x 32,
-//    128 accumulator registers); its B operand (weights W[j][k], 16 contiguous bytes per lane)
-//    streams from L2 through the vector cache, one k-step ahead of the MFMAs.
+//  - Each layer is computed transposed, H'^T = W H^T: the weights are the A operand (W[j][k], 16
+//    contiguous bytes per lane, streamed from L2 one k-step ahead of the MFMAs) and the tile's
+//    activations the B operand, read from LDS ([128][264] BF16, row = sample; rows padded by 8
+//    values so that the 16-byte reads of eight consecutive lanes cover every bank once).
+//  - A lane's accumulators then hold, for one sample, runs of four consecutive neurons: the bias
+//    + GELU epilogue converts each run to BF16 with the hardware's paired conversion and writes
+//    it back to the sample's LDS row with one 8-byte store, between two barriers.
+//  - Wave w computes neurons 64 w .. 64 w + 63 for all 128 samples (2 x 4 tiles of 32 x 32, 128
+//    accumulator registers).
 //  - The output head (256 -> 6) keeps FP32 precision with three BF16 products: the last hidden
 //    activation h = hi + lo and the head's weights W = W_hi + W_lo, each split into BF16 parts,
-//    and y = hi W_hi + hi W_lo + lo W_hi (the lo W_lo term is below FP32's resolution of y).
+//    and y = W_hi hi + W_lo hi + W_hi lo (the W_lo lo term is below FP32's resolution of y).
 //    hi, then lo, pass through the same LDS buffer.
 // ------------------------------------------------------------------------------------------
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
 
 constexpr int MF_ROWS = 128;
 constexpr int MF_PITCH = 264;
@@ -141,133 +144,149 @@ __device__ __forceinline__ v8bf ld8(const uint16_t* p) { return *(const v8bf*)p;
 
 __global__ __launch_bounds__(256, 2) void k_mlp_fused(MlpFusedArgs a) {
     __shared__ __attribute__((aligned(16))) uint16_t act[MF_ROWS * MF_PITCH];
+    __shared__ __attribute__((aligned(16))) float bias_s[4 * MLP_H];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int li = lane & 31, lh = lane >> 5;
+    for (int e = t; e < 4 * MLP_H; e += 256) bias_s[e] = a.b[e / MLP_H][e % MLP_H];
     const int64_t n = a.n_dev ? (int64_t)*a.n_dev : a.n;
+    const int j0 = wave * 64;  // this wave's neurons
     // tiles of 128 rows, grid-stride (the row count may live on the device)
     for (int64_t row0 = (int64_t)blockIdx.x * MF_ROWS; row0 < n; row0 += (int64_t)gridDim.x * MF_ROWS) {
-    __syncthreads();  // the previous tile's last reads of act are done
-    // layer-0 input: x padded to 16 columns (row r of the tile is x[idx[r]] when gathered)
-    for (int e = t; e < MF_ROWS * 16; e += 256) {
-        const int r = e >> 4, k = e & 15;
-        const int64_t gr = row0 + r;
-        float v = 0.f;
-        if (k < SIRIUS_MLP_IN && gr < n) v = a.x[(a.idx ? (int64_t)a.idx[gr] : gr) * a.x_stride + k];
-        act[r * MF_PITCH + k] = f2bf(v);
-    }
-    __syncthreads();
-    const int col0 = wave * 64;
-    v16f acc[4][2];
-    uint32_t lo_pk[4][2][8];  // low BF16 parts of the last hidden activation
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < 2; ++nt)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[mt][nt][r] = 0.f;
-    {
-        v8bf b[2];
-#pragma unroll
-        for (int nt = 0; nt < 2; ++nt) b[nt] = ld8(a.w0 + (col0 + nt * 32 + li) * 16 + 8 * lh);
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt) {
-            const v8bf av = ld8(&act[(mt * 32 + li) * MF_PITCH + 8 * lh]);
-#pragma unroll
-            for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b[nt], acc[mt][nt], 0, 0, 0);
-        }
-    }
-    for (int l = 0; l < 4; ++l) {
-        if (l > 0) {
-            const uint16_t* W = a.w[l - 1];
-#pragma unroll
-            for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-                for (int nt = 0; nt < 2; ++nt)
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) acc[mt][nt][r] = 0.f;
-            v8bf b[2], bn[2];
-#pragma unroll
-            for (int nt = 0; nt < 2; ++nt) b[nt] = ld8(W + (col0 + nt * 32 + li) * MLP_H + 8 * lh);
-#pragma unroll
-            for (int ks = 0; ks < MLP_H / 16; ++ks) {
-                const int k0 = ks * 16;
-                if (ks + 1 < MLP_H / 16) {
-#pragma unroll
-                    for (int nt = 0; nt < 2; ++nt) bn[nt] = ld8(W + (col0 + nt * 32 + li) * MLP_H + k0 + 16 + 8 * lh);
-                }
-#pragma unroll
-                for (int mt = 0; mt < 4; ++mt) {
-                    const v8bf av = ld8(&act[(mt * 32 + li) * MF_PITCH + k0 + 8 * lh]);
-#pragma unroll
-                    for (int nt = 0; nt < 2; ++nt)
-                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b[nt], acc[mt][nt], 0, 0, 0);
-                }
-                if (ks + 1 < MLP_H / 16) {
-#pragma unroll
-                    for (int nt = 0; nt < 2; ++nt) b[nt] = bn[nt];
-                }
-            }
-        }
-        __syncthreads();  // every wave has read this layer's input
-        // bias + GELU; the hidden layers' BF16 outputs (the last one's high parts) back into LDS,
-        // the last one's low parts kept in registers, two BF16 values to a register
-        const float* bias = a.b[l];
-#pragma unroll
-        for (int nt = 0; nt < 2; ++nt) {
-            const int j = col0 + nt * 32 + li;
-            const float bj = bias[j];
-#pragma unroll
-            for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const float h = gelu_f(acc[mt][nt][r] + bj);
-                    const uint16_t hi = f2bf(h);
-                    act[(mt * 32 + 8 * (r >> 2) + 4 * lh + (r & 3)) * MF_PITCH + j] = hi;
-                    if (l == 3) {
-                        const uint32_t lo = f2bf(h - __uint_as_float((uint32_t)hi << 16));
-                        lo_pk[mt][nt][r >> 1] = (r & 1) ? (lo_pk[mt][nt][r >> 1] | (lo << 16)) : lo;
-                    }
-                }
+        __syncthreads();  // bias_s written; the previous tile's last reads of act are done
+        // layer-0 input: x padded to 16 columns (row r of the tile is x[idx[r]] when gathered)
+        for (int e = t; e < MF_ROWS * 16; e += 256) {
+            const int r = e >> 4, k = e & 15;
+            const int64_t gr = row0 + r;
+            float v = 0.f;
+            if (k < SIRIUS_MLP_IN && gr < n) v = a.x[(a.idx ? (int64_t)a.idx[gr] : gr) * a.x_stride + k];
+            act[r * MF_PITCH + k] = f2bf(v);
         }
         __syncthreads();
-    }
-    // output head, rows 32 w .. 32 w + 31 of the tile: hi W_hi + hi W_lo, then lo W_hi
-    v16f y;
+        v16f acc[2][4];
+        uint32_t lo_pk[2][4][8];  // low BF16 parts of the last hidden activation, four to two registers
+        // the accumulators start at the layer's bias (row 8 q + 4 lh + e of tile mt is neuron
+        // j0 + 32 mt + 8 q + 4 lh + e)
+        auto init_bias = [&](int l) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) y[r] = 0.f;
+            for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-    for (int ks = 0; ks < MLP_H / 16; ++ks) {
-        const int k0 = ks * 16;
-        const v8bf av = ld8(&act[(wave * 32 + li) * MF_PITCH + k0 + 8 * lh]);
-        y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, ld8(a.w4h + li * MLP_H + k0 + 8 * lh), y, 0, 0, 0);
-        y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, ld8(a.w4l + li * MLP_H + k0 + 8 * lh), y, 0, 0, 0);
-    }
-    __syncthreads();  // every wave has read the high parts
+                for (int q = 0; q < 4; ++q) {
+                    const float4 bb = *(const float4*)&bias_s[l * MLP_H + j0 + mt * 32 + 8 * q + 4 * lh];
 #pragma unroll
-    for (int nt = 0; nt < 2; ++nt) {
-        const int j = col0 + nt * 32 + li;
+                    for (int nt = 0; nt < 4; ++nt) {
+                        acc[mt][nt][4 * q + 0] = bb.x;
+                        acc[mt][nt][4 * q + 1] = bb.y;
+                        acc[mt][nt][4 * q + 2] = bb.z;
+                        acc[mt][nt][4 * q + 3] = bb.w;
+                    }
+                }
+        };
+        init_bias(0);
+        {
+            v8bf w[2];
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
+            for (int mt = 0; mt < 2; ++mt) w[mt] = ld8(a.w0 + (j0 + mt * 32 + li) * 16 + 8 * lh);
 #pragma unroll
-            for (int r = 0; r < 16; ++r)
-                act[(mt * 32 + 8 * (r >> 2) + 4 * lh + (r & 3)) * MF_PITCH + j] =
-                    (uint16_t)(lo_pk[mt][nt][r >> 1] >> (16 * (r & 1)));
-    }
-    __syncthreads();
+            for (int nt = 0; nt < 4; ++nt) {
+                const v8bf hv = ld8(&act[(nt * 32 + li) * MF_PITCH + 8 * lh]);
 #pragma unroll
-    for (int ks = 0; ks < MLP_H / 16; ++ks) {
-        const int k0 = ks * 16;
-        const v8bf av = ld8(&act[(wave * 32 + li) * MF_PITCH + k0 + 8 * lh]);
-        y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, ld8(a.w4h + li * MLP_H + k0 + 8 * lh), y, 0, 0, 0);
-    }
-    if (li < SIRIUS_MLP_OUT) {
-        const float bo = a.b4[li];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int64_t gr = row0 + wave * 32 + 8 * (r >> 2) + 4 * lh + (r & 3);
-            if (gr < n) a.y[gr * SIRIUS_MLP_OUT + li] = y[r] + bo;
+                for (int mt = 0; mt < 2; ++mt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[mt], hv, acc[mt][nt], 0, 0, 0);
+            }
         }
-    }
+        for (int l = 0; l < 4; ++l) {
+            if (l > 0) {
+                const uint16_t* Wl = a.w[l - 1];
+                init_bias(l);
+                v8bf w[2], wn[2];
+#pragma unroll
+                for (int mt = 0; mt < 2; ++mt) w[mt] = ld8(Wl + (j0 + mt * 32 + li) * MLP_H + 8 * lh);
+#pragma unroll
+                for (int ks = 0; ks < MLP_H / 16; ++ks) {
+                    const int k0 = ks * 16;
+                    if (ks + 1 < MLP_H / 16) {
+#pragma unroll
+                        for (int mt = 0; mt < 2; ++mt) wn[mt] = ld8(Wl + (j0 + mt * 32 + li) * MLP_H + k0 + 16 + 8 * lh);
+                    }
+#pragma unroll
+                    for (int nt = 0; nt < 4; ++nt) {
+                        const v8bf hv = ld8(&act[(nt * 32 + li) * MF_PITCH + k0 + 8 * lh]);
+#pragma unroll
+                        for (int mt = 0; mt < 2; ++mt)
+                            acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[mt], hv, acc[mt][nt], 0, 0, 0);
+                    }
+                    if (ks + 1 < MLP_H / 16) {
+#pragma unroll
+                        for (int mt = 0; mt < 2; ++mt) w[mt] = wn[mt];
+                    }
+                }
+            }
+            __syncthreads();  // every wave has read this layer's input
+            // GELU on runs of four neurons; BF16 (the last layer: its high parts) back into the
+            // sample's LDS row, the last layer's low parts kept in registers
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int jb = j0 + mt * 32 + 8 * q + 4 * lh;
+#pragma unroll
+                    for (int nt = 0; nt < 4; ++nt) {
+                        const float h0 = gelu_f(acc[mt][nt][4 * q + 0]), h1 = gelu_f(acc[mt][nt][4 * q + 1]);
+                        const float h2 = gelu_f(acc[mt][nt][4 * q + 2]), h3 = gelu_f(acc[mt][nt][4 * q + 3]);
+                        const v4bf hv = {(__bf16)h0, (__bf16)h1, (__bf16)h2, (__bf16)h3};
+                        *(v4bf*)&act[(nt * 32 + li) * MF_PITCH + jb] = hv;
+                        if (l == 3) {
+                            const v4bf lv = {(__bf16)(h0 - (float)hv[0]), (__bf16)(h1 - (float)hv[1]),
+                                             (__bf16)(h2 - (float)hv[2]), (__bf16)(h3 - (float)hv[3])};
+                            uint32_t u[2];
+                            __builtin_memcpy(u, &lv, 8);
+                            lo_pk[mt][nt][2 * q] = u[0];
+                            lo_pk[mt][nt][2 * q + 1] = u[1];
+                        }
+                    }
+                }
+            __syncthreads();
+        }
+        // output head, samples 32 w .. 32 w + 31 of the tile: W_hi hi + W_lo hi, then W_hi lo
+        v16f y;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) y[r] = 0.f;
+#pragma unroll
+        for (int ks = 0; ks < MLP_H / 16; ++ks) {
+            const int k0 = ks * 16;
+            const v8bf hv = ld8(&act[(wave * 32 + li) * MF_PITCH + k0 + 8 * lh]);
+            y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(a.w4h + li * MLP_H + k0 + 8 * lh), hv, y, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(a.w4l + li * MLP_H + k0 + 8 * lh), hv, y, 0, 0, 0);
+        }
+        __syncthreads();  // every wave has read the high parts
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int jb = j0 + mt * 32 + 8 * q + 4 * lh;
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) {
+                    const uint32_t u[2] = {lo_pk[mt][nt][2 * q], lo_pk[mt][nt][2 * q + 1]};
+                    __builtin_memcpy(&act[(nt * 32 + li) * MF_PITCH + jb], u, 8);
+                }
+            }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < MLP_H / 16; ++ks) {
+            const int k0 = ks * 16;
+            const v8bf hv = ld8(&act[(wave * 32 + li) * MF_PITCH + k0 + 8 * lh]);
+            y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(a.w4h + li * MLP_H + k0 + 8 * lh), hv, y, 0, 0, 0);
+        }
+        // y^T: this lane's sample is 32 w + li, its outputs are rows 4 lh + r (r < 4) of the first
+        // 8-row group
+        const int64_t gr = row0 + wave * 32 + li;
+        if (gr < n) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int o = 4 * lh + r;
+                if (o < SIRIUS_MLP_OUT) a.y[gr * SIRIUS_MLP_OUT + o] = y[r] + a.b4[o];
+            }
+        }
     }
 }
 

@@ -101,7 +101,7 @@ def test_surrogate_c4_training_decision_audit_and_ablation(acc):
     lab = default_opts(abi.PRECISION_FP64)
     acc.mlp_init(0x51121006)
     print("[case] surrogate train C4 2^24 rays", flush=True)
-    loss = acc.surrogate_train(cams, metrics, lab, 40, 8192, 2e-3, 0x51121006)
+    loss = acc.surrogate_train(cams, metrics, lab, 200, 8192, 2e-3, 0x51121006)
     assert np.isfinite(loss)
     err, n, bad = _calibration(acc)
     usable = (n >= 20) & (bad == 0) & (err <= lab.surrogate_err_max)
