@@ -291,9 +291,9 @@ def _surrogate_calibration(acc):
     L = acc._L
     L.sirius_hip_surrogate_calibration.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                                    C.POINTER(C.c_int)]
-    err, n, bad = np.zeros(512, np.float32), np.zeros(512, np.uint32), np.zeros(512, np.uint32)
+    err, n, bad = np.zeros(1024, np.float32), np.zeros(1024, np.uint32), np.zeros(1024, np.uint32)
     nb = C.c_int()
-    L.sirius_hip_surrogate_calibration(acc._h, err.ctypes.data, n.ctypes.data, bad.ctypes.data, 512, C.byref(nb))
+    L.sirius_hip_surrogate_calibration(acc._h, err.ctypes.data, n.ctypes.data, bad.ctypes.data, 1024, C.byref(nb))
     return err[:nb.value], n[:nb.value], bad[:nb.value]
 
 

@@ -210,7 +210,8 @@ typedef struct SiriusHipStats {
     double audit_dir_err_max;    /* largest escape-direction error of audited answers (rad) */
     uint64_t audit_class_mismatch; /* audited answers whose class the integrator disputes    */
     double surrogate_flops;      /* algorithmic flops of the network inference               */
-    uint64_t rays_candidates;    /* rays the surrogate's analytic test passed to the network  */
+    uint64_t rays_candidates;    /* rays the surrogate's analytic test passed to the network
+                                    (only rays in a calibration bin that answers)            */
     uint64_t rays_gated;         /* candidates the calibrated error gate left to the integrator */
     uint64_t precision_audited;  /* rays of the FP64 precision audit (precision_audit)        */
     uint64_t precision_class_mismatch; /* audited rays whose FP32 class differs from FP64's    */
@@ -348,7 +349,7 @@ int sirius_hip_surrogate_features(SiriusHipHandle h, const SiriusHipCamera* cams
                                   const SiriusHipMetric* metrics, int n_metric, float* x, float* ref);
 /* Trains the network on the candidate rays of the cameras, labelled by the integrator with
  * label_opts (FP64; its surrogate_band, surrogate_margin and r_escape define the candidates):
- * inputs and targets are built on the device; a seeded 1 in 16 of the candidates is held out,
+ * inputs and targets are built on the device; a seeded 1 in 4 of the candidates is held out,
  * the rest trained on for `epochs` passes of Adam over batches of `batch` rays in a seeded
  * random order, at learning rate lr with a cosine decay to lr/20; then the held-out rays
  * calibrate the error gate (label_opts' surrogate_pmin). *loss (host) receives the mean batch

@@ -265,7 +265,8 @@ hipError_t launch_sur_targets(const DevOut& o, const float* ref, int64_t n, int3
 hipError_t launch_sur_gate(const CamConst* cams, int n_cam, const float* tabs, int64_t n, double band, double crit_band,
                            double long_band, double sur_band, double margin, double r_exit, int32_t* list_a,
                            int32_t* list_b, int32_t* list_l, int32_t* list_s, uint32_t* counts, uint8_t* flags,
-                           float* rec, hipStream_t st);
+                           float* rec, const float* cal_err, const uint32_t* cal_n, const uint32_t* cal_bad,
+                           double err_max, hipStream_t st);
 hipError_t launch_sur_decide(const int32_t* list_s, const uint32_t* count_s, int64_t cap, const float* y,
                              const float* rec, const float* cal_err, const uint32_t* cal_n, const uint32_t* cal_bad,
                              double pmin, double err_max, double audit_fraction, uint64_t audit_seed, const DevOut& o,
@@ -294,7 +295,8 @@ int surrogate_stage(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& do
     hipError_t e;
     if ((e = launch_sur_gate(B.cams_dev, (int)B.cc.size(), tabs, n, std::max(o.escalate_band, 0.0),
                              std::max(o.critical_band, 0.0), 1.0, o.surrogate_band, o.surrogate_margin, o.r_escape,
-                             list_a, list_b, list_l, list_s, work + 16, dout.flags, rec, H.stream)) != hipSuccess)
+                             list_a, list_b, list_l, list_s, work + 16, dout.flags, rec, S.cal_err, S.cal_n, S.cal_bad,
+                             o.surrogate_err_max, H.stream)) != hipSuccess)
         return H.hip_fail(e, "surrogate gate");
     MlpFusedArgs a = mlp_fused_args(S, rec, y, 0);
     a.n_dev = work + 19;
@@ -474,7 +476,7 @@ int sirius_hip_surrogate_train(SiriusHipHandle hh, const SiriusHipCamera* cams, 
                                  H->stream)) != hipSuccess ||
         (e = launch_sur_targets(dout, ref, n, cls, dir, H->stream)) != hipSuccess)
         return H->hip_fail(e, "training set");
-    // training and held-out candidates: a seeded 1 in 16 of them calibrates the error gate
+    // training and held-out candidates: a seeded 1 in 4 of them calibrates the error gate
     std::vector<uint8_t> c((size_t)n);
     if ((e = hipMemcpyAsync(c.data(), cand, (size_t)n, hipMemcpyDeviceToHost, H->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(H->stream)) != hipSuccess)
@@ -483,7 +485,7 @@ int sirius_hip_surrogate_train(SiriusHipHandle hh, const SiriusHipCamera* cams, 
     for (int64_t i = 0; i < n; ++i) {
         if (!c[(size_t)i]) continue;
         uint64_t z = seed ^ (0xD1B54A32D192ED03ULL * (uint64_t)(i + 1));
-        (((splitmix(z) >> 32) & 15u) == 0 ? ho : tr).push_back((int32_t)i);
+        (((splitmix(z) >> 32) & 3u) == 0 ? ho : tr).push_back((int32_t)i);
     }
     if (tr.empty()) return H->fail(SIRIUS_HIP_E_INVALID, "no ray of these cameras is a surrogate candidate");
     const int64_t nt = (int64_t)tr.size(), nh = (int64_t)ho.size();

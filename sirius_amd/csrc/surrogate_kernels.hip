@@ -32,18 +32,39 @@ __device__ __forceinline__ int sur_bin(float a, float d, float r_obs) {
     return (ia * 9 + id) * 3 + ir;
 }
 
+// The bin of a ray from its record (spin rec[0], distance rec[5], scaled observer radius rec[2]).
+__device__ __forceinline__ int sur_bin_rec(const float* rr) { return sur_bin(rr[0], rr[5], rr[2] * 40.f + 60.f); }
+
+// Calibration of the error gate: a bin answers when its held-out rays number at least
+// SUR_CAL_MIN, none was misclassified, and the largest direction error is within err_max.
+struct SurCal {
+    const float* err;
+    const uint32_t* n;
+    const uint32_t* bad;
+    float err_max;
+};
+__device__ __forceinline__ bool sur_bin_usable(const SurCal& cal, int b) {
+    return cal.n[b] >= SUR_CAL_MIN && cal.bad[b] == 0u && cal.err[b] <= cal.err_max;
+}
+
 // The analytic reference of one ray (FP32, from the camera): its signed distance from the
 // critical curve, whether the surrogate may be asked (outside the fallback band and the shadow,
-// its Schwarzschild orbit scattered and crossing the equatorial plane only outside the thin disk
-// widened by `margin`), and for such a candidate the record rec[SUR_REC] (network inputs,
-// distance, b, reference exit direction). Returns the candidate decision.
+// in a bin of `cal` that answers when cal is given, its Schwarzschild orbit scattered and crossing
+// the equatorial plane only outside the thin disk widened by `margin`), and for such a candidate
+// the record rec[SUR_REC] (network inputs, distance, b, reference exit direction). Returns the
+// candidate decision.
 __device__ __forceinline__ bool sur_reference(const CamConst& c, const float* __restrict__ tab, int px, int py,
-                                              float sur_band, float margin, float r_exit, float& dist, float* rec) {
+                                              float sur_band, float margin, float r_exit, float& dist, float* rec,
+                                              const SurCal* cal = nullptr) {
     Spacetime<float> s = make_spacetime<float>(c);
     float x[4], p[4];
     init_state<float>(c, s, px, py, x, p);
     dist = crit_distance<float>(tab, s, x, p);
     if (!(dist >= sur_band)) return false;
+    // a ray whose calibration bin cannot answer skips the orbit test and the network (the bin
+    // from the same FP32 values the record carries)
+    if (cal && !sur_bin_usable(*cal, sur_bin((float)c.a, dist, (float)((c.r_obs - 60.0) / 40.0) * 40.f + 60.f)))
+        return false;
     float n[3];
     camera_cart_dir<float>(c, px, py, n);
     // to first order in a, frame dragging deflects a distant ray as if the lens sat a away from the
@@ -143,14 +164,16 @@ constexpr int SG_RAYS = 256 * SG_PASSES;
 
 // Gate of the FP32 path with the surrogate: like the plain gate (flags, band list B, long list L,
 // list A), with the network's candidates in list S; candidates leave their record in rec[id].
-// counts[0..3] = |A|, |B|, |L|, |S|.
+// Only rays in a bin whose calibration answers are candidates: with an uncalibrated network, or
+// a scene whose bins the calibration refuses, the network does not run. counts[0..3] = |A|, |B|,
+// |L|, |S|.
 __global__ __launch_bounds__(256) void k_sur_gate(const CamConst* __restrict__ cams, int n_cam,
                                                   const float* __restrict__ tabs, int64_t n, float band, float crit_band,
                                                   float long_band, float sur_band, float margin, float r_exit,
                                                   int32_t* __restrict__ list_a, int32_t* __restrict__ list_b,
                                                   int32_t* __restrict__ list_l, int32_t* __restrict__ list_s,
                                                   uint32_t* __restrict__ counts, uint8_t* __restrict__ flags,
-                                                  float* __restrict__ rec) {
+                                                  float* __restrict__ rec, SurCal cal) {
     __shared__ uint32_t off[4][SG_PASSES][4];
     __shared__ uint32_t base[4];
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
@@ -167,7 +190,8 @@ __global__ __launch_bounds__(256) void k_sur_gate(const CamConst* __restrict__ c
             const int64_t pix = id - c.offset;
             float d;
             const bool cand = sur_reference(c, tabs + (int64_t)cam * (SIRIUS_CC_TABLE + 1), (int)(pix % c.width),
-                                            (int)(pix / c.width), sur_band, margin, r_exit, d, rec + id * SUR_REC);
+                                            (int)(pix / c.width), sur_band, margin, r_exit, d, rec + id * SUR_REC,
+                                            &cal);
             const float ad = fabsf(d);
             flags[id] = ad < crit_band ? (uint8_t)SIRIUS_FLAG_CRITICAL : (uint8_t)0;
             kind = ad < band ? 1u : (cand ? 3u : (ad < long_band ? 2u : 0u));
@@ -236,8 +260,7 @@ __global__ __launch_bounds__(256) void k_sur_decide(const int32_t* __restrict__ 
             const float e0 = expf(yy[0] - m), e1 = expf(yy[1] - m), e2 = expf(yy[2] - m);
             const float p0 = e0 / (e0 + e1 + e2);
             const bool esc = e0 >= e1 && e0 >= e2 && p0 >= pmin;
-            const int b = sur_bin(rr[0], rr[5], rr[2] * 40.f + 60.f);
-            const bool cal = cal_n[b] >= SUR_CAL_MIN && cal_bad[b] == 0u && cal_err[b] <= err_max;
+            const bool cal = sur_bin_usable(SurCal{cal_err, cal_n, cal_bad, err_max}, sur_bin_rec(rr));
             gated = esc && !cal;
             if (esc && cal) {
                 uint8_t fl = o.flags[id] | (uint8_t)SIRIUS_FLAG_SURROGATE;
@@ -303,7 +326,7 @@ __global__ void k_sur_calibrate(const int32_t* __restrict__ list, int64_t m, con
     const float mx = fmaxf(yy[0], fmaxf(yy[1], yy[2]));
     const float e0 = expf(yy[0] - mx), e1 = expf(yy[1] - mx), e2 = expf(yy[2] - mx);
     if (!(e0 >= e1 && e0 >= e2 && e0 / (e0 + e1 + e2) >= pmin)) return;
-    const int b = sur_bin(rr[0], rr[5], rr[2] * 40.f + 60.f);
+    const int b = sur_bin_rec(rr);
     if (o.cls[id] != SIRIUS_RAY_ESCAPED) {
         atomicAdd(&cal_bad[b], 1u);
         return;
@@ -366,10 +389,12 @@ hipError_t launch_sur_targets(const DevOut& o, const float* ref, int64_t n, int3
 hipError_t launch_sur_gate(const CamConst* cams, int n_cam, const float* tabs, int64_t n, double band, double crit_band,
                            double long_band, double sur_band, double margin, double r_exit, int32_t* list_a,
                            int32_t* list_b, int32_t* list_l, int32_t* list_s, uint32_t* counts, uint8_t* flags,
-                           float* rec, hipStream_t st) {
+                           float* rec, const float* cal_err, const uint32_t* cal_n, const uint32_t* cal_bad,
+                           double err_max, hipStream_t st) {
     hipLaunchKernelGGL(k_sur_gate, dim3((unsigned)((n + SG_RAYS - 1) / SG_RAYS)), dim3(256), 0, st, cams, n_cam, tabs, n,
                        (float)band, (float)crit_band, (float)long_band, (float)sur_band, (float)margin, (float)r_exit,
-                       list_a, list_b, list_l, list_s, counts, flags, rec);
+                       list_a, list_b, list_l, list_s, counts, flags, rec,
+                       SurCal{cal_err, cal_n, cal_bad, (float)err_max});
     return hipGetLastError();
 }
 hipError_t launch_sur_decide(const int32_t* list_s, const uint32_t* count_s, int64_t cap, const float* y,

@@ -88,9 +88,9 @@ def _calibration(acc):
     L = acc._L
     L.sirius_hip_surrogate_calibration.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                                    C.POINTER(C.c_int)]
-    err, n, bad = np.zeros(512, np.float32), np.zeros(512, np.uint32), np.zeros(512, np.uint32)
+    err, n, bad = np.zeros(1024, np.float32), np.zeros(1024, np.uint32), np.zeros(1024, np.uint32)
     nb = C.c_int()
-    assert L.sirius_hip_surrogate_calibration(acc._h, err.ctypes.data, n.ctypes.data, bad.ctypes.data, 512,
+    assert L.sirius_hip_surrogate_calibration(acc._h, err.ctypes.data, n.ctypes.data, bad.ctypes.data, 1024,
                                               C.byref(nb)) == 0
     return err[:nb.value], n[:nb.value], bad[:nb.value]
 
@@ -154,4 +154,9 @@ def test_uncalibrated_network_answers_nothing(acc):
     on.use_surrogate = 1
     got, st = acc.trace_batch([cam], [m], on)
     assert st["rays_surrogate"] == 0 and not ((got.flags & abi.FLAG_SURROGATE) != 0).any()
-    assert st["rays_candidates"] > 0 and st["rays_gated"] >= 0
+    # no bin answers, so the gate passes no ray to the network: the frame is the FP32 path's own
+    assert st["rays_candidates"] == 0 and st["rays_gated"] == 0
+    on.use_surrogate = 0
+    base, _ = acc.trace_batch([cam], [m], on)
+    for k in ("cls", "theta", "phi", "flags", "steps", "intensity"):
+        assert np.array_equal(getattr(got, k), getattr(base, k)), k
