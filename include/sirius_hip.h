@@ -238,7 +238,34 @@ int sirius_hip_device_info(SiriusHipHandle h, char* name, int name_len, size_t* 
                            size_t* free_mem, int* cu_count, int* max_threads_per_block);
 
 /* ---- rendering: mirrors sirius_optix_launch / get_frame_buffer (RDOP006A.h:22-29) ------ */
-/* Traces one camera and shades the RGBA framebuffer (synchronous, like RDOP001A.cu:1044). */
+/* Framebuffer shading of sirius_hip_launch: linear RGB radiance, as RenderSession fills its display
+ * buffer before post-processing (SNRS001A.cpp:480-560 per pixel, shadeDiskHit :315-434):
+ *  - each pixel averages a g x g grid of rays, g = floor(sqrt(samples_per_pixel)), through the
+ *    points ((sx + 1/2) / g, (sy + 1/2) / g) of the pixel;
+ *  - thin-disk hit (emission SIRIUS_EMISSION_THIN_DISK): emitted temperature T_e(r) =
+ *    disk_temperature * (F(r) / F_max)^(1/4) with the Page-Thorne flux F (temperature_model 1,
+ *    Novikov-Thorne) or disk_temperature * (r_in / r)^(3/4) (0, Shakura-Sunyaev); with the ray's g
+ *    and t = T_e g / 30000 K, colour = blackbody RGB of clamp(T_e g, 1000 K, 1e5 K) (brightest channel
+ *    1; PHSP001A.h:181-205) * t^4 g^4 (ColorModes TrueColor, PHSC001A.h:352-366) * the channel's limb
+ *    darkening (1 + u cos i) / (1 + u) (PHSP001A.h:306-322; cos i = |k_z| / |k| at the hit) * g^4 *
+ *    disk_boost;
+ *  - escaped: the background image (or black), plus star light when a catalogue is installed;
+ *  - captured: black; ended on max_steps or failed: grey max_steps_level.
+ * The reference's photon-ring magnification and boosts (heuristics of its CPU tracer,
+ * GTRC001A.cpp:395-431, 528-546) are not applied. */
+typedef struct SiriusHipShading {
+    int32_t samples_per_pixel;  /* LaunchConfig::samplesPerPixel (default 1)                     */
+    int32_t temperature_model;  /* 0 Shakura-Sunyaev, 1 Novikov-Thorne (LaunchConfig default 1)  */
+    double disk_temperature;    /* K (LaunchConfig::diskTemperature, default 50000)              */
+    double disk_boost;          /* DISK_INTENSITY_BOOST (SNRS001A.cpp:35), default 5              */
+    double max_steps_level;     /* MAX_STEPS_BRIGHTNESS (SNRS001A.cpp:41), default 0.01           */
+} SiriusHipShading;
+void sirius_hip_default_shading(SiriusHipShading* s);
+/* Shading of later sirius_hip_launch calls (the defaults until called). */
+int sirius_hip_set_shading(SiriusHipHandle h, const SiriusHipShading* s);
+/* Traces one camera and shades the RGBA framebuffer (synchronous, like RDOP001A.cu:1044). With
+ * g x g samples per pixel the held results (sirius_hip_fetch_results) are those of the camera of
+ * g W x g H pixels whose rays the frame averages. */
 int sirius_hip_launch(SiriusHipHandle h, const SiriusHipCamera* cam, const SiriusHipMetric* metric,
                       const SiriusHipTraceOpts* opts);
 int sirius_hip_resize(SiriusHipHandle h, int width, int height);

@@ -115,9 +115,12 @@ public:
     // converted on the device: what SREP001A.cpp:252-276 does with EXRWriter / BufferWriter.
     bool writeImage(const std::string& path, int format, float exposure = 1.0f);
 
-    // LaunchConfig -> C-ABI descriptors (units of the hole's mass).
-    static void convertConfig(const LaunchConfig& cfg, SiriusHipCamera& cam, SiriusHipMetric& metric,
-                              SiriusHipTraceOpts& opts, int precision);
+    // LaunchConfig -> C-ABI descriptors (units of the hole's mass); the spacetime metricType names.
+    // Returns SIRIUS_HIP_OK, or an error code with the reason in *why.
+    static int convertConfig(const LaunchConfig& cfg, SiriusHipCamera& cam, SiriusHipMetric& metric,
+                             SiriusHipTraceOpts& opts, int precision, std::string* why = nullptr);
+    // LaunchConfig -> framebuffer shading (samplesPerPixel, temperatureModel, diskTemperature).
+    static void convertShading(const LaunchConfig& cfg, SiriusHipShading& sh);
 
 private:
     // LaunchConfig::enableStarfield -> the default catalogue on both celestial spheres

@@ -14,6 +14,8 @@
 //   AccretionDiskD (ISCO, Page-Thorne quadrature)  Sirius.Core/Disk/PHAD001A.h:88-314
 //   SymplecticIntegratorD             Sirius.Core/Symplectic/PHSI001A.h:136-455
 //   StarfieldGenerator                Sirius.Core/Environment/PHSF001A.h:126-272
+//   Spectral::blackbodyToRGB, applyLimbDarkening  Sirius.Core/Spectral/PHSP001A.h:181-205, 306-322
+//   ColorModes::applyColorMode (TrueColor)        Sirius.Core/Spectral/PHSC001A.h:343-366
 // What the harness supplies itself: GeodesicTracer::initializeLightray is private, so its
 // steps (GTRC001A.cpp:15-134, including the float casts) are repeated here; the step loop
 // retries rejected steps (GeodesicTracer::trace ends a ray on the first rejection,
@@ -25,6 +27,8 @@
 #include "PHAD001A.h"
 #include "PHSI001A.h"
 #include "Environment/PHSF001A.h"
+#include "PHSC001A.h"
+#include "PHSP001A.h"
 
 #include <cmath>
 #include <cstdint>
@@ -313,6 +317,37 @@ void ref_starfield_sample(float aperture_mm, float focus_pc, float brightness, i
     for (int j = 0; j < m; ++j)
         g.sampleStarfield(dirs[3 * j], dirs[3 * j + 1], dirs[3 * j + 2], 0.f, 0.f, 0.f, cat, rgb[3 * j], rgb[3 * j + 1],
                           rgb[3 * j + 2]);
+}
+
+// Spectral::blackbodyToRGB (PHSP001A.h:181-205) at n temperatures (K): rgb, 3 floats each.
+void ref_blackbody_rgb(const double* T, int n, float* rgb) {
+    for (int i = 0; i < n; ++i) {
+        Spectral::RGB c = Spectral::blackbodyToRGB(T[i]);
+        rgb[3 * i] = c.r;
+        rgb[3 * i + 1] = c.g;
+        rgb[3 * i + 2] = c.b;
+    }
+}
+
+// ColorModes::applyColorMode(TrueColor, T_emit, g, intensity) (PHSC001A.h:343-366), n cases.
+void ref_true_color(const float* T_emit, const float* g, const float* intensity, int n, float* rgb) {
+    for (int i = 0; i < n; ++i) {
+        Spectral::RGB c = ColorModes::applyColorMode(ColorModes::Mode::TrueColor, T_emit[i], g[i], intensity[i], nullptr);
+        rgb[3 * i] = c.r;
+        rgb[3 * i + 1] = c.g;
+        rgb[3 * i + 2] = c.b;
+    }
+}
+
+// Spectral::applyLimbDarkening (PHSP001A.h:306-322) of colours rgb_in at cosines cos_i, n cases.
+void ref_limb_darkening(const float* rgb_in, const double* cos_i, int n, float* rgb) {
+    for (int i = 0; i < n; ++i) {
+        Spectral::RGB c = Spectral::applyLimbDarkening(Spectral::RGB(rgb_in[3 * i], rgb_in[3 * i + 1], rgb_in[3 * i + 2]),
+                                                       cos_i[i]);
+        rgb[3 * i] = c.r;
+        rgb[3 * i + 1] = c.g;
+        rgb[3 * i + 2] = c.b;
+    }
 }
 
 }  // extern "C"

@@ -44,6 +44,9 @@ def lib():
         L.ref_starfield_generate.restype = i
         L.ref_starfield_generate.argtypes = [u32, f, f, f, u32, ip, i]
         L.ref_starfield_sample.argtypes = [f, f, f, i, ip, i, ip, i, ip]
+        L.ref_blackbody_rgb.argtypes = [ip, i, ip]
+        L.ref_true_color.argtypes = [ip, ip, ip, i, ip]
+        L.ref_limb_darkening.argtypes = [ip, ip, i, ip]
         _lib = L
     return _lib
 
@@ -226,4 +229,29 @@ def load_exr(data, h, w):
     wi, hi = C.c_int(), C.c_int()
     assert image_lib().ref_load_exr(data, len(data), C.byref(wi), C.byref(hi), out.ctypes.data, out.size) == 1
     assert (wi.value, hi.value) == (w, h)
+    return out
+
+
+def blackbody_rgb(T):
+    """Spectral::blackbodyToRGB (PHSP001A.h:181-205) at temperatures T (K): float32 [n, 3]."""
+    T = np.ascontiguousarray(T, np.float64)
+    out = np.zeros((len(T), 3), np.float32)
+    lib().ref_blackbody_rgb(T.ctypes.data, len(T), out.ctypes.data)
+    return out
+
+
+def true_color(T_emit, g, intensity):
+    """ColorModes::applyColorMode(TrueColor, ...) (PHSC001A.h:343-366): float32 [n, 3]."""
+    T_emit, g, intensity = (np.ascontiguousarray(v, np.float32) for v in (T_emit, g, intensity))
+    out = np.zeros((len(T_emit), 3), np.float32)
+    lib().ref_true_color(T_emit.ctypes.data, g.ctypes.data, intensity.ctypes.data, len(T_emit), out.ctypes.data)
+    return out
+
+
+def limb_darkening(rgb, cos_i):
+    """Spectral::applyLimbDarkening (PHSP001A.h:306-322): float32 [n, 3]."""
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    cos_i = np.ascontiguousarray(cos_i, np.float64)
+    out = np.zeros((len(cos_i), 3), np.float32)
+    lib().ref_limb_darkening(rgb.ctypes.data, cos_i.ctypes.data, len(cos_i), out.ctypes.data)
     return out

@@ -125,6 +125,12 @@ class RayBundle(C.Structure):
                 ("min_bundle_size", C.c_float), ("max_ellipse_ratio", C.c_float), ("reserved0", C.c_int32)]
 
 
+class Shading(C.Structure):
+    """SiriusHipShading (framebuffer shading of launch)."""
+    _fields_ = [("samples_per_pixel", C.c_int32), ("temperature_model", C.c_int32), ("disk_temperature", C.c_double),
+                ("disk_boost", C.c_double), ("max_steps_level", C.c_double)]
+
+
 class BundleOutput(C.Structure):
     _fields_ = [("cls", C.c_void_p), ("flags", C.c_void_p), ("jac", C.c_void_p), ("beam", C.c_void_p),
                 ("on_device", C.c_int32), ("reserved0", C.c_int32)]

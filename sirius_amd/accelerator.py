@@ -83,6 +83,9 @@ def lib():
     L.sirius_hip_default_ray_bundle.argtypes = [C.POINTER(abi.RayBundle)]
     L.sirius_hip_default_ray_bundle.restype = None
     L.sirius_hip_set_ray_bundle.argtypes = [H, C.POINTER(abi.RayBundle)]
+    L.sirius_hip_default_shading.argtypes = [C.POINTER(abi.Shading)]
+    L.sirius_hip_default_shading.restype = None
+    L.sirius_hip_set_shading.argtypes = [H, C.POINTER(abi.Shading)]
     L.sirius_hip_trace_bundle.argtypes = [H, C.POINTER(abi.Camera), i, C.POINTER(abi.Metric), i,
                                           C.POINTER(abi.TraceOpts), C.POINTER(abi.BundleOutput)]
     L.sirius_hip_default_image_opts.argtypes = [C.POINTER(abi.ImageOpts), i]
@@ -102,6 +105,12 @@ def ray_bundle_default():
     b = abi.RayBundle()
     lib().sirius_hip_default_ray_bundle(C.byref(b))
     return b
+
+
+def shading_default():
+    s = abi.Shading()
+    lib().sirius_hip_default_shading(C.byref(s))
+    return s
 
 
 def starfield_default():
@@ -221,6 +230,18 @@ class HipAccelerator:
                 setattr(b, k, v)
         self._check(self._L.sirius_hip_set_ray_bundle(self._h, C.byref(b)))
         return b
+
+    def set_shading(self, samples_per_pixel=None, temperature_model=None, disk_temperature=None, disk_boost=None,
+                    max_steps_level=None):
+        """Framebuffer shading of later launches (SiriusHipShading); None keeps the default."""
+        sh = shading_default()
+        for k, v in (("samples_per_pixel", samples_per_pixel), ("temperature_model", temperature_model),
+                     ("disk_temperature", disk_temperature), ("disk_boost", disk_boost),
+                     ("max_steps_level", max_steps_level)):
+            if v is not None:
+                setattr(sh, k, v)
+        self._check(self._L.sirius_hip_set_shading(self._h, C.byref(sh)))
+        return sh
 
     def trace_bundle(self, cams, metrics, opts, resident=False):
         """Every pixel's ray bundle: dict of cls, flags, jac (4, n) and beam (3, n); with resident

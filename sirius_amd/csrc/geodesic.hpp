@@ -84,6 +84,14 @@ SIRIUS_HD __forceinline__ bool sfinite(T x) {
     return y == y;
 }
 
+// Framebuffer shading of launch (the public SiriusHipShading's fields the shading kernel reads).
+struct ShadeParams {
+    int32_t temperature_model;  // 0 Shakura-Sunyaev, 1 Novikov-Thorne
+    double disk_temperature;    // K
+    double disk_boost;
+    double max_steps_level;
+};
+
 // Device result arrays (structure of arrays, one slot per ray; intensity is bin-major).
 struct DevOut {
     uint8_t* cls;

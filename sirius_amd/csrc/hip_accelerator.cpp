@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace Sirius::Acceleration::HIP {
@@ -49,22 +50,53 @@ bool HIPAccelerator::initialise(int width, int height) {
 
 bool HIPAccelerator::isInitialised() const { return m_handle && sirius_hip_is_initialized(m_handle); }
 
-void HIPAccelerator::convertConfig(const LaunchConfig& cfg, SiriusHipCamera& cam, SiriusHipMetric& metric,
-                                   SiriusHipTraceOpts& opts, int precision) {
-    // Lengths in units of M (G = c = M = 1); blackHoleSpin is a/M.
+int HIPAccelerator::convertConfig(const LaunchConfig& cfg, SiriusHipCamera& cam, SiriusHipMetric& metric,
+                                  SiriusHipTraceOpts& opts, int precision, std::string* why) {
+    // Lengths in units of M (G = c = M = 1); blackHoleSpin is a/M. The spacetime is the one
+    // metricType names (MetricType, RDOP003A.h:63-75: the OptiX backend selects its ray-generation
+    // program by it, RDOX001A.cpp:102); metricFamily must agree with it.
     double M = cfg.blackHoleMass > 0 ? cfg.blackHoleMass : 1.0;
+    int family;
+    double a = std::clamp((double)cfg.blackHoleSpin, -0.9999, 0.9999);
+    switch (cfg.metricType) {
+        case 1:  // Schwarzschild: Kerr-Schild with a = Q = 0
+        case 3:  // Reissner-Nordstrom: a = 0 (LaunchConfig carries no charge: Q = 0)
+            family = SIRIUS_METRIC_KERR_SCHILD;
+            a = 0.0;
+            break;
+        case 2:  // Kerr
+        case 6:  // Kerr-Schild, native
+            family = SIRIUS_METRIC_KERR_SCHILD;
+            break;
+        case 7:  // Ellis drainhole: the Morris-Thorne family
+            family = SIRIUS_METRIC_MORRIS_THORNE;
+            break;
+        case 8:  // Alcubierre: the warp-drive family
+            family = SIRIUS_METRIC_ALCUBIERRE;
+            break;
+        default:
+            if (why)
+                *why = "metricType " + std::to_string(cfg.metricType) +
+                       " (Minkowski 0, Godel 4, Taub-NUT 5, de Sitter 9, custom 99) is not supported by the HIP backend";
+            return SIRIUS_HIP_E_UNSUPPORTED;
+    }
+    if (cfg.metricFamily != family) {
+        if (why)
+            *why = "metricType " + std::to_string(cfg.metricType) + " needs metricFamily " + std::to_string(family) +
+                   ", not " + std::to_string(cfg.metricFamily);
+        return SIRIUS_HIP_E_INVALID;
+    }
     std::memset(&metric, 0, sizeof(metric));
-    metric.family = cfg.metricFamily;
-    if (cfg.metricFamily == SIRIUS_METRIC_MORRIS_THORNE) {
+    metric.family = family;
+    if (family == SIRIUS_METRIC_MORRIS_THORNE) {
         metric.throat_b0 = cfg.throatRadius / M;
-    } else if (cfg.metricFamily == SIRIUS_METRIC_ALCUBIERRE) {
+    } else if (family == SIRIUS_METRIC_ALCUBIERRE) {
         metric.warp_v = cfg.warpVelocity;
         metric.warp_R = cfg.bubbleRadius / M;
         metric.warp_sigma = cfg.bubbleSigma * M;
     } else {
-        metric.family = SIRIUS_METRIC_KERR_SCHILD;
         metric.M = 1.0;
-        metric.a = std::clamp((double)cfg.blackHoleSpin, -0.9999, 0.9999);
+        metric.a = a;
     }
     std::memset(&cam, 0, sizeof(cam));
     cam.r_obs = cfg.observerDistance / M;
@@ -79,6 +111,14 @@ void HIPAccelerator::convertConfig(const LaunchConfig& cfg, SiriusHipCamera& cam
     sirius_hip_default_opts(&opts, precision);
     opts.n_bins = 16;
     if (cfg.maxSteps > 0) opts.max_steps = cfg.maxSteps;
+    return SIRIUS_HIP_OK;
+}
+
+void HIPAccelerator::convertShading(const LaunchConfig& cfg, SiriusHipShading& sh) {
+    sirius_hip_default_shading(&sh);
+    sh.samples_per_pixel = std::clamp(cfg.samplesPerPixel, 1, 4096);
+    sh.temperature_model = cfg.temperatureModel == 0 ? 0 : 1;
+    sh.disk_temperature = std::max(0.0, (double)cfg.diskTemperature);
 }
 
 // The OptiX backend turns LaunchConfig::enableStarfield into its kernel's procedural star field
@@ -143,7 +183,13 @@ void HIPAccelerator::launch(const LaunchConfig& config) {
     SiriusHipCamera cam;
     SiriusHipMetric metric;
     SiriusHipTraceOpts opts;
-    convertConfig(config, cam, metric, opts, m_precision);
+    if (convertConfig(config, cam, metric, opts, m_precision, &m_error) != SIRIUS_HIP_OK) return;
+    SiriusHipShading shading;
+    convertShading(config, shading);
+    if (sirius_hip_set_shading(m_handle, &shading) != SIRIUS_HIP_OK) {
+        m_error = sirius_hip_last_error(m_handle);
+        return;
+    }
     // The FP32 path's launch shape is tuned on the first frame of each size and spacetime (shapes
     // never change results); if tuning fails, the default shape is kept.
     if (opts.precision == SIRIUS_PRECISION_FP32_ESCALATE && opts.integrator == SIRIUS_INTEGRATOR_DP45) {

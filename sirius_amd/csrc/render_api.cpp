@@ -18,8 +18,9 @@ int bundle_for_launch(Handle& H, Batch& B, int emission, int n_cam, BundleOut& b
 hipError_t launch_star_shade(const DevOut& o, int64_t n, const float* rec0, int n0, float b0, const float* rec1, int n1,
                              float b1, const float* bjac, const uint8_t* bcls, const uint8_t* bflags, float bsize,
                              float* fb, hipStream_t st);
-hipError_t launch_shade(const DevOut& o, int64_t n, int width, int height, int n_bins, const uint8_t* bg, int bgw,
-                        int bgh, float* fb, hipStream_t st);
+hipError_t launch_shade(const CamConst* cam, const DevOut& o, int64_t n, int emission, const ShadeParams& sp,
+                        const uint8_t* bg, int bgw, int bgh, float* fb, hipStream_t st);
+hipError_t launch_pixel_average(const float* sub, int w, int h, int gs, float* fb, hipStream_t st);
 hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float* tabs, int64_t n, float* out,
                                     hipStream_t st);
 int persistent_blocks_per_cu(bool fp64, int family, int emission, bool multi, int block);
@@ -135,6 +136,29 @@ using namespace sirius;
 
 extern "C" {
 
+void sirius_hip_default_shading(SiriusHipShading* s) {
+    if (!s) return;
+    s->samples_per_pixel = 1;
+    s->temperature_model = 1;
+    s->disk_temperature = 50000.0;
+    s->disk_boost = 5.0;
+    s->max_steps_level = 0.01;
+}
+
+int sirius_hip_set_shading(SiriusHipHandle hh, const SiriusHipShading* s) {
+    if (!hh) return SIRIUS_HIP_E_INVALID;
+    Handle* H = static_cast<Handle*>(hh);
+    if (!s) return H->fail(SIRIUS_HIP_E_INVALID, "shading");
+    if (s->samples_per_pixel < 1 || s->samples_per_pixel > 4096)
+        return H->fail(SIRIUS_HIP_E_INVALID, "samples_per_pixel must be in [1, 4096]");
+    if (s->temperature_model != 0 && s->temperature_model != 1)
+        return H->fail(SIRIUS_HIP_E_INVALID, "temperature_model is 0 (Shakura-Sunyaev) or 1 (Novikov-Thorne)");
+    if (!(s->disk_temperature >= 0) || !(s->disk_boost >= 0) || !(s->max_steps_level >= 0))
+        return H->fail(SIRIUS_HIP_E_INVALID, "disk_temperature, disk_boost and max_steps_level must be >= 0");
+    H->shading = *s;
+    return SIRIUS_HIP_OK;
+}
+
 int sirius_hip_launch(SiriusHipHandle hh, const SiriusHipCamera* cam, const SiriusHipMetric* metric,
                       const SiriusHipTraceOpts* opts) {
     if (!hh) return SIRIUS_HIP_E_INVALID;
@@ -144,8 +168,14 @@ int sirius_hip_launch(SiriusHipHandle hh, const SiriusHipCamera* cam, const Siri
     if (!cam || !metric || !opts) return H->fail(SIRIUS_HIP_E_INVALID, "camera, metric and opts are required");
     if (cam->width != H->fb_w || cam->height != H->fb_h)
         return H->fail(SIRIUS_HIP_E_INVALID, "camera size differs from the framebuffer (resize first)");
+    // g x g rays per pixel (shadePixel, SNRS001A.cpp:484-495): the camera of g W x g H pixels has
+    // its pixel (g px + sx, g py + sy) centred on (px + (sx + 1/2) / g, py + (sy + 1/2) / g) of this
+    // one, at the same field of view
+    const int gs = std::max(1, (int)std::floor(std::sqrt((double)H->shading.samples_per_pixel) + 1e-9));
     SiriusHipCamera c = *cam;
     c.metric_index = 0;
+    c.width *= gs;
+    c.height *= gs;
     Batch B;
     int rc = prepare_batch(*H, &c, 1, metric, 1, *opts, B);
     if (rc) return rc;
@@ -158,8 +188,16 @@ int sirius_hip_launch(SiriusHipHandle hh, const SiriusHipCamera* cam, const Siri
     H->last_bins = opts->n_bins;
     rc = run_trace(*H, B, *opts, dout, nullptr, nullptr, 0, nullptr, false);
     if (rc) return rc;
-    hipError_t e = launch_shade(dout, B.n, c.width, c.height, opts->n_bins, H->bg_dev, H->bg_w, H->bg_h, H->fb_dev,
-                                H->stream);
+    // radiance per ray (into the framebuffer itself at one ray per pixel), star light added per
+    // ray, then each pixel's average
+    float* sub = H->fb_dev;
+    if (gs > 1) {
+        sub = (float*)H->scratch("fb_sub", (size_t)B.n * 4 * sizeof(float), &rc);
+        if (rc) return rc;
+    }
+    const ShadeParams sp{H->shading.temperature_model, H->shading.disk_temperature, H->shading.disk_boost,
+                         H->shading.max_steps_level};
+    hipError_t e = launch_shade(B.cams_dev, dout, B.n, opts->emission, sp, H->bg_dev, H->bg_w, H->bg_h, sub, H->stream);
     if (e != hipSuccess) return H->hip_fail(e, "shade");
     // with a ray bundle, star light is averaged over each pixel's beam (the bundle pass traces
     // the frame's rays again in FP64 with their deviation)
@@ -171,8 +209,10 @@ int sirius_hip_launch(SiriusHipHandle hh, const SiriusHipCamera* cam, const Siri
     }
     e = launch_star_shade(dout, B.n, H->star_dev[0], H->star_n[0], H->star_brightness[0], H->star_dev[1],
                           H->star_n[1], H->star_brightness[1], bo.jac, bo.cls, bo.flags, H->bundle.initial_size,
-                          H->fb_dev, H->stream);
+                          sub, H->stream);
     if (e != hipSuccess) return H->hip_fail(e, "star shade");
+    if (gs > 1 && (e = launch_pixel_average(sub, cam->width, cam->height, gs, H->fb_dev, H->stream)) != hipSuccess)
+        return H->hip_fail(e, "pixel average");
     e = hipMemcpyAsync(H->fb_host, H->fb_dev, H->fb_n * sizeof(float), hipMemcpyDeviceToHost,
                        H->stream);
     if (e != hipSuccess) return H->hip_fail(e, "framebuffer copy");

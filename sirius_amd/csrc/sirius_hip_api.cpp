@@ -43,8 +43,6 @@ hipError_t launch_critical_distance(const CamConst* cams, int n_cam, const float
 hipError_t launch_gate(const CamConst* cams, int n_cam, const float* tabs, int64_t n, double band, double crit_band,
                        double long_band, int32_t* list_a, int32_t* list_b, int32_t* list_l, uint32_t* counts,
                        uint8_t* flags, hipStream_t st);
-hipError_t launch_shade(const DevOut& o, int64_t n, int width, int height, int n_bins, const uint8_t* bg, int bgw,
-                        int bgh, float* fb, hipStream_t st);
 hipError_t launch_sur_audit(const CamConst* cams, const TraceParams& tp, const DevOut& oa, const DevOut& o,
                             const int32_t* list, const uint32_t* count, int64_t cap, unsigned int* acc,
                             hipStream_t st);

@@ -91,6 +91,8 @@ struct Handle {
     float star_brightness[2] = {1.f, 1.f};
     // ray bundle of launch (sirius_hip_set_ray_bundle)
     SiriusHipRayBundle bundle = {0, 0, 0.28867513f, 1e-6f, 100.0f, 0};
+    // framebuffer shading of launch (sirius_hip_set_shading)
+    SiriusHipShading shading = {1, 1, 50000.0, 5.0, 0.01};
     // image writers: the last encoded file and the pinned host staging of converted payloads
     std::vector<unsigned char> image;
     unsigned char* pinned = nullptr;

@@ -189,9 +189,11 @@ hipError_t launch_gate(const CamConst* cams, int n_cam, const float* tabs, int64
     return hipGetLastError();
 }
 
-// RGBA shading of one camera's results: disk intensity bins split into red/green/blue thirds
-// by frequency, escaped rays sample the equirectangular background (black without one),
-// captured rays are black; x / (1 + x) tone curve.
+// ------------------------------------------------------------------------------------------
+// Framebuffer shading of launch: linear RGB radiance per ray (RenderSession::shadeDiskHit,
+// SNRS001A.cpp:315-434, for the thin disk's first crossing), then the average of each pixel's
+// g x g rays (shadePixel, SNRS001A.cpp:480-560).
+// ------------------------------------------------------------------------------------------
 namespace {
 // c00 w00 + c10 w10 + c01 w01 + c11 w11 on texel values / 255, each product and sum rounded on its own
 __device__ __forceinline__ float bilinear(uint8_t a, uint8_t b, uint8_t c, uint8_t d, float w00, float w10, float w01,
@@ -202,21 +204,100 @@ __device__ __forceinline__ float bilinear(uint8_t a, uint8_t b, uint8_t c, uint8
     const float fa = (float)(a / 255.0), fb = (float)(b / 255.0), fc = (float)(c / 255.0), fd = (float)(d / 255.0);
     return fa * w00 + fb * w10 + fc * w01 + fd * w11;
 }
+
+// CIE 1931 colour-matching functions by the piecewise-Gaussian fit of Wyman, Sloan and Shirley
+// (JCGT 2(2), 2013): each lobe a exp(-((l - mu) s)^2 / 2), s taking one value below mu and another
+// above it.
+__device__ __forceinline__ double cie_lobe(double nm, double a, double mu, double s_lo, double s_hi) {
+    const double t = (nm - mu) * (nm < mu ? s_lo : s_hi);
+    return a * exp(-0.5 * t * t);
+}
+
+// Blackbody colour of temperature T (K), brightest channel 1 (PHSP001A.h:181-205): Planck's
+// spectral radiance B(l, T) = c1 / (l^5 (e^{c2 / (l T)} - 1)) (c1 = 2 h c^2, c2 = h c / k, SI
+// values; zero past c2 / (l T) = 700) weighs the matching functions at the 32 midpoints of
+// 380-780 nm; the XYZ sums go through the sRGB (D65) matrix. The spectrum is FP64 and the colour
+// sums float, each operation rounded on its own, as PHSP001A.h:72-80, 144-166 round them.
+__device__ void blackbody_rgb(double T, float rgb[3]) {
+#pragma clang fp contract(off)
+    const double h = 6.62607015e-34, c = 2.99792458e8, k = 1.380649e-23;
+    const double c1 = 2.0 * h * c * c, c2 = h * c / k;
+    const double l0 = 380e-9, dl = (780e-9 - 380e-9) / 32;
+    float X = 0.f, Y = 0.f, Z = 0.f;
+    for (int i = 0; i < 32; ++i) {
+        const double l = l0 + (i + 0.5) * dl, nm = l * 1e9;
+        const double x = c2 / (l * T);
+        const double B = (T <= 0 || x > 700) ? 0.0 : c1 / (pow(l, 5.0) * (exp(x) - 1.0));
+        const float w = (float)(B * dl);
+        const float sx = (float)(cie_lobe(nm, 0.362, 442.0, 0.0624, 0.0374) + cie_lobe(nm, 1.056, 599.8, 0.0264, 0.0323) -
+                                 cie_lobe(nm, 0.065, 501.1, 0.0490, 0.0382));
+        const float sy = (float)(cie_lobe(nm, 0.821, 568.8, 0.0213, 0.0247) + cie_lobe(nm, 0.286, 530.9, 0.0613, 0.0322));
+        const float sz = (float)(cie_lobe(nm, 1.217, 437.0, 0.0845, 0.0278) + cie_lobe(nm, 0.681, 459.0, 0.0385, 0.0725));
+        X = X + sx * w;
+        Y = Y + sy * w;
+        Z = Z + sz * w;
+    }
+    const float r = 3.2404542f * X - 1.5371385f * Y - 0.4985314f * Z;
+    const float g = -0.9692660f * X + 1.8760108f * Y + 0.0415560f * Z;
+    const float b = 0.0556434f * X - 0.2040259f * Y + 1.0572252f * Z;
+    const float m = fmaxf(fmaxf(fmaxf(r, g), b), 0.001f);
+    rgb[0] = __fdiv_rn(r, m);
+    rgb[1] = __fdiv_rn(g, m);
+    rgb[2] = __fdiv_rn(b, m);
+}
+
+// Linear limb-darkening law I(i) / I(0) = (1 + u cos i) / (1 + u) with u(l) falling linearly
+// from 0.9 at 400 nm to 0.4 at 700 nm, at 650, 550 and 450 nm for red, green and blue
+// (PHSP001A.h:280-322).
+__device__ __forceinline__ float limb_factor(double lambda_nm, double cos_i) {
+    const double t = fmin(fmax((lambda_nm - 400.0) / 300.0, 0.0), 1.0);
+    const double u = 0.9 + t * (0.4 - 0.9);
+    return (float)((1.0 + u * cos_i) / (1.0 + u));
+}
 }  // namespace
 
-__global__ __launch_bounds__(256) void k_shade(DevOut o, int64_t n, int width, int height, int n_bins,
-                                               const uint8_t* __restrict__ bg, int bgw, int bgh,
+// One ray's radiance into fb[4 id..] (RGBA, alpha 1).
+__global__ __launch_bounds__(256) void k_shade(const CamConst* __restrict__ cam, DevOut o, int64_t n, int emission,
+                                               ShadeParams sp, const uint8_t* __restrict__ bg, int bgw, int bgh,
                                                float* __restrict__ fb) {
     int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= n) return;
     float rgb[3] = {0.f, 0.f, 0.f};
-    uint8_t cls = o.cls[id];
-    if (cls == SIRIUS_RAY_DISK && n_bins > 0) {
-        for (int k = 0; k < n_bins; ++k) {
-            int ch = (k * 3) / n_bins;
-            rgb[ch] += o.intensity[(int64_t)k * n + id];
+    const uint8_t cls = o.cls[id];
+    if (cls == SIRIUS_RAY_DISK && emission == SIRIUS_EMISSION_THIN_DISK && cam->family == 0) {
+#pragma clang fp contract(off)
+        const CamConst& c = *cam;
+        const double r = o.disk_r[id];
+        // emitted temperature (K): Novikov-Thorne from the Page-Thorne flux, or Shakura-Sunyaev
+        double te = 0.0;
+        if (sp.temperature_model == 1) {
+            const double f = page_thorne_f(c, r) / r;
+            te = (c.fmax > 0 && f > 0) ? sp.disk_temperature * sqrt(sqrt(f / c.fmax)) : 0.0;
+        } else {
+            te = r > 0 ? sp.disk_temperature * pow(c.r_in / r, 0.75) : 0.0;
         }
-        for (int ch = 0; ch < 3; ++ch) rgb[ch] = rgb[ch] / (1.f + rgb[ch]);
+        const float g = o.redshift[id];
+        if (te > 0 && g > 0.f) {
+            // direction of the photon at the hit: cos i = |k_z| / |k| (the disk's normal is z)
+            const double* st = o.state + 8 * id;
+            const double xd[4] = {st[0], st[1], st[2], st[3]}, pd[4] = {st[4], st[5], st[6], st[7]};
+            const Spacetime<double> sd = make_spacetime<double>(c);
+            double kd[4];
+            raise_index<double, 0>(sd, xd, pd, kd);
+            const double kn = sqrt(kd[1] * kd[1] + kd[2] * kd[2] + kd[3] * kd[3]);
+            const double cos_i = kn > 0 ? fabs(kd[3]) / kn : 0.0;
+            // TrueColor (PHSC001A.h:352-366): T_obs = T_e g in units of 30000 K, colour of
+            // clamp(T_obs, 1000 K, 1e5 K), times T_obs^4 g^4
+            const float t_obs = (float)(te / 30000.0) * g;
+            const float tk = fminf(fmaxf(t_obs * 30000.0f, 1000.0f), 100000.0f);
+            blackbody_rgb((double)tk, rgb);
+            const float inten = t_obs * t_obs * t_obs * t_obs;
+            const float g4 = g * g * g * g;
+            const float s1 = inten * g4;
+            const float limb[3] = {limb_factor(650.0, cos_i), limb_factor(550.0, cos_i), limb_factor(450.0, cos_i)};
+            if (cos_i <= 0.0) rgb[0] = rgb[1] = rgb[2] = 0.f;
+            for (int ch = 0; ch < 3; ++ch) rgb[ch] = ((rgb[ch] * s1) * limb[ch]) * g4 * (float)sp.disk_boost;
+        }
     } else if (cls == SIRIUS_RAY_ESCAPED && bg) {
         // RenderSession::sampleStarfield (SNRS001A.cpp:631-703): phi in [0, 2 pi), u = phi / 2 pi,
         // v = theta / pi, bilinear between the four texels around (u (W - 1), v (H - 1)); the
@@ -236,6 +317,8 @@ __global__ __launch_bounds__(256) void k_shade(DevOut o, int64_t n, int width, i
         const uint8_t* c01 = bg + 4 * ((int64_t)y1 * bgw + x0);
         const uint8_t* c11 = bg + 4 * ((int64_t)y1 * bgw + x1);
         for (int ch = 0; ch < 3; ++ch) rgb[ch] = bilinear(c00[ch], c10[ch], c01[ch], c11[ch], w00, w10, w01, w11);
+    } else if (cls == SIRIUS_RAY_MAX_STEPS || cls == SIRIUS_RAY_FAILED) {
+        rgb[0] = rgb[1] = rgb[2] = (float)sp.max_steps_level;
     }
     fb[4 * id + 0] = rgb[0];
     fb[4 * id + 1] = rgb[1];
@@ -243,10 +326,34 @@ __global__ __launch_bounds__(256) void k_shade(DevOut o, int64_t n, int width, i
     fb[4 * id + 3] = 1.f;
 }
 
-hipError_t launch_shade(const DevOut& o, int64_t n, int width, int height, int n_bins, const uint8_t* bg, int bgw,
-                        int bgh, float* fb, hipStream_t st) {
-    hipLaunchKernelGGL(k_shade, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, o, n, width, height, n_bins, bg,
-                       bgw, bgh, fb);
+// Average of each pixel's g x g rays (sub-image of gW x gH pixels, row-major) into the W x H frame.
+__global__ __launch_bounds__(256) void k_pixel_average(const float* __restrict__ sub, int w, int h, int gs,
+                                                       float* __restrict__ fb) {
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (int64_t)w * h) return;
+    const int px = (int)(id % w), py = (int)(id / w);
+    const int64_t sw = (int64_t)w * gs;
+    float acc[3] = {0.f, 0.f, 0.f};
+    for (int sy = 0; sy < gs; ++sy)
+        for (int sx = 0; sx < gs; ++sx) {
+            const int64_t s = ((int64_t)py * gs + sy) * sw + (int64_t)px * gs + sx;
+            for (int ch = 0; ch < 3; ++ch) acc[ch] += sub[4 * s + ch];
+        }
+    const float inv = 1.f / (float)(gs * gs);
+    for (int ch = 0; ch < 3; ++ch) fb[4 * id + ch] = acc[ch] * inv;
+    fb[4 * id + 3] = 1.f;
+}
+
+hipError_t launch_shade(const CamConst* cam, const DevOut& o, int64_t n, int emission, const ShadeParams& sp,
+                        const uint8_t* bg, int bgw, int bgh, float* fb, hipStream_t st) {
+    hipLaunchKernelGGL(k_shade, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cam, o, n, emission, sp, bg, bgw,
+                       bgh, fb);
+    return hipGetLastError();
+}
+
+hipError_t launch_pixel_average(const float* sub, int w, int h, int gs, float* fb, hipStream_t st) {
+    const int64_t n = (int64_t)w * h;
+    hipLaunchKernelGGL(k_pixel_average, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sub, w, h, gs, fb);
     return hipGetLastError();
 }
 

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+
 #include "sirius_hip_accelerator.hpp"
 
 using Sirius::Acceleration::LaunchConfig;
@@ -51,6 +52,43 @@ int main(int argc, char** argv) {
     }
     if (!(mx > 0) || !finite) return fail("disk frame lit and finite", acc);
     std::printf("ok launch: max %.4g\n", mx);
+
+    // metricType names the spacetime (RDOP003A.h:63-75): Schwarzschild ignores the spin
+    cfg.samplesPerPixel = 1;
+    cfg.metricType = 1;
+    acc.launch(cfg);
+    std::vector<float> schw(acc.getFrameBuffer(), acc.getFrameBuffer() + (size_t)w * h * 4);
+    cfg.metricType = 2;
+    cfg.blackHoleSpin = 0.0f;
+    acc.launch(cfg);
+    if (std::vector<float>(acc.getFrameBuffer(), acc.getFrameBuffer() + (size_t)w * h * 4) != schw)
+        return fail("metricType 1 is Kerr with a = 0", acc);
+    cfg.blackHoleSpin = 0.9f;
+    acc.launch(cfg);
+    if (std::vector<float>(acc.getFrameBuffer(), acc.getFrameBuffer() + (size_t)w * h * 4) == schw)
+        return fail("metricType 2 keeps the spin", acc);
+    cfg.metricType = 4;  // Goedel: not a spacetime of this backend
+    acc.launch(cfg);
+    if (acc.getLastError().find("metricType 4") == std::string::npos) return fail("metricType 4 refused", acc);
+    cfg.metricType = 7;  // Ellis drainhole with a Kerr-Schild family: inconsistent
+    acc.launch(cfg);
+    if (acc.getLastError().find("metricFamily") == std::string::npos) return fail("metricType 7 / family 0", acc);
+    cfg.metricType = 2;
+    std::printf("ok metricType\n");
+
+    // samplesPerPixel: a 2 x 2 grid per pixel averages four rays; the frame changes, its mean
+    // brightness barely
+    acc.launch(cfg);
+    std::vector<float> one(acc.getFrameBuffer(), acc.getFrameBuffer() + (size_t)w * h * 4);
+    cfg.samplesPerPixel = 4;
+    acc.launch(cfg);
+    std::vector<float> four(acc.getFrameBuffer(), acc.getFrameBuffer() + (size_t)w * h * 4);
+    double s1 = 0, s4 = 0;
+    for (size_t i = 0; i < one.size(); ++i)
+        if (i % 4 != 3) { s1 += one[i]; s4 += four[i]; }
+    if (one == four || !(std::fabs(s4 - s1) < 0.2 * s1)) return fail("samplesPerPixel", acc);
+    std::printf("ok samplesPerPixel: mean %.4g at 1, %.4g at 4\n", s1 / (w * h), s4 / (w * h));
+    cfg.samplesPerPixel = 1;
 
     // a wide view of the sky: the default catalogue's stars, averaged over each pixel's beam
     cfg.cameraFOV = 100.0f;

@@ -3,7 +3,8 @@
 place) and from the FP64 oracle (the build's "pure FP64" columns).
 
 Run in the build container (it needs the reference tree): python tests/golden/make_golden.py
-(`python tests/golden/make_golden.py starfield` writes only the star-field fixture).
+(`python tests/golden/make_golden.py starfield`, `... images` or `... shading` writes only that
+fixture).
 Every fixture records its seed. The reference's GeodesicTracer::trace cannot be used as-is
 (it ends a ray on its first rejected step, GTRC001A.cpp:269-277), so the trace fixtures use the
 reference's Geodesic::integrateStepRK45 inside the harness loop of oracle/ref_harness.cpp.
@@ -180,7 +181,26 @@ def images_fixture():
     np.savez_compressed(os.path.join(HERE, "ref_images.npz"), **out)
 
 
+def shading_fixture(seed=0x5112100C):
+    """The reference's disk-colour functions: blackbodyToRGB over 1000 K - 1e5 K,
+    applyColorMode(TrueColor) on random (T_emit, g, intensity) and applyLimbDarkening on random
+    colours and cosines (endpoints included)."""
+    rng = np.random.default_rng(seed)
+    T = np.concatenate([np.geomspace(1000.0, 1e5, 400), rng.uniform(1000.0, 1e5, 200), [5778.0, 6500.0]])
+    te = rng.uniform(0.02, 4.0, 300).astype(np.float32)
+    g = rng.uniform(0.1, 2.0, 300).astype(np.float32)
+    inten = rng.uniform(0.0, 50.0, 300).astype(np.float32)
+    rgb = rng.uniform(0.0, 2.0, (300, 3)).astype(np.float32)
+    cos_i = np.concatenate([rng.uniform(-0.2, 1.0, 296), [0.0, 1.0, -1e-3, 0.5]])
+    np.savez_compressed(os.path.join(HERE, "ref_shading.npz"), seed=seed, T=T, bb_rgb=RF.blackbody_rgb(T),
+                        tc_T=te, tc_g=g, tc_I=inten, tc_rgb=RF.true_color(te, g, inten), limb_in=rgb, limb_cos=cos_i,
+                        limb_rgb=RF.limb_darkening(rgb, cos_i))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "shading":
+        shading_fixture()
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "starfield":
         starfield_fixture()
         return
@@ -201,6 +221,7 @@ def main():
     trace_fixture("ref_trace_c2.npz", 1.0, 0.9, 50.0, 75.0, 20.0, 1920, 1080, (0.0, 30.0), 16, pix, seed=seed)
     starfield_fixture()
     images_fixture()
+    shading_fixture()
 
 
 if __name__ == "__main__":
