@@ -70,7 +70,7 @@ struct MlpFusedArgs {
     const int32_t* idx;    // optional gather list
     int64_t x_stride;      // floats per input record (5 for packed inputs)
     const uint16_t* w0;    // [256][16] BF16, input columns padded to 16
-    const uint16_t* w[3];  // layers 1..3, [256][256] BF16
+    const uint16_t* w[3];  // layers 1..3, [256][256] BF16 in MFMA fragment order (k_pack_fused)
     const float* b[4];     // biases of layers 0..3, [256]
     const uint16_t* w4h;   // head weights [32][256], BF16 high parts (rows 6..31 zero)
     const uint16_t* w4l;   // and low parts
@@ -95,8 +95,9 @@ hipError_t launch_head_bwd(const float* gy64, const float* W4, const float* z3, 
                            uint16_t* gzb, hipStream_t st);
 hipError_t launch_head_wgrad(const float* gy64, const float* z3, int64_t n_pad, float* part, int nsplit, float* dW4,
                              hipStream_t st);
-hipError_t launch_pack_fused(const float* params, uint16_t* w0, uint16_t* w4h, uint16_t* w4l, size_t off_w0,
-                             size_t off_w4, hipStream_t st);
+// off_w: parameter offsets of W_0 .. W_4 (mlp_param_offset_w)
+hipError_t launch_pack_fused(const float* params, uint16_t* w0, uint16_t* wh, uint16_t* w4h, uint16_t* w4l,
+                             const size_t off_w[5], hipStream_t st);
 
 // Device state of the network of one handle.
 struct MlpState {
@@ -107,7 +108,8 @@ struct MlpState {
     int64_t step = 0;
     uint16_t* w[MLP_NL] = {};  // padded BF16 weights, W_l[NP][KP]
     float* b[MLP_NL] = {};     // padded biases [NP]
-    uint16_t* fw0 = nullptr;   // fused inference: W0 [256][16], head W_hi / W_lo [32][256]
+    uint16_t* fw0 = nullptr;   // fused inference: W0 [256][16], hidden layers 1..3 (3 x 256 x 256,
+    uint16_t* fwh = nullptr;   // fragment order), head W_hi / W_lo [32][256]
     uint16_t* fw4h = nullptr;
     uint16_t* fw4l = nullptr;
     // calibration of the error gate (SUR_BINS each): held-out rays the network would answer,

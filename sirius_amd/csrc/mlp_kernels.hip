@@ -120,8 +120,9 @@ hipError_t launch_gemm(int epi, const GemmArgs& g, int nsplit, hipStream_t st) {
 // v_mfma_f32_32x32x16_bf16 (A 32 x 16 and B 16 x 32 BF16 operands, 8 values per lane: lane
 // l holds row / column l % 32 and k = 8 (l / 32) .. +7; C 32 x 32 FP32, 16 values per lane:
 // column l % 32, rows 8 (r / 4) + 4 (l / 32) + r % 4).
-//  - Each layer is computed transposed, H'^T = W H^T: the weights are the A operand (W[j][k], 16
-//    contiguous bytes per lane, streamed from L2 one k-step ahead of the MFMAs) and the tile's
+//  - Each layer is computed transposed, H'^T = W H^T: the weights are the A operand (16 bytes per
+//    lane, stored in fragment order so that a wave's load for one k-step is one contiguous
+//    kilobyte, streamed from L2 one k-step ahead of the MFMAs) and the tile's
 //    activations the B operand, read from LDS ([128][264] BF16, row = sample; rows padded by 8
 //    values so that the 16-byte reads of eight consecutive lanes cover every bank once).
 //  - A lane's accumulators then hold, for one sample, runs of four consecutive neurons: the bias
@@ -198,15 +199,19 @@ __global__ __launch_bounds__(256, 2) void k_mlp_fused(MlpFusedArgs a) {
             if (l > 0) {
                 const uint16_t* Wl = a.w[l - 1];
                 init_bias(l);
+                // this lane's fragment of k-step ks, row tile mt: Wp + 1024 ks + 512 mt (one running
+                // pointer; the row-tile offset is an immediate)
+                const uint16_t* Wp = Wl + wave * 16384 + lane * 8;
                 v8bf w[2], wn[2];
 #pragma unroll
-                for (int mt = 0; mt < 2; ++mt) w[mt] = ld8(Wl + (j0 + mt * 32 + li) * MLP_H + 8 * lh);
+                for (int mt = 0; mt < 2; ++mt) w[mt] = ld8(Wp + 512 * mt);
 #pragma unroll
                 for (int ks = 0; ks < MLP_H / 16; ++ks) {
                     const int k0 = ks * 16;
                     if (ks + 1 < MLP_H / 16) {
+                        Wp += 1024;
 #pragma unroll
-                        for (int mt = 0; mt < 2; ++mt) wn[mt] = ld8(Wl + (j0 + mt * 32 + li) * MLP_H + k0 + 16 + 8 * lh);
+                        for (int mt = 0; mt < 2; ++mt) wn[mt] = ld8(Wp + 512 * mt);
                     }
 #pragma unroll
                     for (int nt = 0; nt < 4; ++nt) {
@@ -247,7 +252,11 @@ __global__ __launch_bounds__(256, 2) void k_mlp_fused(MlpFusedArgs a) {
                 }
             __syncthreads();
         }
-        // output head, samples 32 w .. 32 w + 31 of the tile: W_hi hi + W_lo hi, then W_hi lo
+        // output head, samples 32 w .. 32 w + 31 of the tile: W_hi hi + W_lo hi, then W_hi lo. Rows
+        // 6..31 of the head's weights are zero: the lanes holding them all read row 6, so a head
+        // load fetches the six live rows and one more line.
+        const uint16_t* Hh = a.w4h + (li < SIRIUS_MLP_OUT ? li : SIRIUS_MLP_OUT) * MLP_H + 8 * lh;
+        const uint16_t* Hl = a.w4l + (li < SIRIUS_MLP_OUT ? li : SIRIUS_MLP_OUT) * MLP_H + 8 * lh;
         v16f y;
 #pragma unroll
         for (int r = 0; r < 16; ++r) y[r] = 0.f;
@@ -255,8 +264,8 @@ __global__ __launch_bounds__(256, 2) void k_mlp_fused(MlpFusedArgs a) {
         for (int ks = 0; ks < MLP_H / 16; ++ks) {
             const int k0 = ks * 16;
             const v8bf hv = ld8(&act[(wave * 32 + li) * MF_PITCH + k0 + 8 * lh]);
-            y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(a.w4h + li * MLP_H + k0 + 8 * lh), hv, y, 0, 0, 0);
-            y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(a.w4l + li * MLP_H + k0 + 8 * lh), hv, y, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(Hh + k0), hv, y, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(Hl + k0), hv, y, 0, 0, 0);
         }
         __syncthreads();  // every wave has read the high parts
 #pragma unroll
@@ -275,7 +284,7 @@ __global__ __launch_bounds__(256, 2) void k_mlp_fused(MlpFusedArgs a) {
         for (int ks = 0; ks < MLP_H / 16; ++ks) {
             const int k0 = ks * 16;
             const v8bf hv = ld8(&act[(wave * 32 + li) * MF_PITCH + k0 + 8 * lh]);
-            y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(a.w4h + li * MLP_H + k0 + 8 * lh), hv, y, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ld8(Hh + k0), hv, y, 0, 0, 0);
         }
         // y^T: this lane's sample is 32 w + li, its outputs are rows 4 lh + r (r < 4) of the first
         // 8-row group
@@ -389,12 +398,25 @@ hipError_t launch_head_wgrad(const float* gy64, const float* z3, int64_t n_pad, 
 
 // Operands of the fused kernel from the FP32 masters: W0 padded to 16 input columns, W1..W3 as
 // they are, the head's weights split into BF16 high and low parts (rows 6..31 zero).
-__global__ void k_pack_fused(const float* __restrict__ P, uint16_t* __restrict__ w0, uint16_t* __restrict__ w4h,
-                             uint16_t* __restrict__ w4l, size_t off_w0, size_t off_w4) {
+// Fused-inference operands from the FP32 masters: W0 [256][16] (input columns padded), and the
+// hidden layers' and the head's weights in MFMA fragment order: the 16-byte A operand of lane l
+// for k-step ks, row tile mt of wave w sits at (((w 16 + ks) 2 + mt) 64 + l) 8 (hidden layer:
+// row 64 w + 32 mt + l % 32, columns 16 ks + 8 (l / 32) .. +7), so each wave's weight load is one
+// contiguous kilobyte; the head's high and low BF16 parts row-major [32][256] (rows 6..31 zero).
+__global__ void k_pack_fused(const float* __restrict__ P, uint16_t* __restrict__ w0, uint16_t* __restrict__ wh,
+                             uint16_t* __restrict__ w4h, uint16_t* __restrict__ w4l, size_t off_w0, size_t off_w1,
+                             size_t off_w2, size_t off_w3, size_t off_w4) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e < MLP_H * 16) {
         const int j = e / 16, k = e % 16;
         w0[e] = k < SIRIUS_MLP_IN ? f2bf(P[off_w0 + (size_t)j * SIRIUS_MLP_IN + k]) : (uint16_t)0;
+    }
+    if (e < 3 * MLP_H * MLP_H) {
+        const int l = e / (MLP_H * MLP_H), r = e % (MLP_H * MLP_H);
+        const int i = r & 7, lane = (r >> 3) & 63, mt = (r >> 9) & 1, ks = (r >> 10) & 15, w = r >> 14;
+        const int j = 64 * w + 32 * mt + (lane & 31), k = 16 * ks + 8 * (lane >> 5) + i;
+        const size_t off = l == 0 ? off_w1 : (l == 1 ? off_w2 : off_w3);
+        wh[e] = f2bf(P[off + (size_t)j * MLP_H + k]);
     }
     if (e < 32 * MLP_H) {
         const int j = e / MLP_H, k = e % MLP_H;
@@ -404,10 +426,10 @@ __global__ void k_pack_fused(const float* __restrict__ P, uint16_t* __restrict__
         w4l[e] = f2bf(w - __uint_as_float((uint32_t)hi << 16));
     }
 }
-hipError_t launch_pack_fused(const float* params, uint16_t* w0, uint16_t* w4h, uint16_t* w4l, size_t off_w0,
-                             size_t off_w4, hipStream_t st) {
-    hipLaunchKernelGGL(k_pack_fused, dim3((32 * MLP_H + 255) / 256), dim3(256), 0, st, params, w0, w4h, w4l, off_w0,
-                       off_w4);
+hipError_t launch_pack_fused(const float* params, uint16_t* w0, uint16_t* wh, uint16_t* w4h, uint16_t* w4l,
+                             const size_t off_w[5], hipStream_t st) {
+    hipLaunchKernelGGL(k_pack_fused, dim3((3 * MLP_H * MLP_H + 255) / 256), dim3(256), 0, st, params, w0, wh, w4h, w4l,
+                       off_w[0], off_w[1], off_w[2], off_w[3], off_w[4]);
     return hipGetLastError();
 }
 

@@ -30,7 +30,8 @@ void mlp_release(Handle& H) {
         if (S.w[l]) (void)hipFree(S.w[l]);
         if (S.b[l]) (void)hipFree(S.b[l]);
     }
-    for (void* p : {(void*)S.fw0, (void*)S.fw4h, (void*)S.fw4l, (void*)S.cal_err, (void*)S.cal_n, (void*)S.cal_bad})
+    for (void* p : {(void*)S.fw0, (void*)S.fwh, (void*)S.fw4h, (void*)S.fw4l, (void*)S.cal_err, (void*)S.cal_n,
+                    (void*)S.cal_bad})
         if (p) (void)hipFree(p);
     delete H.mlp;
     H.mlp = nullptr;
@@ -52,6 +53,7 @@ static int mlp_alloc(Handle& H) {
             return H.hip_fail(e, "mlp alloc");
     }
     if ((e = hipMalloc((void**)&S.fw0, (size_t)MLP_H * 16 * 2)) != hipSuccess ||
+        (e = hipMalloc((void**)&S.fwh, (size_t)3 * MLP_H * MLP_H * 2)) != hipSuccess ||
         (e = hipMalloc((void**)&S.fw4h, (size_t)32 * MLP_H * 2)) != hipSuccess ||
         (e = hipMalloc((void**)&S.fw4l, (size_t)32 * MLP_H * 2)) != hipSuccess ||
         (e = hipMalloc((void**)&S.cal_err, SUR_BINS * 4)) != hipSuccess ||
@@ -77,7 +79,9 @@ static hipError_t mlp_clear_calibration(MlpState& S, hipStream_t st) {
 static hipError_t mlp_pack(MlpState& S, hipStream_t st) {
     hipError_t e = launch_pack(S.params, S.w, S.b, st);
     if (e != hipSuccess) return e;
-    return launch_pack_fused(S.params, S.fw0, S.fw4h, S.fw4l, mlp_param_offset_w(0), mlp_param_offset_w(4), st);
+    const size_t off_w[5] = {mlp_param_offset_w(0), mlp_param_offset_w(1), mlp_param_offset_w(2),
+                             mlp_param_offset_w(3), mlp_param_offset_w(4)};
+    return launch_pack_fused(S.params, S.fw0, S.fwh, S.fw4h, S.fw4l, off_w, st);
 }
 
 static int mlp_reset_adam(Handle& H) {
@@ -243,7 +247,7 @@ MlpFusedArgs mlp_fused_args(const MlpState& S, const float* x, float* y, int64_t
     a.idx = nullptr;
     a.x_stride = SIRIUS_MLP_IN;
     a.w0 = S.fw0;
-    for (int l = 1; l < 4; ++l) a.w[l - 1] = S.w[l];
+    for (int l = 1; l < 4; ++l) a.w[l - 1] = S.fwh + (size_t)(l - 1) * MLP_H * MLP_H;
     for (int l = 0; l < 4; ++l) a.b[l] = S.b[l];
     a.w4h = S.fw4h;
     a.w4l = S.fw4l;
