@@ -298,9 +298,10 @@ def _surrogate_calibration(acc):
 
 
 def surrogate_ablation(acc, cam, metric, ref, steps, warmup):
-    """The surrogate as C4 configures it: trained once on 2^24 integrator-labelled rays (4096
-    cameras of 64 x 64, a ~ U[0, 0.99], inclination ~ U[5, 89] deg, r_obs ~ U[20, 100] M, FOV
-    20 deg; training and its held-out calibration excluded from the frame time, reported apart).
+    """The surrogate as C4 configures it: trained once on 2^24 integrator-labelled rays (4113
+    cameras of 85 x 48 - the frame's 16:9 aspect - a ~ U[0, 0.99], inclination ~ U[5, 89] deg,
+    r_obs ~ U[20, 100] M, FOV 20 deg; training and its held-out calibration excluded from the
+    frame time, reported apart).
     Then: the error gate's coverage against its threshold on the held-out rays; the headline frame
     with the surrogate on; a sky-dominated view (r_obs 80 M, inclination 85 deg) on and off; and
     the fused inference kernel's throughput."""
@@ -311,10 +312,11 @@ def surrogate_ablation(acc, cam, metric, ref, steps, warmup):
     from sirius_amd.accelerator import default_opts
     g = MT19937_64(0x51121006)
     cams, metrics = [], []
-    for i in range(4096):
+    n_train_cam, tw, th = 4113, 85, 48  # 2^24 rays and more, the frame's 16:9 aspect
+    for i in range(n_train_cam):
         a, incl, r_obs = g.uniform(0.0, 0.99), g.uniform(5.0, 89.0), g.uniform(20.0, 100.0)
         metrics.append(abi.metric_ks(1.0, a))
-        cams.append(abi.camera(r_obs, incl, 20.0, 64, 64, disk=(0.0, 30.0), metric_index=i))
+        cams.append(abi.camera(r_obs, incl, 20.0, tw, th, disk=(0.0, 30.0), metric_index=i))
     lab = default_opts(abi.PRECISION_FP64)
     acc.mlp_init(0x51121006)
     epochs = 200
@@ -326,7 +328,8 @@ def surrogate_ablation(acc, cam, metric, ref, steps, warmup):
     for tau in (2e-5, 5e-5, 1e-4, 3e-4, 1e-3):
         ok = (n >= 20) & (bad == 0) & (err <= tau)
         curve.append({"err_max_rad": tau, "bins": int(ok.sum()), "held_out_answers_kept": int(n[ok].sum())})
-    out = {"train_rays": 4096 * 64 * 64, "train_epochs": epochs, "train_s": train_s, "train_final_loss": loss,
+    out = {"train_rays": n_train_cam * tw * th, "train_cameras": f"{n_train_cam} x {tw}x{th}, FOV 20 deg",
+           "train_epochs": epochs, "train_s": train_s, "train_final_loss": loss,
            "held_out_answers": int(n.sum()), "held_out_misclassified": int(bad.sum()),
            "coverage_vs_err_max": curve, "err_max_default": lab.surrogate_err_max}
 

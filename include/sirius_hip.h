@@ -337,11 +337,12 @@ int sirius_hip_autotune(SiriusHipHandle h, const SiriusHipCamera* cams, int n_ca
  * beta: the ray's impact parameters on the observer's sky, M).
  * The network is asked only about candidates: rays outside surrogate_band of the critical curve
  * and outside the shadow whose reference orbit scatters and crosses the equatorial plane only
- * outside the thin disk widened by surrogate_margin (the reference's own test, analytic). A
- * candidate is answered when its most probable class is escaped with probability >=
- * surrogate_pmin and the calibration of its bin (spin x critical distance x observer radius)
- * measured, on held-out rays, no misclassification and a largest direction error <=
- * surrogate_err_max; every other ray is integrated.
+ * outside the thin disk widened by surrogate_margin (the reference's own test, analytic), inside
+ * the training cameras' field of view (image-plane box), and in a calibration bin (spin x
+ * critical distance x observer radius) whose held-out rays showed no misclassification and a
+ * largest direction error <= surrogate_err_max, at a critical distance no larger than those
+ * held-out rays reached. A candidate is answered when its most probable class is escaped with
+ * probability >= surrogate_pmin; every other ray is integrated.
  * Parameters are FP32 masters on the device, laid out layer by layer as W_l (out x in,
  * row-major) then b_l. The hidden layers run on BF16 MFMA with FP32 accumulation (operands
  * rounded to BF16: activations, weights and pre-activation gradients); the output head (256 ->

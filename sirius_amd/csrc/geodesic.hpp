@@ -42,6 +42,7 @@ struct CamConst {
     double M, a, Q2, Lambda;
     double b0;                       // wormhole throat
     double wv, wR, wsig, winv2tanh; // warp: v, R, sigma, 1/(2 tanh(sigma R))
+    double wE, wN;                  // warp: e^{-4 sigma R}, (1 - e^{-4 sigma R}) / tanh(sigma R)
     // termination and emission
     double r_cap, r_in, r_out;
     double fmax;                     // max of f(r)/r over the disk (Page-Thorne)
@@ -206,6 +207,7 @@ using ::fabs;
 using ::fma;
 using ::fmax;
 using ::fmin;
+using ::exp;
 using ::sqrt;
 using ::tanh;
 template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> sqrt(const Dual<T, N>& a) {
@@ -215,6 +217,10 @@ template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> sqrt(const Dual<T
 template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> fabs(const Dual<T, N>& a) { return a.v < T(0) ? -a : a; }
 template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> fmax(const Dual<T, N>& a, const Dual<T, N>& b) { return a.v < b.v ? b : a; }
 template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> fmin(const Dual<T, N>& a, const Dual<T, N>& b) { return b.v < a.v ? b : a; }
+template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> exp(const Dual<T, N>& a) {
+    const T e = exp(a.v);
+    return chain(a, e, e);
+}
 template <class T, int N> SIRIUS_HD __forceinline__ Dual<T, N> tanh(const Dual<T, N>& a) {
     const T t = tanh(a.v);
     return chain(a, t, T(1) - t * t);
@@ -233,6 +239,7 @@ struct Spacetime {
     T M, a, a2, Q2, Lambda;
     T b0, b02;
     T wv, wR, wsig, winv2tanh;
+    T wE, wN;
 };
 
 template <class T>
@@ -242,7 +249,26 @@ SIRIUS_HD __forceinline__ Spacetime<T> make_spacetime(const CamConst& c) {
     s.M = T(c.M); s.a = T(c.a); s.a2 = T(c.a * c.a); s.Q2 = T(c.Q2); s.Lambda = T(c.Lambda);
     s.b0 = T(c.b0); s.b02 = T(c.b0 * c.b0);
     s.wv = T(c.wv); s.wR = T(c.wR); s.wsig = T(c.wsig); s.winv2tanh = T(c.winv2tanh);
+    s.wE = T(c.wE); s.wN = T(c.wN);
     return s;
+}
+
+// Alcubierre shape function f(r_s) = (tanh(sigma (r_s + R)) - tanh(sigma (r_s - R))) / (2 tanh(sigma R))
+// and df/dr_s, by tanh A - tanh B = sinh(A - B) / (cosh A cosh B) with A - B = 2 sigma R: with
+// v = e^{-2 sigma (r_s - R)}, E = e^{-4 sigma R} and N = (1 - E) / tanh(sigma R),
+//   f = N v / D,  df/dr_s = -2 sigma f (1 - E v^2) / D,  D = 1 + v (1 + E + E v).
+// One exponential instead of two hyperbolic tangents; its argument is small at the wall, where f
+// varies, so it keeps the digits the tanh of sigma (r_s - R) keeps; E v = e^{-2 sigma (r_s + R)}
+// and E v^2 = e^{-4 sigma r_s} stay below one, and v below e^{2 sigma R} (sigma R <= 40).
+template <class T>
+SIRIUS_HD __forceinline__ T warp_shape(const Spacetime<T>& s, T rs, T& dfdr) {
+    const T v = exp(-T(2) * s.wsig * (rs - s.wR));
+    const T ev = s.wE * v;
+    const T D = tfma(v, T(1) + s.wE + ev, T(1));
+    const T iD = T(1) / D;
+    const T f = s.wN * v * iD;
+    dfdr = -T(2) * s.wsig * f * (T(1) - ev * v) * iD;
+    return f;
 }
 
 // Kerr radius (value only), PHMT100A.h:198-216.
@@ -298,7 +324,8 @@ SIRIUS_HD __forceinline__ void metric_cov(const Spacetime<T>& s, const T x[4], T
     }
     if (s.family == 2) {
         T rs = sqrt(x[1] * x[1] + x[2] * x[2] + x[3] * x[3]);
-        T f = (tanh(s.wsig * (rs + s.wR)) - tanh(s.wsig * (rs - s.wR))) * s.winv2tanh;
+        T dfdr;
+        T f = warp_shape(s, rs, dfdr);
         g[0][0] = -(T(1) - s.wv * s.wv * f * f);
         g[0][1] = g[1][0] = -s.wv * f;
         return;
@@ -413,9 +440,8 @@ SIRIUS_HD __forceinline__ T rhs_warp(const Spacetime<T>& s, const T x[4], const 
     T R2 = x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
     R2 = R2 < T(1e-20) ? T(1e-20) : R2;
     T rs = sqrt(R2);
-    T tp = tanh(s.wsig * (rs + s.wR)), tm = tanh(s.wsig * (rs - s.wR));
-    T f = (tp - tm) * s.winv2tanh;
-    T dfdr = s.wsig * ((T(1) - tp * tp) - (T(1) - tm * tm)) * s.winv2tanh;
+    T dfdr;
+    T f = warp_shape(s, rs, dfdr);
     T vf = s.wv * f;
     k[0] = -p[0] - vf * p[1];
     k[1] = -vf * p[0] + (T(1) - vf * vf) * p[1];
@@ -443,7 +469,8 @@ SIRIUS_HD __forceinline__ void raise_index(const Spacetime<T>& s, const T x[4], 
         k[0] = -p[0]; k[1] = p[1]; k[2] = p[2] / (x[1] * x[1] + s.b02); k[3] = p[3];
     } else if constexpr (FAM == 2) {
         T rs = sqrt(x[1] * x[1] + x[2] * x[2] + x[3] * x[3]);
-        T vf = s.wv * (tanh(s.wsig * (rs + s.wR)) - tanh(s.wsig * (rs - s.wR))) * s.winv2tanh;
+        T dfdr;
+        T vf = s.wv * warp_shape(s, rs, dfdr);
         k[0] = -p[0] - vf * p[1];
         k[1] = -vf * p[0] + (T(1) - vf * vf) * p[1];
         k[2] = p[2];

@@ -118,6 +118,13 @@ struct MlpState {
     float* cal_err = nullptr;
     uint32_t* cal_n = nullptr;
     uint32_t* cal_bad = nullptr;
+    // the largest distance from the critical curve among a bin's held-out answers: the bin
+    // answers no ray beyond it (the calibration holds no evidence there)
+    float* cal_dmax = nullptr;
+    // the training cameras' field of view: the largest |x| and |y| of a pixel centre's direction
+    // on the image plane at unit distance. Rays outside this box lie outside the training set's
+    // support in (alpha, beta) for their observer and are never answered.
+    float sup_x = 0.f, sup_y = 0.f;
     bool ready = false;
 };
 

@@ -166,7 +166,11 @@ bool make_cam_const(const SiriusHipCamera& cam, const SiriusHipMetric& m, int64_
     c.wv = m.warp_v; c.wR = m.warp_R; c.wsig = m.warp_sigma;
     if (m.family == SIRIUS_METRIC_ALCUBIERRE) {
         if (!(m.warp_sigma > 0 && m.warp_R > 0)) { err = "warp sigma and R must be positive"; return false; }
+        // the shape function's exponential stays inside the FP32 range for sigma R <= 40
+        if (m.warp_sigma * m.warp_R > 40.0) { err = "warp sigma R must be at most 40"; return false; }
         c.winv2tanh = 1.0 / (2.0 * std::tanh(m.warp_sigma * m.warp_R));
+        c.wE = std::exp(-4.0 * m.warp_sigma * m.warp_R);
+        c.wN = (1.0 - c.wE) / std::tanh(m.warp_sigma * m.warp_R);
     }
     if (m.family == SIRIUS_METRIC_MORRIS_THORNE) {
         if (!(m.throat_b0 > 0) || cam.r_obs <= m.throat_b0) { err = "wormhole needs 0 < b0 < r_obs"; return false; }

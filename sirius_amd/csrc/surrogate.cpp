@@ -31,7 +31,7 @@ void mlp_release(Handle& H) {
         if (S.b[l]) (void)hipFree(S.b[l]);
     }
     for (void* p : {(void*)S.fw0, (void*)S.fwh, (void*)S.fw4h, (void*)S.fw4l, (void*)S.cal_err, (void*)S.cal_n,
-                    (void*)S.cal_bad})
+                    (void*)S.cal_bad, (void*)S.cal_dmax})
         if (p) (void)hipFree(p);
     delete H.mlp;
     H.mlp = nullptr;
@@ -58,7 +58,8 @@ static int mlp_alloc(Handle& H) {
         (e = hipMalloc((void**)&S.fw4l, (size_t)32 * MLP_H * 2)) != hipSuccess ||
         (e = hipMalloc((void**)&S.cal_err, SUR_BINS * 4)) != hipSuccess ||
         (e = hipMalloc((void**)&S.cal_n, SUR_BINS * 4)) != hipSuccess ||
-        (e = hipMalloc((void**)&S.cal_bad, SUR_BINS * 4)) != hipSuccess)
+        (e = hipMalloc((void**)&S.cal_bad, SUR_BINS * 4)) != hipSuccess ||
+        (e = hipMalloc((void**)&S.cal_dmax, SUR_BINS * 4)) != hipSuccess)
         return H.hip_fail(e, "mlp alloc");
     return SIRIUS_HIP_OK;
 }
@@ -68,11 +69,13 @@ hipError_t launch_fill(uint32_t* p, int n, uint32_t v, hipStream_t st);
 // New parameters invalidate the calibration: the error gate answers nothing until the network
 // is calibrated again (sirius_hip_surrogate_train calibrates on held-out rays).
 static hipError_t mlp_clear_calibration(MlpState& S, hipStream_t st) {
+    S.sup_x = S.sup_y = 0.f;
     hipError_t e;
     if ((e = launch_fill((uint32_t*)S.cal_err, SUR_BINS, 0u, st)) != hipSuccess ||
         (e = launch_fill(S.cal_n, SUR_BINS, 0u, st)) != hipSuccess)
         return e;
-    return launch_fill(S.cal_bad, SUR_BINS, 0u, st);
+    if ((e = launch_fill(S.cal_bad, SUR_BINS, 0u, st)) != hipSuccess) return e;
+    return launch_fill((uint32_t*)S.cal_dmax, SUR_BINS, 0u, st);
 }
 
 // BF16 operands of every kernel from the FP32 masters (after each update).
@@ -270,14 +273,16 @@ hipError_t launch_sur_gate(const CamConst* cams, int n_cam, const float* tabs, i
                            double long_band, double sur_band, double margin, double r_exit, int32_t* list_a,
                            int32_t* list_b, int32_t* list_l, int32_t* list_s, uint32_t* counts, uint8_t* flags,
                            float* rec, const float* cal_err, const uint32_t* cal_n, const uint32_t* cal_bad,
-                           double err_max, hipStream_t st);
+                           const float* cal_dmax, double err_max, float sup_x, float sup_y, hipStream_t st);
 hipError_t launch_sur_decide(const int32_t* list_s, const uint32_t* count_s, int64_t cap, const float* y,
                              const float* rec, const float* cal_err, const uint32_t* cal_n, const uint32_t* cal_bad,
+                             const float* cal_dmax,
                              double pmin, double err_max, double audit_fraction, uint64_t audit_seed, const DevOut& o,
                              int n_bins, int32_t* list_a, uint32_t* count_a, int32_t* list_audit,
                              uint32_t* count_audit, uint32_t* n_gated, hipStream_t st);
 hipError_t launch_sur_calibrate(const int32_t* list, int64_t m, const float* y, const float* rec, const DevOut& o,
-                                double pmin, float* cal_err, uint32_t* cal_n, uint32_t* cal_bad, hipStream_t st);
+                                double pmin, float* cal_err, uint32_t* cal_n, uint32_t* cal_bad, float* cal_dmax,
+                                hipStream_t st);
 hipError_t launch_sur_gather(const int32_t* perm, int64_t start, int64_t m, const float* x, const int32_t* cls,
                              const float* dir, float* xb, int32_t* cb, float* db, hipStream_t st);
 
@@ -300,7 +305,7 @@ int surrogate_stage(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& do
     if ((e = launch_sur_gate(B.cams_dev, (int)B.cc.size(), tabs, n, std::max(o.escalate_band, 0.0),
                              std::max(o.critical_band, 0.0), 1.0, o.surrogate_band, o.surrogate_margin, o.r_escape,
                              list_a, list_b, list_l, list_s, work + 16, dout.flags, rec, S.cal_err, S.cal_n, S.cal_bad,
-                             o.surrogate_err_max, H.stream)) != hipSuccess)
+                             S.cal_dmax, o.surrogate_err_max, S.sup_x, S.sup_y, H.stream)) != hipSuccess)
         return H.hip_fail(e, "surrogate gate");
     MlpFusedArgs a = mlp_fused_args(S, rec, y, 0);
     a.n_dev = work + 19;
@@ -308,7 +313,7 @@ int surrogate_stage(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& do
     a.x_stride = SUR_REC;
     if ((e = launch_mlp_fused(a, n, 2 * std::max(H.cu_count, 1), H.stream)) != hipSuccess)
         return H.hip_fail(e, "surrogate network");
-    if ((e = launch_sur_decide(list_s, work + 19, n, y, rec, S.cal_err, S.cal_n, S.cal_bad, o.surrogate_pmin,
+    if ((e = launch_sur_decide(list_s, work + 19, n, y, rec, S.cal_err, S.cal_n, S.cal_bad, S.cal_dmax, o.surrogate_pmin,
                                o.surrogate_err_max, o.audit_fraction, o.audit_seed, dout, o.n_bins, list_a, work + 16,
                                list_audit, work + 10, work + 20, H.stream)) != hipSuccess)
         return H.hip_fail(e, "surrogate decision");
@@ -536,9 +541,18 @@ int sirius_hip_surrogate_train(SiriusHipHandle hh, const SiriusHipCamera* cams, 
         a.x_stride = SUR_REC;
         if ((e = launch_mlp_fused(a, nh, 1 << 30, H->stream)) != hipSuccess ||
             (e = launch_sur_calibrate(hol, nh, yh, rec, dout, label_opts->surrogate_pmin, S.cal_err, S.cal_n,
-                                      S.cal_bad, H->stream)) != hipSuccess)
+                                      S.cal_bad, S.cal_dmax, H->stream)) != hipSuccess)
             return H->hip_fail(e, "calibration");
     }
+    // the support of the training set on the image plane: the largest pixel-centre coordinates
+    // of the training cameras (as the device's ray construction computes them, in FP32)
+    float sx = 0.f, sy = 0.f;
+    for (const CamConst& c : B.cc) {
+        sx = std::max(sx, (1.f - 1.f / (float)c.width) * (float)c.aspect * (float)c.tan_half);
+        sy = std::max(sy, (1.f - 1.f / (float)c.height) * (float)c.tan_half);
+    }
+    S.sup_x = sx;
+    S.sup_y = sy;
     if ((e = hipStreamSynchronize(H->stream)) != hipSuccess) return H->hip_fail(e, "synchronize");
     if (loss) *loss = (float)last;
     return SIRIUS_HIP_OK;

@@ -36,20 +36,25 @@ __device__ __forceinline__ int sur_bin(float a, float d, float r_obs) {
 __device__ __forceinline__ int sur_bin_rec(const float* rr) { return sur_bin(rr[0], rr[5], rr[2] * 40.f + 60.f); }
 
 // Calibration of the error gate: a bin answers when its held-out rays number at least
-// SUR_CAL_MIN, none was misclassified, and the largest direction error is within err_max.
+// SUR_CAL_MIN, none was misclassified, and the largest direction error is within err_max; it
+// answers a ray only within the distance from the critical curve that its held-out answers
+// reached (no extrapolation beyond the evidence, such as the corners of a wider frame).
 struct SurCal {
     const float* err;
     const uint32_t* n;
     const uint32_t* bad;
+    const float* dmax;
     float err_max;
+    float sup_x, sup_y;  // the training cameras' image-plane box (see MlpState)
 };
-__device__ __forceinline__ bool sur_bin_usable(const SurCal& cal, int b) {
-    return cal.n[b] >= SUR_CAL_MIN && cal.bad[b] == 0u && cal.err[b] <= cal.err_max;
+__device__ __forceinline__ bool sur_bin_usable(const SurCal& cal, int b, float d) {
+    return cal.n[b] >= SUR_CAL_MIN && cal.bad[b] == 0u && cal.err[b] <= cal.err_max && d <= cal.dmax[b];
 }
 
 // The analytic reference of one ray (FP32, from the camera): its signed distance from the
 // critical curve, whether the surrogate may be asked (outside the fallback band and the shadow,
-// in a bin of `cal` that answers when cal is given, its Schwarzschild orbit scattered and crossing
+// inside the training cameras' field of view and in a bin of `cal` that answers when cal is
+// given, its Schwarzschild orbit scattered and crossing
 // the equatorial plane only outside the thin disk widened by `margin`), and for such a candidate
 // the record rec[SUR_REC] (network inputs, distance, b, reference exit direction). Returns the
 // candidate decision.
@@ -61,9 +66,16 @@ __device__ __forceinline__ bool sur_reference(const CamConst& c, const float* __
     init_state<float>(c, s, px, py, x, p);
     dist = crit_distance<float>(tab, s, x, p);
     if (!(dist >= sur_band)) return false;
+    if (cal) {
+        // inside the training cameras' field of view (the same image-plane coordinates as the
+        // camera's ray construction)
+        const float sx = ((2.f * (px + 0.5f)) / (float)c.width - 1.f) * (float)c.aspect * (float)c.tan_half;
+        const float sy = (1.f - (2.f * (py + 0.5f)) / (float)c.height) * (float)c.tan_half;
+        if (!(fabsf(sx) <= cal->sup_x && fabsf(sy) <= cal->sup_y)) return false;
+    }
     // a ray whose calibration bin cannot answer skips the orbit test and the network (the bin
     // from the same FP32 values the record carries)
-    if (cal && !sur_bin_usable(*cal, sur_bin((float)c.a, dist, (float)((c.r_obs - 60.0) / 40.0) * 40.f + 60.f)))
+    if (cal && !sur_bin_usable(*cal, sur_bin((float)c.a, dist, (float)((c.r_obs - 60.0) / 40.0) * 40.f + 60.f), dist))
         return false;
     float n[3];
     camera_cart_dir<float>(c, px, py, n);
@@ -237,7 +249,8 @@ __global__ __launch_bounds__(256) void k_sur_gate(const CamConst* __restrict__ c
 __global__ __launch_bounds__(256) void k_sur_decide(const int32_t* __restrict__ list_s, const uint32_t* __restrict__ count_s,
                                                     const float* __restrict__ y, const float* __restrict__ rec,
                                                     const float* __restrict__ cal_err, const uint32_t* __restrict__ cal_n,
-                                                    const uint32_t* __restrict__ cal_bad, float pmin, float err_max,
+                                                    const uint32_t* __restrict__ cal_bad,
+                                                    const float* __restrict__ cal_dmax, float pmin, float err_max,
                                                     double audit_fraction, uint64_t audit_seed, DevOut o, int n_bins,
                                                     int32_t* __restrict__ list_a, uint32_t* __restrict__ count_a,
                                                     int32_t* __restrict__ list_audit, uint32_t* __restrict__ count_audit,
@@ -260,7 +273,8 @@ __global__ __launch_bounds__(256) void k_sur_decide(const int32_t* __restrict__ 
             const float e0 = expf(yy[0] - m), e1 = expf(yy[1] - m), e2 = expf(yy[2] - m);
             const float p0 = e0 / (e0 + e1 + e2);
             const bool esc = e0 >= e1 && e0 >= e2 && p0 >= pmin;
-            const bool cal = sur_bin_usable(SurCal{cal_err, cal_n, cal_bad, err_max}, sur_bin_rec(rr));
+            const bool cal = sur_bin_usable(SurCal{cal_err, cal_n, cal_bad, cal_dmax, err_max, 0.f, 0.f}, sur_bin_rec(rr),
+                                            rr[5]);
             gated = esc && !cal;
             if (esc && cal) {
                 uint8_t fl = o.flags[id] | (uint8_t)SIRIUS_FLAG_SURROGATE;
@@ -317,7 +331,8 @@ __global__ __launch_bounds__(256) void k_sur_decide(const int32_t* __restrict__ 
 // bin counts it, records a misclassification, or keeps the largest direction error.
 __global__ void k_sur_calibrate(const int32_t* __restrict__ list, int64_t m, const float* __restrict__ y,
                                 const float* __restrict__ rec, DevOut o, float pmin, float* __restrict__ cal_err,
-                                uint32_t* __restrict__ cal_n, uint32_t* __restrict__ cal_bad) {
+                                uint32_t* __restrict__ cal_n, uint32_t* __restrict__ cal_bad,
+                                float* __restrict__ cal_dmax) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m) return;
     const int64_t id = list[k];
@@ -346,6 +361,7 @@ __global__ void k_sur_calibrate(const int32_t* __restrict__ list, int64_t m, con
     const float err = (float)(2.0 * asin(fmin(1.0, 0.5 * sqrt(dd))));
     atomicAdd(&cal_n[b], 1u);
     atomicMax((unsigned int*)&cal_err[b], __float_as_uint(err));
+    atomicMax((unsigned int*)&cal_dmax[b], __float_as_uint(rr[5]));  // distances are positive
 }
 
 // Gathers of a candidate subset: network inputs from the records (by ray id) into x[5 k..].
@@ -390,30 +406,32 @@ hipError_t launch_sur_gate(const CamConst* cams, int n_cam, const float* tabs, i
                            double long_band, double sur_band, double margin, double r_exit, int32_t* list_a,
                            int32_t* list_b, int32_t* list_l, int32_t* list_s, uint32_t* counts, uint8_t* flags,
                            float* rec, const float* cal_err, const uint32_t* cal_n, const uint32_t* cal_bad,
-                           double err_max, hipStream_t st) {
+                           const float* cal_dmax, double err_max, float sup_x, float sup_y, hipStream_t st) {
     hipLaunchKernelGGL(k_sur_gate, dim3((unsigned)((n + SG_RAYS - 1) / SG_RAYS)), dim3(256), 0, st, cams, n_cam, tabs, n,
                        (float)band, (float)crit_band, (float)long_band, (float)sur_band, (float)margin, (float)r_exit,
                        list_a, list_b, list_l, list_s, counts, flags, rec,
-                       SurCal{cal_err, cal_n, cal_bad, (float)err_max});
+                       SurCal{cal_err, cal_n, cal_bad, cal_dmax, (float)err_max, sup_x, sup_y});
     return hipGetLastError();
 }
 hipError_t launch_sur_decide(const int32_t* list_s, const uint32_t* count_s, int64_t cap, const float* y,
                              const float* rec, const float* cal_err, const uint32_t* cal_n, const uint32_t* cal_bad,
-                             double pmin, double err_max, double audit_fraction, uint64_t audit_seed, const DevOut& o,
+                             const float* cal_dmax, double pmin, double err_max, double audit_fraction,
+                             uint64_t audit_seed, const DevOut& o,
                              int n_bins, int32_t* list_a, uint32_t* count_a, int32_t* list_audit,
                              uint32_t* count_audit, uint32_t* n_gated, hipStream_t st) {
     int64_t blocks = (cap + 255) / 256;
     blocks = blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks);
     hipLaunchKernelGGL(k_sur_decide, dim3((unsigned)blocks), dim3(256), 0, st, list_s, count_s, y, rec, cal_err, cal_n,
-                       cal_bad, (float)pmin, (float)err_max, audit_fraction, audit_seed, o, n_bins, list_a, count_a,
-                       list_audit, count_audit, n_gated);
+                       cal_bad, cal_dmax, (float)pmin, (float)err_max, audit_fraction, audit_seed, o, n_bins, list_a,
+                       count_a, list_audit, count_audit, n_gated);
     return hipGetLastError();
 }
 hipError_t launch_sur_calibrate(const int32_t* list, int64_t m, const float* y, const float* rec, const DevOut& o,
-                                double pmin, float* cal_err, uint32_t* cal_n, uint32_t* cal_bad, hipStream_t st) {
+                                double pmin, float* cal_err, uint32_t* cal_n, uint32_t* cal_bad, float* cal_dmax,
+                                hipStream_t st) {
     if (m <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_sur_calibrate, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, list, m, y, rec, o,
-                       (float)pmin, cal_err, cal_n, cal_bad);
+                       (float)pmin, cal_err, cal_n, cal_bad, cal_dmax);
     return hipGetLastError();
 }
 hipError_t launch_sur_inputs(const int32_t* list, int64_t m, const float* rec, float* x, hipStream_t st) {

@@ -73,8 +73,9 @@ SIRIUS_HD __forceinline__ void renorm_energy64(const Spacetime<double>& s, const
 // carries the code (and registers) of its own model. TIME (FP64 only) integrates the
 // coordinate time, which enters the reference's error norm (PHGD001A.cpp:508-517).
 // ------------------------------------------------------------------------------------------
-// Volumetric emission: up to this many spectral bins accumulate in registers (FP64) for the
-// ray's whole trace and are stored once when it ends; more bins accumulate in device memory.
+// Volumetric emission: up to this many spectral bins accumulate in registers for the ray's whole
+// trace (in the path's precision: an FP32 ray's few hundred node contributions add up to well
+// inside its 1e-3 bar) and are stored once when it ends; more bins accumulate in device memory.
 constexpr int SIRIUS_VOL_REG = 8;
 
 template <class T, int FAM>
@@ -91,7 +92,7 @@ struct Ray {
     int cam;
     int cls;        // termination class of a finished ray
     uint32_t attempts, rejected, landings, flags;
-    double vacc[SIRIUS_VOL_REG];  // volumetric emission accumulators (n_bins <= SIRIUS_VOL_REG)
+    T vacc[SIRIUS_VOL_REG];  // volumetric emission accumulators (n_bins <= SIRIUS_VOL_REG)
 };
 
 template <class T, int FAM, int EMIT>
@@ -133,7 +134,7 @@ SIRIUS_HD __forceinline__ void ray_start(Ray<T, FAM>& R, const CamConst* cams, i
     R.flags = tp.gate_flags ? (o.flags[id] & SIRIUS_FLAG_CRITICAL) : 0;
     if constexpr (EMIT == SIRIUS_EMISSION_VOLUMETRIC) {
 #pragma unroll
-        for (int k = 0; k < SIRIUS_VOL_REG; ++k) R.vacc[k] = 0.0;
+        for (int k = 0; k < SIRIUS_VOL_REG; ++k) R.vacc[k] = T(0);
         if (tp.n_bins > SIRIUS_VOL_REG)
             for (int k = 0; k < tp.n_bins; ++k) o.vol_acc[(int64_t)k * o.n_rays + id] = 0.0;
     }
@@ -149,7 +150,7 @@ SIRIUS_HD __forceinline__ void ray_store(const Ray<T, FAM>& R, const DevOut& o, 
         if (tp.n_bins <= SIRIUS_VOL_REG) {
 #pragma unroll
             for (int k = 0; k < SIRIUS_VOL_REG; ++k)
-                if (k < tp.n_bins) o.vol_acc[(int64_t)k * o.n_rays + id] = R.vacc[k];
+                if (k < tp.n_bins) o.vol_acc[(int64_t)k * o.n_rays + id] = (double)R.vacc[k];
         }
     }
     o.cls[id] = (uint8_t)R.cls;
@@ -249,6 +250,8 @@ SIRIUS_HD __forceinline__ void vol_accumulate(Ray<T, FAM>& R, const CamConst& c,
     const T gw[3] = {T(5.0 / 18.0), T(8.0 / 18.0), T(5.0 / 18.0)};
     double xk0, qk0;
     bin_geometry(tp, xk0, qk0);
+    // one node at a time: unrolled, the three nodes' spectra would be live together
+#pragma unroll 1
     for (int q = 0; q < 3; ++q) {
         T xq[4];
         hermite(R.x, R.d.k, x1, d1.k, R.h, gn[q], xq);
@@ -286,7 +289,7 @@ SIRIUS_HD __forceinline__ void vol_accumulate(Ray<T, FAM>& R, const CamConst& c,
             for (int k = 0; k < SIRIUS_VOL_REG; ++k) {
                 if (k < tp.n_bins) {
                     T xe = xk / gf;
-                    R.vacc[k] += (double)(w * xe * xe * xe / expm1(xe / tv));
+                    R.vacc[k] += w * xe * xe * xe / expm1(xe / tv);
                 }
                 xk *= qk;
             }

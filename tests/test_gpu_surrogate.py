@@ -30,7 +30,7 @@ def _angle(u, v):
     return 2.0 * np.arcsin(np.minimum(np.linalg.norm(u - v, axis=1) / 2.0, 1.0))
 
 
-def _c4_cameras(n_cam, side, seed):
+def _c4_cameras(n_cam, width, height, seed):
     """C4's sampling: a ~ U[0, 0.99], inclination ~ U[5, 89] deg, r_obs ~ U[20, 100] M, FOV 20 deg
     (the pixel grid spans the field of view), thin disk from the ISCO to 30 M."""
     g = MT19937_64(seed)
@@ -38,8 +38,13 @@ def _c4_cameras(n_cam, side, seed):
     for i in range(n_cam):
         a, incl, r_obs = g.uniform(0.0, 0.99), g.uniform(5.0, 89.0), g.uniform(20.0, 100.0)
         metrics.append(abi.metric_ks(1.0, a))
-        cams.append(abi.camera(r_obs, incl, 20.0, side, side, disk=(0.0, 30.0), metric_index=i))
+        cams.append(abi.camera(r_obs, incl, 20.0, width, height, disk=(0.0, 30.0), metric_index=i))
     return cams, metrics
+
+
+# C4's training set: 2^24 rays and more (4113 cameras of 85 x 48), pixel offsets uniform over the
+# image of a FOV-20 camera with the benchmark frame's 16:9 aspect
+C4_TRAIN = (4113, 85, 48)
 
 
 def test_features_reference_and_candidates_match_the_restatement(acc):
@@ -96,8 +101,10 @@ def _calibration(acc):
 
 
 def test_surrogate_c4_training_decision_audit_and_ablation(acc):
-    # C4: 2^24 integrator-labelled rays (4096 cameras of 64 x 64), labels from the FP64 integrator
-    cams, metrics = _c4_cameras(4096, 64, 0x51121006)
+    # C4: 2^24 integrator-labelled rays (4113 cameras of 85 x 48), labels from the FP64 integrator
+    n_cam, tw, th = C4_TRAIN
+    assert n_cam * tw * th >= 1 << 24
+    cams, metrics = _c4_cameras(n_cam, tw, th, 0x51121006)
     lab = default_opts(abi.PRECISION_FP64)
     acc.mlp_init(0x51121006)
     print("[case] surrogate train C4 2^24 rays", flush=True)
@@ -110,7 +117,7 @@ def test_surrogate_c4_training_decision_audit_and_ablation(acc):
     assert usable.any()
 
     # evaluation on cameras the network never saw, at C4's settings
-    ecams, emets = _c4_cameras(64, 128, 0x5112100B)
+    ecams, emets = _c4_cameras(64, 128, 128, 0x5112100B)
     on = default_opts(abi.PRECISION_FP32_ESCALATE)
     on.use_surrogate = 1
     assert (on.surrogate_pmin, on.surrogate_band, on.audit_fraction) == (0.99, 0.5, 0.01)
@@ -134,6 +141,18 @@ def test_surrogate_c4_training_decision_audit_and_ablation(acc):
     if aud.any():
         ea = _angle(got.direction()[aud], ref.direction()[aud])
         assert abs(st["audit_dir_err_max"] - ea.max()) <= 1e-6 + 1e-3 * ea.max()
+
+    # the 1080p headline geometry at 320 x 180: every answer meets the bar
+    wide = abi.camera(50.0, 75.0, 20.0, 320, 180, disk=(0.0, 30.0))
+    wm = abi.metric_ks(1.0, 0.9)
+    print("[case] surrogate on, 320x180 Kerr a=0.9", flush=True)
+    gw, stw = acc.trace_batch([wide], [wm], on)
+    rw, _ = acc.trace_batch([wide], [wm], default_opts(abi.PRECISION_FP64))
+    sw = (gw.flags & abi.FLAG_SURROGATE) != 0
+    if sw.any():
+        ew = _angle(gw.direction()[sw], rw.direction()[sw])
+        print(f"[surrogate] 320x180 frame: answered {sw.sum()}, direction error max {ew.max():.2e} rad", flush=True)
+        assert np.all(rw.cls[sw] == abi.RAY_ESCAPED) and ew.max() <= 1e-4
 
     # ablation: the rays the surrogate did not answer are the FP32 path's own, bit for bit
     off = default_opts(abi.PRECISION_FP32_ESCALATE)
