@@ -307,7 +307,6 @@ def surrogate_ablation(acc, cam, metric, ref, steps, warmup):
     the fused inference kernel's throughput."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from workloads import MT19937_64
-    from devmem import DeviceArray
     from sirius_amd import abi
     from sirius_amd.accelerator import default_opts
     g = MT19937_64(0x51121006)
@@ -359,19 +358,26 @@ def surrogate_ablation(acc, cam, metric, ref, steps, warmup):
     out["sky_view"] = {"workload": "1920x1080 Kerr a=0.9, r_obs 80 M, inclination 85 deg, FOV 20 deg",
                        "off_ms": ms_off * 1e3, "on_ms": ms_on * 1e3, "answered": st_on["rays_surrogate"],
                        "candidates": st_on["rays_candidates"], "surrogate_ms": st_on["surrogate_ms"]}
-    rows = 1 << 21
+    out["fused_inference"] = fused_inference(acc, 1 << 21)
+    return out
+
+
+def fused_inference(acc, rows, reps=10):
+    """The fused inference kernel on `rows` random inputs with the current network: wall time per
+    call over `reps` calls after one warm-up, and its BF16 MFMA rate (398,848 flops per row)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from devmem import DeviceArray
     xd = DeviceArray(np.random.default_rng(1).uniform(-1, 1, (rows, 5)).astype(np.float32))
     yd = DeviceArray(shape=(rows, 6), dtype=np.float32)
     acc.mlp_forward(xd.data_ptr(), yd.data_ptr(), rows)
     acc.synchronise()
     t0 = time.perf_counter()
-    for _ in range(10):
+    for _ in range(reps):
         acc.mlp_forward(xd.data_ptr(), yd.data_ptr(), rows)
     acc.synchronise()
-    dt = (time.perf_counter() - t0) / 10
-    out["fused_inference"] = {"rows": rows, "ms": dt * 1e3, "tflops": rows * 398848 / dt / 1e12,
-                              "frac_of_bf16_peak": rows * 398848 / dt / 1e12 / 2500.0}
-    return out
+    dt = (time.perf_counter() - t0) / reps
+    return {"rows": rows, "ms": dt * 1e3, "tflops": rows * 398848 / dt / 1e12,
+            "frac_of_bf16_peak": rows * 398848 / dt / 1e12 / 2500.0}
 
 
 def symplectic_leg(acc, cam, metric):
@@ -454,6 +460,9 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a copy kernel of known size that "
                          "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs scale against)")
+    ap.add_argument("--mlp-only", type=int, default=0, metavar="ROWS",
+                    help="time only the surrogate's fused inference kernel on ROWS random rows (a freshly "
+                         "initialised network) and print that; for profiling the kernel alone")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this backend runs on one GPU; --gpus {args.gpus} is not supported "
@@ -465,6 +474,11 @@ def main():
     acc = HipAccelerator()
     if args.copy_calibration > 0:
         calibration_copy(args.copy_calibration << 20)
+    if args.mlp_only > 0:
+        acc.mlp_init(0x51121006)
+        print(json.dumps({"fused_inference": fused_inference(acc, args.mlp_only)}))
+        acc.close()
+        return
     cam, metric = workload()
     n_rays = W * H
 
