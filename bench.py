@@ -8,6 +8,9 @@ One step = one full frame traced on the GPU with every result left in device mem
   fp64_same_step_policy
                  the FP64 integrator at the FP32 path's step policy (rtol, far-field cap,
                  inward step proposal): the part of the speed-up that precision alone buys
+  precision_audit
+                 the per-call FP64 error estimate: 1 % of the frame's rays re-traced by the plain
+                 FP64 integrator inside the call, with class mismatches and error maxima/medians
   symplectic     the frame by the fixed-step symplectic integrator (h = 0.01 M, FP64) with its
                  null-constraint drift
   accuracy       the FP32 frame against the FP64 frame, pixel by pixel, outside the critical band
@@ -552,6 +555,19 @@ def main():
         "steps_per_ray": st64p[-1]["steps_bulk"] / n_rays, "speedup_of_fp32": ms64p / ms32,
         "policy": {"rtol": o64p.rtol, "atol": o64p.atol, "far_slope": o64p.far_slope,
                    "step_radial_exp": o64p.step_radial_exp}}
+    # the per-call error estimate: 1 % of the frame's rays re-traced by the plain FP64 integrator
+    # inside the same call and compared with the FP32 results (timed apart by HIP events)
+    oa = default_opts(abi.PRECISION_FP32_ESCALATE)
+    oa.block_size, oa.blocks_per_cu, oa.refill_min = o32.block_size, o32.blocks_per_cu, o32.refill_min
+    oa.precision_audit, oa.audit_seed = 0.01, 0x51121010
+    msa, sta = time_frames(acc, cam, metric, oa, max(1, args.steps // 2), 1)
+    sa = sta[-1]
+    result["precision_audit"] = {
+        "fraction": oa.precision_audit, "ms_per_step": msa * 1e3,
+        "audit_ms": float(np.mean([x["precision_audit_ms"] for x in sta])),
+        "audited": sa["precision_audited"], "class_mismatch": sa["precision_class_mismatch"],
+        "dir_err_max_rad": sa["precision_dir_err_max"], "dir_err_median_rad": sa["precision_dir_err_median"],
+        "int_err_max": sa["precision_int_err_max"], "int_err_median": sa["precision_int_err_median"]}
     if not args.no_symplectic:
         result["symplectic"] = symplectic_leg(acc, cam, metric)
     if not args.no_accuracy:

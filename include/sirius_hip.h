@@ -158,7 +158,9 @@ typedef struct SiriusHipTraceOpts {
     double surrogate_margin; /* surrogate: the analytic candidate test refuses a ray whose
                               * reference orbit crosses the equatorial plane within this distance
                               * (M) of the thin disk's [r_in, r_out]; default 2.5              */
-    double precision_audit;  /* FP32 path: this fraction of the traced rays (seeded by audit_seed)
+    double precision_audit;  /* FP32 path: this fraction of the traced rays outside the critical
+                              * band (not flagged SIRIUS_FLAG_CRITICAL: the rays the accuracy
+                              * contract covers; chosen by a hash of audit_seed and the ray index)
                               * is re-traced after the batch by the plain FP64 integrator at the
                               * FP64 defaults' tolerances and compared: the stats' precision_*
                               * fields estimate the call's error against FP64. 0 = off (default) */

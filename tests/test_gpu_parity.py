@@ -386,3 +386,75 @@ def test_results_in_caller_device_arrays_are_not_held(acc):
     ref, _ = acc.trace_batch([cam], [m], o, with_state=True)
     assert np.array_equal(want_cls, ref.cls)
     assert np.array_equal(got.state, ref.state) and np.array_equal(got.phi, ref.phi)
+
+
+def _audit_selection(n, fraction, seed):
+    """The precision audit's ray choice restated: splitmix64 of (seed ^ ray index), taken when it
+    is at most fraction x 2^64 (every ray at fraction >= 1)."""
+    ids = np.arange(n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = (np.uint64(seed) ^ ids) + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z ^= z >> np.uint64(31)
+    thr = (1 << 64) - 1 if fraction >= 1 else int(fraction * 18446744073709551615.0)
+    return z <= np.uint64(thr)
+
+
+def _chord_angle(ta, pa, tb, pb):
+    a, b = _dirs(ta, pa), _dirs(tb, pb)
+    return 2.0 * np.arcsin(np.minimum(1.0, 0.5 * np.linalg.norm(a - b, axis=1)))
+
+
+@pytest.mark.parametrize("fraction,seed", [(0.05, 0x51121010), (1.0, 0)])
+def test_precision_audit_matches_a_direct_fp64_comparison(acc, fraction, seed):
+    """The per-call error estimate of the FP32 path (SiriusHipTraceOpts::precision_audit): the
+    library re-traces a seeded fraction of the frame's rays with the plain FP64 integrator and
+    reports class mismatches, the largest escape-direction and per-bin intensity errors and their
+    medians, over rays outside the critical band (those the accuracy contract covers). Checked
+    against the same comparison made here between the FP32 frame and the plain FP64 frame: the
+    same rays chosen (the selection hash restated), the same counts, the maxima
+    equal to float rounding, the medians within one quarter-decade histogram bin."""
+    from sirius_amd.accelerator import default_opts
+    w, h = 480, 270
+    cam = abi.camera(50.0, 75.0, 20.0, w, h, disk=(0.0, 30.0))
+    m = abi.metric_ks(1.0, 0.9)
+    print(f"[case] precision audit fraction {fraction} seed {seed:#x} {w}x{h}", flush=True)
+    ref, _ = acc.trace_batch([cam], [m], default_opts(abi.PRECISION_FP64))
+    o = default_opts(abi.PRECISION_FP32_ESCALATE)
+    o.precision_audit, o.audit_seed = fraction, seed
+    got, st = acc.trace_batch([cam], [m], o)
+    crit = (got.flags & abi.FLAG_CRITICAL) != 0
+    assert crit.sum() > 1000  # the band is skipped: the audit estimates the contract's error
+    sel = _audit_selection(w * h, fraction, seed) & ~crit
+    assert sel.sum() > 1000
+    assert st["precision_audited"] == int(sel.sum())
+    same = sel & (got.cls == ref.cls)
+    assert st["precision_class_mismatch"] == int((sel & ~same).sum())
+    esc = same & (ref.cls == abi.RAY_ESCAPED)
+    de = _chord_angle(ref.theta[esc], ref.phi[esc], got.theta[esc], got.phi[esc])
+    dk = same & (ref.cls == abi.RAY_DISK)
+    Ir, If = ref.intensity[:, dk].astype(np.float64), got.intensity[:, dk].astype(np.float64)
+    imax = Ir.max(0)
+    ie = np.abs(If - Ir).max(0)[imax > 0] / imax[imax > 0]
+    print(f"audited {st['precision_audited']}, mismatches {st['precision_class_mismatch']}, "
+          f"dir max {st['precision_dir_err_max']:.3e} (direct {de.max():.3e}), "
+          f"int max {st['precision_int_err_max']:.3e} (direct {ie.max():.3e}), "
+          f"medians {st['precision_dir_err_median']:.2e} {st['precision_int_err_median']:.2e}", flush=True)
+    assert esc.sum() > 30 and ie.size > 1000  # the view is mostly disk
+    assert st["precision_dir_err_max"] == pytest.approx(de.max(), rel=1e-5, abs=1e-15)
+    assert st["precision_int_err_max"] == pytest.approx(ie.max(), rel=1e-5, abs=1e-15)
+
+    def median_bin(e):
+        b = np.clip(((np.log10(np.maximum(e, 1e-300)) + 12.0) * 4.0).astype(int), 0, 63)
+        b[e <= 0] = 0
+        return np.sort(b)[(len(b) - 1) // 2]
+
+    for key, e in (("precision_dir_err_median", de), ("precision_int_err_median", ie)):
+        got_bin = (np.log10(st[key]) + 12.0) * 4.0 - 0.5
+        assert abs(got_bin - median_bin(e)) <= 1.0 + 1e-9, (key, st[key], got_bin, median_bin(e))
+    assert st["precision_audit_ms"] >= 0.0
+    # the FP32 results themselves are those of a frame without the audit
+    plain, _ = acc.trace_batch([cam], [m], default_opts(abi.PRECISION_FP32_ESCALATE))
+    assert np.array_equal(plain.cls, got.cls) and np.array_equal(plain.theta, got.theta)
+    assert np.array_equal(plain.flags, got.flags)
