@@ -384,7 +384,9 @@ int sirius_hip_surrogate_features(SiriusHipHandle h, const SiriusHipCamera* cams
  * random order, at learning rate lr with a cosine decay to lr/20; then the held-out rays
  * calibrate the error gate (label_opts' surrogate_pmin). *loss (host) receives the mean batch
  * loss of the last epoch. Uses the current parameters (sirius_hip_mlp_init first for a fresh
- * network). */
+ * network). epochs = 0 trains nothing and only calibrates the current parameters, receiving
+ * *loss = 0: the calibration of parameters restored by sirius_hip_mlp_set_params (which clears
+ * it), whose training cameras and seed give the same held-out rays again. */
 int sirius_hip_surrogate_train(SiriusHipHandle h, const SiriusHipCamera* cams, int n_cam,
                                const SiriusHipMetric* metrics, int n_metric,
                                const SiriusHipTraceOpts* label_opts, int epochs, int batch,

@@ -458,8 +458,8 @@ int sirius_hip_surrogate_train(SiriusHipHandle hh, const SiriusHipCamera* cams, 
     Handle* H = static_cast<Handle*>(hh);
     MLP_GUARD_BEGIN
     if (!H->mlp || !H->mlp->ready) return H->fail(SIRIUS_HIP_E_STATE, "network not initialised");
-    if (!label_opts || epochs <= 0 || batch <= 0 || !(lr > 0))
-        return H->fail(SIRIUS_HIP_E_INVALID, "label_opts, epochs > 0, batch > 0, lr > 0");
+    if (!label_opts || epochs < 0 || batch <= 0 || !(lr > 0))
+        return H->fail(SIRIUS_HIP_E_INVALID, "label_opts, epochs >= 0, batch > 0, lr > 0");
     if (label_opts->use_surrogate) return H->fail(SIRIUS_HIP_E_INVALID, "labels come from the integrator");
     int rc;
     Batch B;

@@ -546,11 +546,21 @@ SIRIUS_HD __forceinline__ int ray_attempt(Ray<T, FAM>& R, const CamConst* cams, 
 // Kernels
 // ------------------------------------------------------------------------------------------
 
-// Persistent tracer. Each wave pulls batches of work-list slots from *head (one atomic per
-// refill) and refills its idle lanes when at least refill_min of them are idle; slot -> ray
-// through ray_list when given. FP32 lanes that must escalate append their ray to esc_list
-// (count in *esc_count); each lane adds the attempts of every ray it traced to
-// *attempts_out when the wave retires.
+// Slots of the work list are reserved per wave from the queue head. One atomic on a single
+// address per refill stalls the refilling wave for the atomic's round trip, which under the
+// contention of a whole-chip grid costs more than the refill itself (1080p Kerr frame, FP32:
+// 1.33 ms at a refill threshold of 32, 2.56 ms at 2). A wave therefore reserves SLOT_CHUNK
+// slots at a time and serves its refills from them, until the head comes within SLOT_TAIL_WAVES
+// chunks per wave of the list's end; from there on it reserves exactly its idle lanes, so no
+// wave sits on unstarted rays while others have drained.
+constexpr int64_t SLOT_CHUNK = 128;
+constexpr int64_t SLOT_TAIL_WAVES = 2;
+static_assert(SLOT_CHUNK >= 64, "a reservation covers a whole wave's refill");
+
+// Persistent tracer. Each wave serves its refills from reserved work-list slots (above) and
+// refills its idle lanes when at least refill_min of them are idle; slot -> ray through ray_list
+// when given. FP32 lanes that must escalate append their ray to esc_list (count in *esc_count);
+// each lane adds the attempts of every ray it traced to *attempts_out when the wave retires.
 template <class T, int FAM, int EMIT, bool MULTI>
 __device__ __forceinline__ void persistent_body(const CamConst* __restrict__ cams, TraceParams tp, DevOut o,
                                                           const int32_t* __restrict__ ray_list,
@@ -569,21 +579,46 @@ __device__ __forceinline__ void persistent_body(const CamConst* __restrict__ cam
     Spacetime<T> s;
     if (!MULTI) s = make_spacetime<T>(cams[0]);
     bool active = false;
-    bool exhausted = false;
+    bool exhausted = false;    // no slot left for this wave: neither reserved nor in the queue
+    bool queue_done = false;   // the queue head has passed the list's end
+    // slots are 32-bit (work lists hold int32 ray ids); the wave's reserved slots [res_lo, res_hi)
+    uint32_t res_lo = 0, res_hi = 0, last_base = 0;
+    const int64_t tail_from = total - SLOT_TAIL_WAVES * SLOT_CHUNK * ((int64_t)gridDim.x * blockDim.x / 64);
+    const uint32_t tail_start = tail_from > 0 ? (uint32_t)tail_from : 0u;
+    const uint32_t total32 = (uint32_t)total;
     unsigned long long my_attempts = 0;  // attempts of every ray this lane traced
     while (true) {
         unsigned long long idle = __ballot(!active);
         int n_idle = __popcll(idle);
         if (n_idle == 64 && exhausted) break;
         if (!exhausted && (n_idle >= refill_min || n_idle == 64)) {
-            int leader = __ffsll((long long)idle) - 1;
+            // idle lane of rank r takes reserved slot res_lo + r while reserved ones last, then
+            // slot base + (r - from_old) of a new reservation
+            const uint32_t old_lo = res_lo;
+            const uint32_t ni = (uint32_t)n_idle;
+            const uint32_t from_old = res_hi - res_lo < ni ? res_hi - res_lo : ni;
+            const bool reserve = from_old < ni && !queue_done;
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(head, (uint32_t)n_idle);
-            base = __shfl(base, leader);
-            if ((int64_t)base + n_idle >= total) exhausted = true;
+            if (reserve) {
+                const uint32_t want = last_base < tail_start ? (uint32_t)SLOT_CHUNK : ni;
+                // every lane of the wave is here (the branch is wave-uniform), so lane 0 reserves
+                // and the base is read back as a scalar: the reservation state stays in SGPRs
+                uint32_t b = 0;
+                if (lane == 0) b = atomicAdd(head, want);
+                base = __builtin_amdgcn_readfirstlane(b);
+                last_base = base;
+                if ((uint64_t)base + want >= (uint64_t)total32) queue_done = true;
+                res_lo = base + (ni - from_old);
+                res_hi = base + want;
+            } else {
+                res_lo += from_old;
+            }
+            if (queue_done && (res_hi < total32 ? res_hi : total32) <= res_lo) exhausted = true;
             if (!active) {
                 unsigned long long below = lane ? (idle & ((1ull << lane) - 1ull)) : 0ull;
-                int64_t slot = (int64_t)base + __popcll(below);
+                const uint32_t r = (uint32_t)__popcll(below);
+                const int64_t slot = r < from_old ? (int64_t)old_lo + r
+                                                  : (reserve ? (int64_t)base + (r - from_old) : total);
                 if (slot < total) {
                     int64_t id = slot < first_total ? (int64_t)tp.first_list[slot]
                                                     : (ray_list ? (int64_t)ray_list[slot - first_total] : slot - first_total);

@@ -149,6 +149,28 @@ def test_persistent_queue_equals_grid_launch(acc):
         assert np.array_equal(c.cls, a.cls) and np.array_equal(c.intensity, a.intensity)
 
 
+def test_slot_reservation_traces_every_ray_once(acc):
+    """Waves reserve work-list slots in chunks of 128 while the queue head is far from the list's
+    end, and exactly their idle lanes near it. On a small grid (one 64-lane wave per CU at the
+    first shape) most of a 640x360 frame is served from chunks. Every ray is traced exactly once
+    under every shape: outputs and the total attempt count equal the grid schedule's."""
+    m = abi.metric_ks(1.0, 0.9)
+    cam = abi.camera(50.0, 75.0, 20.0, 640, 360, disk=(0.0, 30.0))
+    o = O.default_opts(abi.PRECISION_FP32_ESCALATE)
+    o.escalate_margin = o.escalate_band = o.escalate_disk_edge = 0.0
+    o.schedule = abi.SCHEDULE_GRID
+    print("[case] slot reservation 640x360, grid schedule", flush=True)
+    ref, sref = acc.trace_batch([cam], [m], o)
+    o.schedule = abi.SCHEDULE_PERSISTENT
+    for bs, bpc, rf in ((64, 1, 1), (64, 1, 8), (256, 1, 16), (128, 2, 32), (64, 4, 64), (256, 0, 16)):
+        o.block_size, o.blocks_per_cu, o.refill_min = bs, bpc, rf
+        print(f"[case] slot reservation 640x360, block {bs} x {bpc} per CU, refill {rf}", flush=True)
+        got, st = acc.trace_batch([cam], [m], o)
+        assert st["steps_bulk"] == sref["steps_bulk"], (bs, bpc, rf)
+        for k in ("cls", "theta", "phi", "lam", "disk_r", "steps", "intensity"):
+            assert np.array_equal(getattr(got, k), getattr(ref, k)), (k, bs, bpc, rf)
+
+
 def test_escalated_rays_are_bitwise_fp64(acc):
     """Escalated rays (the gate's critical band, in-flight turning points inside the photon
     region, disk crossings at the inner edge) are traced by the FP64 persistent kernel with the

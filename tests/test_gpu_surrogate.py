@@ -162,6 +162,20 @@ def test_surrogate_c4_training_decision_audit_and_ablation(acc):
         assert np.array_equal(getattr(got, k)[..., ~sur], getattr(base, k)[..., ~sur]), k
     assert np.array_equal(got.flags[~sur], base.flags[~sur])
 
+    # a restored network: new parameters clear the calibration; a calibration-only pass (0 epochs)
+    # over the training cameras with the training seed rebuilds it from the same held-out rays
+    p = acc.mlp_get_params()
+    acc.mlp_set_params(p)
+    assert not _calibration(acc)[1].any()
+    print("[case] calibration-only pass of the restored network", flush=True)
+    assert acc.surrogate_train(cams, metrics, lab, 0, 8192, 2e-3, 0x51121006) == 0.0
+    err2, n2, bad2 = _calibration(acc)
+    assert np.array_equal(n2, n) and np.array_equal(bad2, bad) and np.array_equal(err2, err)
+    assert np.array_equal(acc.mlp_get_params(), p)
+    g2, _ = acc.trace_batch(ecams, emets, on)
+    for k in ("cls", "theta", "phi", "flags"):
+        assert np.array_equal(getattr(g2, k), getattr(got, k)), k
+
 
 def test_uncalibrated_network_answers_nothing(acc):
     acc.mlp_init(7)
