@@ -17,6 +17,9 @@ One step = one full frame traced on the GPU with every result left in device mem
                  and (reported apart) inside it; per-pixel and per-bin intensity errors
   with_edge_escalation
                  the same frame with inner-edge disk crossings re-traced in FP64
+  with_band_escalation
+                 the same frame with the rays within 0.5 M of the critical curve traced in FP64
+                 on a second stream, concurrently with the FP32 tracer
   roofline       dominant kernel (the FP32 persistent tracer): algorithmic flops per launch
                  (1,326 per Kerr-Schild DP45 attempt) over its HIP-event time, against the
                  FP32 vector peak; traffic from the profile's HBM byte counters
@@ -50,7 +53,7 @@ FP32_VALU_PEAK_TFLOPS = 157.3
 FP64_VALU_PEAK_TFLOPS = 78.6
 # HBM bytes per launch of the FP32 bulk tracer on this frame, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a copy kernel of known size (profiles/trace_kerr_a0.9_1080p_counters.md)
-BULK_HBM_BYTES = 232.0e6
+BULK_HBM_BYTES = 222.3e6
 W, H = 1920, 1080
 
 
@@ -590,6 +593,17 @@ def main():
         result["with_edge_escalation"] = {
             "value": n_rays / mse / 1e6, "ms_per_step": mse * 1e3, "rays_escalated": ste[-1]["rays_escalated"],
             "escalate_ms": float(np.mean([x["escalate_ms"] for x in ste])),
+            "accuracy_vs_fp64": accuracy(acc.fetch_results(), ref, crit)}
+        # the same frame with the critical band traced in FP64: rays within 0.5 M of the critical
+        # curve go to the FP64 tracer on a second stream while the FP32 tracer runs the rest
+        ob = default_opts(abi.PRECISION_FP32_ESCALATE)
+        ob.block_size, ob.blocks_per_cu, ob.refill_min = o32.block_size, o32.blocks_per_cu, o32.refill_min
+        ob.escalate_band = ob.critical_band
+        msb, stb = time_frames(acc, cam, metric, ob, args.steps, args.warmup)
+        result["with_band_escalation"] = {
+            "value": n_rays / msb / 1e6, "ms_per_step": msb * 1e3, "escalate_band_m": ob.escalate_band,
+            "rays_escalated": stb[-1]["rays_escalated"],
+            "escalate_ms": float(np.mean([x["escalate_ms"] for x in stb])),
             "accuracy_vs_fp64": accuracy(acc.fetch_results(), ref, crit)}
     if not args.no_extra:
         result["other_workloads"] = other_workloads(acc, args.steps, args.warmup)
