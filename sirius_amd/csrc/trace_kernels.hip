@@ -13,6 +13,8 @@
 //   k_symplectic_grid     fixed-step Tao/Yoshida integrator
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "geodesic.hpp"
 #include "../../include/sirius_hip.h"
 
@@ -724,69 +726,117 @@ __global__ void k_sur_audit(const CamConst* __restrict__ cams, TraceParams tp, D
 
 // Selection of the precision audit: rays of the batch with hash(seed ^ id) below the fraction's
 // threshold that the FP32 path traced (not answered by the surrogate) outside the critical band
-// (the rays the accuracy contract covers), into list (count *count).
-__global__ void k_precision_select(const DevOut o, int64_t n, uint64_t thr, uint64_t seed, int32_t* __restrict__ list,
-                                   uint32_t* __restrict__ count) {
-    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool take = false;
-    if (id < n) {
-        uint64_t z = (seed ^ (uint64_t)id) + 0x9E3779B97F4A7C15ULL;
-        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-        z ^= z >> 31;
-        take = z <= thr && !(o.flags[id] & (SIRIUS_FLAG_SURROGATE | SIRIUS_FLAG_CRITICAL));
-    }
-    const unsigned long long m = __ballot(take);
+// (the rays the accuracy contract covers), into list (count *count). Each wave takes a span of
+// 64 x PREC_SELECT_SPAN consecutive rays: it counts its picks, reserves their places with one
+// atomic (a single-address atomic per wave of 64 rays cost more than the hashing), then writes
+// them in ray order.
+constexpr int PREC_SELECT_SPAN = 32;
+__device__ __forceinline__ bool precision_pick(const DevOut& o, int64_t n, uint64_t thr, uint64_t seed, int64_t id) {
+    if (id >= n) return false;
+    uint64_t z = (seed ^ (uint64_t)id) + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    return z <= thr && !(o.flags[id] & (SIRIUS_FLAG_SURROGATE | SIRIUS_FLAG_CRITICAL));
+}
+__global__ __launch_bounds__(256) void k_precision_select(const DevOut o, int64_t n, uint64_t thr, uint64_t seed,
+                                                          int32_t* __restrict__ list, uint32_t* __restrict__ count) {
     const int lane = __lane_id();
+    const int64_t first = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64 * PREC_SELECT_SPAN;
+    if (first >= n) return;  // whole waves
+    uint32_t total = 0;
+    for (int k = 0; k < PREC_SELECT_SPAN; ++k)
+        total += (uint32_t)__popcll(__ballot(precision_pick(o, n, thr, seed, first + 64 * k + lane)));
     uint32_t base = 0;
-    if (lane == 0 && m) base = atomicAdd(count, (uint32_t)__popcll(m));
+    if (lane == 0 && total) base = atomicAdd(count, total);
     base = __shfl(base, 0);
-    if (take) list[base + __popcll(m & (lane ? ((1ull << lane) - 1ull) : 0ull))] = (int32_t)id;
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int k = 0; k < PREC_SELECT_SPAN; ++k) {
+        const int64_t id = first + 64 * k + lane;
+        const bool take = precision_pick(o, n, thr, seed, id);
+        const unsigned long long m = __ballot(take);
+        if (take) list[base + __popcll(m & below)] = (int32_t)id;
+        base += (uint32_t)__popcll(m);
+    }
+}
+
+// Largest value over the wave (non-negative floats compare as their bit patterns).
+__device__ __forceinline__ unsigned int wave_max_u32(unsigned int v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned int w = (unsigned int)__shfl_xor((int)v, off);
+        v = w > v ? w : v;
+    }
+    return v;
 }
 
 // Comparison of the precision audit: the FP64 re-traces (oa) against the batch's results (o).
 // acc[0] class mismatches, acc[1] audited rays, acc[2] largest direction error (float bits),
 // acc[3] largest relative intensity error (float bits); hist[0..63] and hist[64..127] bin the
 // direction and intensity errors by log10, 1/4 decade per bin from 1e-12, for their medians.
-__global__ void k_precision_compare(const CamConst* __restrict__ cams, TraceParams tp, DevOut oa, DevOut o,
-                                    const int32_t* __restrict__ list, const uint32_t* __restrict__ count, int64_t cap,
-                                    unsigned int* __restrict__ acc, unsigned int* __restrict__ hist) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// A small grid strides over the list (its length is known only on the device). Each block
+// bins in LDS and adds its non-zero bins to hist; counts and maxima are reduced over each wave:
+// the errors crowd into a few bins, where one global atomic per ray serialised.
+constexpr int PREC_COMPARE_BLOCKS = 1024;
+__global__ __launch_bounds__(256) void k_precision_compare(const CamConst* __restrict__ cams, TraceParams tp, DevOut oa,
+                                                           DevOut o, const int32_t* __restrict__ list,
+                                                           const uint32_t* __restrict__ count, int64_t cap,
+                                                           unsigned int* __restrict__ acc, unsigned int* __restrict__ hist) {
+    __shared__ unsigned int sh[128];
+    for (int k = threadIdx.x; k < 128; k += blockDim.x) sh[k] = 0u;
+    __syncthreads();
     const int64_t lim = min((int64_t)*count, cap);
-    if (i >= lim) return;
-    const int64_t id = list[i];
-    const int cam = tp.n_cam == 1 ? 0 : find_camera(cams, tp.n_cam, id);
-    if (cams[cam].family == 1) ray_outputs<1, SIRIUS_EMISSION_NONE>(cams[cam], tp, oa, id);
-    else if (cams[cam].family == 2) ray_outputs<2, SIRIUS_EMISSION_NONE>(cams[cam], tp, oa, id);
-    else if (tp.emission == SIRIUS_EMISSION_THIN_DISK) ray_outputs<0, SIRIUS_EMISSION_THIN_DISK>(cams[cam], tp, oa, id);
-    else ray_outputs<0, SIRIUS_EMISSION_NONE>(cams[cam], tp, oa, id);
-    atomicAdd(&acc[1], 1u);
-    const int ca = oa.cls[id], cs = o.cls[id];
-    if (ca != cs) {
-        atomicAdd(&acc[0], 1u);
-        return;
-    }
+    unsigned int n_valid = 0u, n_mis = 0u, dir_bits = 0u, int_bits = 0u;
     auto bin = [](double e) {
         int b = e > 0.0 ? (int)((log10(e) + 12.0) * 4.0) : 0;
         return b < 0 ? 0 : (b > 63 ? 63 : b);
     };
-    if (ca == SIRIUS_RAY_ESCAPED) {
-        const double err = dir_angle(oa.theta[id], oa.phi[id], o.theta[id], o.phi[id]);
-        atomicMax(&acc[2], __float_as_uint((float)err));
-        atomicAdd(&hist[bin(err)], 1u);
-    } else if (ca == SIRIUS_RAY_DISK && tp.emission == SIRIUS_EMISSION_THIN_DISK && tp.n_bins > 0) {
-        double imax = 0.0, dmax = 0.0;
-        for (int b = 0; b < tp.n_bins; ++b) {
-            const double ia = oa.intensity[(int64_t)b * oa.n_rays + id], is = o.intensity[(int64_t)b * o.n_rays + id];
-            imax = fmax(imax, ia);
-            dmax = fmax(dmax, fabs(is - ia));
-        }
-        if (imax > 0.0) {
-            const double rel = dmax / imax;
-            atomicMax(&acc[3], __float_as_uint((float)rel));
-            atomicAdd(&hist[64 + bin(rel)], 1u);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < lim; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t id = list[i];
+        const int cam = tp.n_cam == 1 ? 0 : find_camera(cams, tp.n_cam, id);
+        if (cams[cam].family == 1) ray_outputs<1, SIRIUS_EMISSION_NONE>(cams[cam], tp, oa, id);
+        else if (cams[cam].family == 2) ray_outputs<2, SIRIUS_EMISSION_NONE>(cams[cam], tp, oa, id);
+        else if (tp.emission == SIRIUS_EMISSION_THIN_DISK) ray_outputs<0, SIRIUS_EMISSION_THIN_DISK>(cams[cam], tp, oa, id);
+        else ray_outputs<0, SIRIUS_EMISSION_NONE>(cams[cam], tp, oa, id);
+        ++n_valid;
+        const int ca = oa.cls[id], cs = o.cls[id];
+        if (ca != cs) {
+            ++n_mis;
+        } else if (ca == SIRIUS_RAY_ESCAPED) {
+            const double err = dir_angle(oa.theta[id], oa.phi[id], o.theta[id], o.phi[id]);
+            const unsigned int eb = __float_as_uint((float)err);
+            dir_bits = eb > dir_bits ? eb : dir_bits;
+            atomicAdd(&sh[bin(err)], 1u);
+        } else if (ca == SIRIUS_RAY_DISK && tp.emission == SIRIUS_EMISSION_THIN_DISK && tp.n_bins > 0) {
+            double imax = 0.0, dmax = 0.0;
+            for (int b = 0; b < tp.n_bins; ++b) {
+                const double ia = oa.intensity[(int64_t)b * oa.n_rays + id], is = o.intensity[(int64_t)b * o.n_rays + id];
+                imax = fmax(imax, ia);
+                dmax = fmax(dmax, fabs(is - ia));
+            }
+            if (imax > 0.0) {
+                const double rel = dmax / imax;
+                const unsigned int rb = __float_as_uint((float)rel);
+                int_bits = rb > int_bits ? rb : int_bits;
+                atomicAdd(&sh[64 + bin(rel)], 1u);
+            }
         }
     }
+    // every lane of the block is here: reduce over the wave, then flush the block's bins
+    for (int off = 32; off > 0; off >>= 1) {
+        n_valid += (unsigned int)__shfl_xor((int)n_valid, off);
+        n_mis += (unsigned int)__shfl_xor((int)n_mis, off);
+    }
+    dir_bits = wave_max_u32(dir_bits);
+    int_bits = wave_max_u32(int_bits);
+    if (__lane_id() == 0) {
+        if (n_valid) atomicAdd(&acc[1], n_valid);
+        if (n_mis) atomicAdd(&acc[0], n_mis);
+        if (dir_bits) atomicMax(&acc[2], dir_bits);
+        if (int_bits) atomicMax(&acc[3], int_bits);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 128; k += blockDim.x)
+        if (sh[k]) atomicAdd(&hist[k], sh[k]);
 }
 
 // Fill a small device array (graph-safe replacement for a memset).
@@ -996,15 +1046,17 @@ hipError_t launch_symplectic(bool fp64, int family, const CamConst* cams, const 
 hipError_t launch_precision_select(const DevOut& o, int64_t n, double fraction, uint64_t seed, int32_t* list,
                                    uint32_t* count, hipStream_t st) {
     const uint64_t thr = fraction >= 1.0 ? ~0ull : (uint64_t)(fraction * 18446744073709551615.0);
-    hipLaunchKernelGGL(k_precision_select, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, o, n, thr, seed, list,
-                       count);
+    const int64_t per_block = 4 * 64 * (int64_t)PREC_SELECT_SPAN;  // four waves of 256 threads
+    hipLaunchKernelGGL(k_precision_select, dim3((unsigned)((n + per_block - 1) / per_block)), dim3(256), 0, st, o, n,
+                       thr, seed, list, count);
     return hipGetLastError();
 }
 hipError_t launch_precision_compare(const CamConst* cams, const TraceParams& tp, const DevOut& oa, const DevOut& o,
                                     const int32_t* list, const uint32_t* count, int64_t cap, unsigned int* acc,
                                     unsigned int* hist, hipStream_t st) {
-    hipLaunchKernelGGL(k_precision_compare, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, cams, tp, oa, o,
-                       list, count, cap, acc, hist);
+    const int64_t blocks = std::min<int64_t>((cap + 255) / 256, PREC_COMPARE_BLOCKS);
+    hipLaunchKernelGGL(k_precision_compare, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0, st, cams, tp,
+                       oa, o, list, count, cap, acc, hist);
     return hipGetLastError();
 }
 
