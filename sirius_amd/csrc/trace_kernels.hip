@@ -661,9 +661,14 @@ __device__ __forceinline__ void persistent_body(const CamConst* __restrict__ cam
 
 // FP64 variants: register allocation left to the compiler (256 VGPRs plus 6-156 AGPRs, one wave
 // per SIMD). Held to the 256 registers of two waves, the thin-disk tracer spills 34 to scratch
-// and the 1920x1080 Kerr frame slows from 11.3 to 14.9 ms.
+// and the 1920x1080 Kerr frame slows from 11.3 to 14.9 ms. The single-camera warp tracer, which
+// re-traces wall crossers (a tenth of a warp frame, too few rays to hide one wave's latency), is
+// held to two waves: 268 -> 256 registers with 6 spilled, and the 1024x1024 warp frame goes from
+// 2.79 to 2.54 ms.
+constexpr int fp64_min_waves(int fam, bool multi) { return fam == 2 && !multi ? 2 : 1; }
 template <class T, int FAM, int EMIT, bool MULTI>
-__global__ __launch_bounds__(256) void k_trace_persistent(SIRIUS_PERSISTENT_PARAMS) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fp64_min_waves(FAM, MULTI))))
+void k_trace_persistent(SIRIUS_PERSISTENT_PARAMS) {
     persistent_body<T, FAM, EMIT, MULTI>(SIRIUS_PERSISTENT_ARGS);
 }
 
