@@ -192,6 +192,7 @@ def other_workloads(acc, steps, warmup):
         o = default_opts(abi.PRECISION_FP32_ESCALATE)
         o.emission = abi.EMISSION_NONE
         out[name] = measure([cam], [metric], o, steps, warmup)
+    out["wormhole_1024_two_spheres"] = wormhole_spheres(acc, steps)
     out["image_writers_1080p"] = image_writers(acc, steps)
     out["ray_bundle_1080p"] = ray_bundle(acc, steps)
     return out
@@ -233,6 +234,39 @@ def ray_bundle(acc, steps):
         acc.set_ray_bundle(False)
         acc.set_starfield(0, None, None, 0)
     return out
+
+
+def wormhole_spheres(acc, steps):
+    """The wormhole scene at 1024x1024 rendered through launch with its two celestial spheres:
+    4096 stars each, magnitudes ~ Exp(1), seeds 0x51121008 (near side) and 0x51121009 (far side)
+    (tests/workloads.py): the trace, the star light of every escaped pixel from its side's
+    sphere, and the framebuffer."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from workloads import WORMHOLE_SPHERE_SEEDS, sphere_catalogue, wormhole_scene
+    from sirius_amd import abi
+    from sirius_amd.accelerator import default_opts, starfield_default
+    cam, metric = wormhole_scene(1024)
+    o = default_opts(abi.PRECISION_FP32_ESCALATE)
+    o.emission = abi.EMISSION_NONE
+    cfg = starfield_default()
+    acc.initialise(1024, 1024)
+    try:
+        for side, seed in enumerate(WORMHOLE_SPHERE_SEEDS):
+            rows = sphere_catalogue(seed)
+            stars = (abi.Star * len(rows))()
+            np.frombuffer(stars, np.float32).reshape(-1, 8)[:] = rows
+            acc.set_starfield(side, cfg, stars, len(rows))
+        acc.launch(cam, metric, o)
+        ts = []
+        for _ in range(max(2, steps // 2)):
+            t0 = time.perf_counter()
+            acc.launch(cam, metric, o)
+            ts.append(time.perf_counter() - t0)
+        ms = float(np.mean(ts)) * 1e3
+        return {"launch_ms": ms, "mrays_per_s": 1024 * 1024 / ms / 1e3, "stars_per_side": 4096}
+    finally:
+        acc.set_starfield(0, None, None, 0)
+        acc.set_starfield(1, None, None, 0)
 
 
 # Algorithmic HBM bytes per pixel of each format's device conversion: the float RGBA frame read

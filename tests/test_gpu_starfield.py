@@ -16,7 +16,7 @@ import pytest
 from oracle import oracle_py as O
 from sirius_amd import abi
 from sirius_amd.accelerator import starfield_default, starfield_generate
-from workloads import wormhole_scene
+from workloads import WORMHOLE_SPHERE_SEEDS, sphere_catalogue, wormhole_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -54,7 +54,11 @@ def test_device_sampler_matches_reference(acc, golden, key, cat, kw):
     assert not acc.sample_starfield(0, d["dirs"][:8]).any()
 
 
-def test_wormhole_spheres_shade_escaped_rays(acc):
+@pytest.mark.parametrize("kind", ["generator", "exp1"])
+def test_wormhole_spheres_shade_escaped_rays(acc, kind):
+    """kind "generator": the reference generator's catalogues (seeds 42 and 43); "exp1": the
+    scene's own spheres, 4096 stars each with magnitudes ~ Exp(1) (seeds 0x51121008 and
+    0x51121009, tests/workloads.py)."""
     cam, m = wormhole_scene(96)
     w = h = 96
     o = O.default_opts(abi.PRECISION_FP32_ESCALATE)
@@ -63,9 +67,14 @@ def test_wormhole_spheres_shade_escaped_rays(acc):
     # a black sky (the session's handle may hold an earlier test's background): escaped pixels
     # then start at exactly 0, and star light of ~1e-7 is not lost against a bright background
     acc.uploadBackground(np.zeros((1, 1, 4), np.uint8))
-    near, n_near = starfield_generate(_cfg(42))
-    far, n_far = starfield_generate(_cfg(43))
-    print("[case] wormhole launch with two spheres", flush=True)
+    if kind == "generator":
+        near, n_near = starfield_generate(_cfg(42))
+        far, n_far = starfield_generate(_cfg(43))
+    else:
+        rows = [sphere_catalogue(seed) for seed in WORMHOLE_SPHERE_SEEDS]
+        near, n_near = _stars(rows[0]), len(rows[0])
+        far, n_far = _stars(rows[1]), len(rows[1])
+    print(f"[case] wormhole launch with two spheres ({kind})", flush=True)
     acc.launch(cam, m, o)
     base = acc.getFrameBuffer().reshape(-1, 4)[:, :3].copy()
     acc.set_starfield(0, _cfg(42), near, n_near)

@@ -1,5 +1,5 @@
-"""Synthetic workloads shared by the tests: the 64-camera random spacetime batch and the
-Morris-Thorne / Alcubierre scenes.
+"""Synthetic workloads shared by the tests: the 64-camera random spacetime batch, the
+Morris-Thorne / Alcubierre scenes and the wormhole's two celestial-sphere catalogues.
 
 The random batch draws, for each of 64 cameras in turn, a ~ U[0, 0.99],
 Q ~ U[0, 0.5 sqrt(1 - a^2)], Lambda ~ U[0, 1e-4], inclination ~ U[5, 89] deg,
@@ -81,3 +81,28 @@ def wormhole_scene(side):
 def warp_scene(side):
     """Static Alcubierre bubble v = 0.5, R = 1, sigma = 8, observer 10 units off-axis."""
     return abi.camera(10.0, 60.0, 40.0, side, side, phi_deg=30.0), abi.metric_warp(0.5, 1.0, 8.0)
+
+
+def sphere_catalogue(seed, n=4096):
+    """A celestial sphere of n stars (rows of SiriusHipStar: direction x, y, z, distance (pc),
+    apparent magnitude, B-V colour index, temperature (K), unused), drawn from an mt19937_64
+    seeded with `seed`: per star, a direction uniform on the sphere (z ~ U[-1, 1],
+    phi ~ U[0, 2 pi)), a magnitude ~ Exp(1) (-ln(1 - u)), a distance ~ U[10, 1000] pc and a B-V
+    index ~ U[-0.3, 1.7] with the temperature the reference's generator gives that index
+    (4600 K / (0.92 + B-V), PHSF001A.h:51-73). The two spheres of the wormhole scene use seeds
+    0x51121008 (near side) and 0x51121009 (far side)."""
+    g = MT19937_64(seed)
+    rows = np.zeros((n, 8), np.float32)
+    for i in range(n):
+        z = g.uniform(-1.0, 1.0)
+        ph = g.uniform(0.0, 2.0 * np.pi)
+        mag = -np.log(1.0 - g.uniform(0.0, 1.0))
+        dist = g.uniform(10.0, 1000.0)
+        bv = g.uniform(-0.3, 1.7)
+        rho = np.sqrt(max(0.0, 1.0 - z * z))
+        temp = 4600.0 / (0.92 + bv) if bv > -0.1 else 30000.0 + 10000.0 * (-bv - 0.1)
+        rows[i] = (rho * np.cos(ph), rho * np.sin(ph), z, dist, mag, bv, min(max(temp, 2500.0), 50000.0), 0.0)
+    return rows
+
+
+WORMHOLE_SPHERE_SEEDS = (0x51121008, 0x51121009)
