@@ -670,8 +670,7 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
             int32_t* ll = (int32_t*)H.scratch("gate_l", 4 * (size_t)B.n, &rc);
             if (rc) return rc;
             uint32_t* counts = work + 16;
-            // rays within 1 M of the critical curve are queued first (longest first)
-            const double long_band = 1.0;
+            const double long_band = SIRIUS_LONG_BAND;  // the longest rays first
             if ((e = launch_gate(B.cams_dev, (int)B.cc.size(), tabs, B.n, std::max(o.escalate_band, 0.0),
                                  std::max(o.critical_band, 0.0), long_band, la, list_b, ll, counts, dout.flags,
                                  H.stream)) != hipSuccess)

@@ -15,6 +15,13 @@
 
 namespace sirius {
 
+// The gates queue rays within this distance (M) of the critical curve, the longest, first, so
+// that they all start in the persistent grid's first fill. The 1080p Kerr a = 0.9 frame's tracer
+// by the band's width (A/B on one box, ms per frame): none 1.87, 0.25 M 1.34, 0.5 M 1.28,
+// 1 M 1.31, 2 M 1.56, 4 M 1.66. A band wider than one fill (5.8 % of that frame lies within
+// 0.5 M) queues moderately long rays ahead of the longest.
+constexpr double SIRIUS_LONG_BAND = 0.5;
+
 struct DevOut;
 
 struct Buf {

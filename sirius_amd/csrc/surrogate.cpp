@@ -303,7 +303,7 @@ int surrogate_stage(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& do
     MlpState& S = *H.mlp;
     hipError_t e;
     if ((e = launch_sur_gate(B.cams_dev, (int)B.cc.size(), tabs, n, std::max(o.escalate_band, 0.0),
-                             std::max(o.critical_band, 0.0), 1.0, o.surrogate_band, o.surrogate_margin, o.r_escape,
+                             std::max(o.critical_band, 0.0), SIRIUS_LONG_BAND, o.surrogate_band, o.surrogate_margin, o.r_escape,
                              list_a, list_b, list_l, list_s, work + 16, dout.flags, rec, S.cal_err, S.cal_n, S.cal_bad,
                              S.cal_dmax, o.surrogate_err_max, S.sup_x, S.sup_y, H.stream)) != hipSuccess)
         return H.hip_fail(e, "surrogate gate");
