@@ -50,6 +50,7 @@ struct CamConst {
     double r_ph_retro;               // retrograde circular photon orbit
     double pt_x0, pt_x[3], pt_c[3];  // Page-Thorne closed-form constants
     double l_obs;                    // wormhole: observer's proper-length coordinate
+    double g_obs[16];                // g_{mu nu} at the observer (every ray's initial position)
 };
 
 struct TraceParams {
@@ -874,7 +875,11 @@ SIRIUS_HD __forceinline__ void init_state_ns(const CamConst& c, const Spacetime<
             k[i + 1] = ns[0] * T(c.er[i]) + ns[1] * T(c.et[i]) + ns[2] * T(c.ep[i]);  // unit Cartesian direction
         }
     }
-    metric_cov(s, x, g);
+    // the metric at the observer is the camera's: computed once on the host in FP64
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
+        for (int nu = 0; nu < 4; ++nu) g[mu][nu] = T(c.g_obs[4 * mu + nu]);
     if (c.family != 1) k[0] = null_k0(g, k);
     for (int mu = 0; mu < 4; ++mu) {
         T acc = T(0);

@@ -208,6 +208,17 @@ bool make_cam_const(const SiriusHipCamera& cam, const SiriusHipMetric& m, int64_
     double gtt = gtt_at(m, x);
     if (!(gtt < 0)) { err = "observer is not static (g_tt >= 0 at the camera)"; return false; }
     c.obs_norm = 1.0 / std::sqrt(-gtt);
+    // the metric at every ray's initial position, for the devices' initial states
+    {
+        double xo[4] = {0.0, 0.0, 0.0, 0.0};
+        if (c.family == 1) xo[1] = c.l_obs;
+        else
+            for (int i = 0; i < 3; ++i) xo[i + 1] = c.r_obs * c.er[i];
+        double g[4][4];
+        metric_cov(make_spacetime<double>(c), xo, g);
+        for (int mu = 0; mu < 4; ++mu)
+            for (int nu = 0; nu < 4; ++nu) c.g_obs[4 * mu + nu] = g[mu][nu];
+    }
     return true;
 }
 
