@@ -284,7 +284,7 @@ int sirius_hip_critical_distance(SiriusHipHandle hh, const SiriusHipCamera* cams
 }
 
 // Autotuner: times the FP32 persistent trace over block sizes {64, 128, 256} x refill threshold
-// {1, 8, 16, 32}, each at full occupancy (blocks per CU from the occupancy API), and keeps the
+// {8, 16, 24, 32}, each at full occupancy (blocks per CU from the occupancy API), and keeps the
 // fastest. Batches of up to 4 M rays are timed whole; larger ones on a strided sample (every
 // k-th ray, about 1/16 of the batch, at least 64k rays).
 int sirius_hip_autotune(SiriusHipHandle hh, const SiriusHipCamera* cams, int n_cam, const SiriusHipMetric* metrics,
@@ -317,7 +317,7 @@ int sirius_hip_autotune(SiriusHipHandle hh, const SiriusHipCamera* cams, int n_c
     double best = 1e300;
     SiriusHipTraceOpts bo = *opts;
     const int blocks[3] = {64, 128, 256};
-    const int refills[4] = {1, 8, 16, 32};
+    const int refills[4] = {8, 16, 24, 32};  // 1 and 4 never won (1080p Kerr frame: 1.3-1.5 ms against 1.1)
     // every block size at full occupancy: on a sample, lower occupancies can time faster (its
     // short trace has little latency to hide) while on whole batches they ran up to 1.7x slower
     for (int bs : blocks) {
