@@ -132,8 +132,8 @@ typedef struct SiriusHipTraceOpts {
     int32_t reserved0;
     /* volumetric emission (emitter frame): j_nu = rho(x) x^3 / (e^{x / tau_v(r)} - 1),
      * rho = j0 r^-2 exp(-z^2 / (2 (h_over_r r)^2)) W(r), tau_v = (6 / r)^{3/4},
-     * alpha = kappa rho; W is a C1 smoothstep window, 0 at r_lo and vol_r_max and 1 on
-     * [1.1 r_lo, 0.9 vol_r_max], r_lo = max(vol_r_min, the camera's disk r_in, by default the
+     * alpha = kappa rho; W is a C2 (quintic) smoothstep window, 0 at r_lo and vol_r_max and 1
+     * on [1.1 r_lo, 0.9 vol_r_max], r_lo = max(vol_r_min, the camera's disk r_in, by default the
      * ISCO); emitters are on Keplerian (Kerr-Newman) circles */
     double vol_j0, vol_h_over_r, vol_kappa, vol_r_min, vol_r_max, vol_tau_max;
     double escalate_disk_edge; /* FP32 path, thin disk: a disk crossing whose radius lies within

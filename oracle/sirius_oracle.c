@@ -768,13 +768,13 @@ static double vol_step_cap(const SiriusHipMetric* m, const SiriusHipTraceOpts* o
     return c > 0.2 * sig ? c : 0.2 * sig;
 }
 
-/* Smooth (C1) radial window of the emitting shell: 0 at r_min and r_max, 1 on
- * [1.1 r_min, 0.9 r_max], cubic smoothstep between. */
+/* Smooth (C2) radial window of the emitting shell: 0 at r_min and r_max, 1 on
+ * [1.1 r_min, 0.9 r_max], quintic smoothstep 6u^5 - 15u^4 + 10u^3 between. */
 static double vol_window(double r, double r_min, double r_max) {
     double u = (r - r_min) / (0.1 * r_min), v = (r_max - r) / (0.1 * r_max);
     u = u < 0 ? 0 : (u > 1 ? 1 : u);
     v = v < 0 ? 0 : (v > 1 ? 1 : v);
-    return u * u * (3 - 2 * u) * v * v * (3 - 2 * v);
+    return u * u * u * (u * (6 * u - 15) + 10) * v * v * v * (v * (6 * v - 15) + 10);
 }
 
 /* Volumetric transfer over one accepted step, 3-node Gauss-Legendre on the Hermite

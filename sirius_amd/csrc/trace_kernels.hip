@@ -233,15 +233,18 @@ SIRIUS_HD __forceinline__ void ray_outputs(const CamConst& c, const TraceParams&
 // Volumetric transfer over an accepted step (3-node Gauss-Legendre on the cubic Hermite
 // interpolant of (x, p)). Emitter-frame j_nu = rho S(x_e), alpha = kappa rho, invariant form:
 //   dtau = kappa rho nu_e dlam,  dI_k = rho S(x_k / g) (nu_o^3 / nu_e^2) e^{-tau} dlam.
-// Smooth (C1) radial window of the emitting shell: zero at r_min and r_max, one between
-// 1.1 r_min and 0.9 r_max (cubic smoothstep), so the emissivity has no jump for the per-step
-// quadrature to straddle.
+// Smooth (C2) radial window of the emitting shell: zero at r_min and r_max, one between
+// 1.1 r_min and 0.9 r_max (quintic smoothstep 6u^5 - 15u^4 + 10u^3), so the emissivity has no
+// jump, and no jump in its first two derivatives, for the per-step quadrature to straddle. With
+// the cubic (C1) smoothstep the quadrature's error at the shell's edges set how long a step
+// could be: on the 64-camera batch, steps of 0.3 sigma left 1.9e-3 of intensity error against
+// the FP64 reference with the cubic window, 6.0e-4 with this one.
 template <class T>
 SIRIUS_HD __forceinline__ T vol_window(T r, T r_min, T r_max) {
     T u = (r - r_min) / (T(0.1) * r_min), v = (r_max - r) / (T(0.1) * r_max);
     u = u < T(0) ? T(0) : (u > T(1) ? T(1) : u);
     v = v < T(0) ? T(0) : (v > T(1) ? T(1) : v);
-    return u * u * (T(3) - T(2) * u) * v * v * (T(3) - T(2) * v);
+    return u * u * u * (u * (T(6) * u - T(15)) + T(10)) * v * v * v * (v * (T(6) * v - T(15)) + T(10));
 }
 
 template <class T, int FAM>
@@ -311,7 +314,11 @@ SIRIUS_HD __forceinline__ T vol_step_cap(const Spacetime<T>& s, const TraceParam
     if (r < T(0.5 * tp.vol_r_min) || r > T(1.5 * tp.vol_r_max)) return T(1e30);
     T sig = T(tp.vol_h_over_r) * r;
     T cap = T(0.5) * (fabs(x[3]) - T(3) * sig);
-    return cap > T(0.2) * sig ? cap : T(0.2) * sig;
+    // inside the layer at most 0.2 sigma per step (FP64, the oracle's), or 0.3 sigma on the FP32
+    // path, whose intensity tolerance is 1e-3: 0.3 sigma keeps the 64-camera batch within
+    // 6.0e-4 of the FP64 reference and takes 95 instead of 118 steps per ray
+    const T inner = T(sizeof(T) == 4 ? 0.3 : 0.2);
+    return cap > inner * sig ? cap : inner * sig;
 }
 
 // Land exactly on an event at fraction th of the step: one DP45 step of th*h from the step's

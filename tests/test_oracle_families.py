@@ -204,7 +204,8 @@ def _formal_volumetric(m, o, x, p, obs_norm, bins, n=40001):
         u = min(max((r - r_lo) / (0.1 * r_lo), 0), 1)
         v = min(max((o.vol_r_max - r) / (0.1 * o.vol_r_max), 0), 1)
         sig = o.vol_h_over_r * r
-        rho = o.vol_j0 / r ** 2 * np.exp(-y[3] ** 2 / (2 * sig ** 2)) * u * u * (3 - 2 * u) * v * v * (3 - 2 * v)
+        window = u ** 3 * (u * (6 * u - 15) + 10) * v ** 3 * (v * (6 * v - 15) + 10)  # quintic smoothstep
+        rho = o.vol_j0 / r ** 2 * np.exp(-y[3] ** 2 / (2 * sig ** 2)) * window
         rc = np.sqrt(max(y[1] ** 2 + y[2] ** 2 - a * a, 1e-12))
         Om = np.sqrt(rc) / (rc * rc + a * np.sqrt(rc))
         vel = np.array([1.0, -Om * y[2], Om * y[1], 0.0])
