@@ -143,8 +143,9 @@ typedef struct SiriusHipTraceOpts {
                               * tolerance there)                                             */
     double escalate_rtol;    /* relative tolerance of escalated FP64 re-traces (atol scales with
                               * it); they keep the FP32 path's step caps. 0 (default): the
-                              * path's rtol, except 1e-8 for the warp bubble, whose wall
-                              * crossings the FP32 tolerance does not control                  */
+                              * path's rtol, except 3e-10 for the warp bubble's wall crossers
+                              * and 1e-8 for the wormhole's near-throat rays, which the FP32
+                              * tolerance does not control                                     */
     double critical_band;    /* FP32 path, Kerr-Schild: rays within this distance (M) of the
                               * critical curve are flagged SIRIUS_FLAG_CRITICAL; the accuracy
                               * contract exempts them and the inner-edge escalation skips them */

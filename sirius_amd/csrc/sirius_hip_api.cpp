@@ -697,8 +697,18 @@ int run_trace(Handle& H, Batch& B, const SiriusHipTraceOpts& o, DevOut& dout, co
         t64.escalate = 0;
         t64.first_list = nullptr;  // the FP64 kernels trace only their own lists
         t64.first_count = nullptr;
-        const double ertol = o.escalate_rtol > 0 ? o.escalate_rtol
-                                                 : (B.family == SIRIUS_METRIC_ALCUBIERRE ? std::min(o.rtol, 1e-8) : 0.0);
+        // Default tolerances of the escalated rays (the path's step caps kept), from the 1024x1024
+        // scenes checked at every pixel against the plain FP64 frame: the warp bubble's wall
+        // crossers at 3e-10 (at 1e-8 or 1e-9 the worst missed by 1.2e-4 rad, while the plain FP64
+        // frame is itself converged to 1.6e-5 there; at 3e-10 every pixel is within 1.6e-5); the
+        // wormhole's near-throat rays at 1e-8 (at the path's 1e-6 the worst missed by 1.2e-3 rad;
+        // at 1e-8 by 6.3e-6). Kerr-Schild escalations keep the path's rtol.
+        double ertol = o.escalate_rtol;
+        if (!(ertol > 0)) {
+            if (B.family == SIRIUS_METRIC_ALCUBIERRE) ertol = std::min(o.rtol, 3e-10);
+            else if (B.family == SIRIUS_METRIC_MORRIS_THORNE) ertol = std::min(o.rtol, 1e-8);
+            else ertol = 0.0;
+        }
         if (ertol > 0) {
             t64.rtol = ertol;
             t64.atol = o.atol * (ertol / o.rtol);

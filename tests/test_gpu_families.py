@@ -64,6 +64,24 @@ def test_wormhole_and_warp_match_oracle(acc, scene):
         assert 0 < far.sum() < far.size  # both sides of the throat are seen
 
 
+@pytest.mark.parametrize("scene", ["wormhole", "warp"])
+def test_scene_frame_meets_the_bar_at_every_pixel(acc, scene):
+    """The 1024x1024 wormhole and warp scenes with the library's default options: the FP32 path
+    against the plain FP64 tracer at every pixel (classes, sides of the throat and escape
+    directions within 1e-4 rad). The plain FP64 tracer's own agreement with the oracle is pinned
+    at 96x96 above. The warp's wall crossers, re-traced in FP64, decide its maximum."""
+    from sirius_amd.accelerator import default_opts
+    cam, m = (wormhole_scene if scene == "wormhole" else warp_scene)(1024)
+    o64, o32 = default_opts(abi.PRECISION_FP64), default_opts(abi.PRECISION_FP32_ESCALATE)
+    o64.emission = o32.emission = abi.EMISSION_NONE
+    print(f"[case] {scene} 1024x1024 fp64 and fp32 frames, every pixel", flush=True)
+    ref, _ = acc.trace_batch([cam], [m], o64)
+    got, _ = acc.trace_batch([cam], [m], o32)
+    everything = np.ones(ref.cls.shape, bool)
+    ang, _ = _check(got, ref, everything, 1e-4, 1e-3)
+    print(f"[{scene} 1024] max escape-direction error {ang:.2e} rad", flush=True)
+
+
 def test_symplectic_matches_oracle_and_reports_null_drift(acc):
     """Fixed-step Tao + Yoshida-4 (h = 0.01 M) on Kerr a = 0.9: the device and the oracle run
     the same fixed-step arithmetic, so classes, directions and null drift agree closely."""
