@@ -53,7 +53,7 @@ FP32_VALU_PEAK_TFLOPS = 157.3
 FP64_VALU_PEAK_TFLOPS = 78.6
 # HBM bytes per launch of the FP32 bulk tracer on this frame, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a copy kernel of known size (profiles/trace_kerr_a0.9_1080p_counters.md)
-BULK_HBM_BYTES = 222.3e6
+BULK_HBM_BYTES = 221.0e6
 W, H = 1920, 1080
 
 
