@@ -224,6 +224,38 @@ def test_random_spacetime_batch_full_size(acc):
     assert rel.max() < 2e-3 and np.percentile(rel, 99) < 1e-3, (rel.max(), np.percentile(rel, 99))
 
 
+def test_random_spacetime_batch_fp32_against_fp64_every_ray(acc):
+    """The full 64-camera volumetric batch, every one of its 18.9 M rays: the FP32 path against
+    the plain FP64 tracer outside the critical band. Classes identical; intensity relative to the
+    pixel's brightest bin: 99 % within 1e-4 and 99.99 % within 2e-3, every ray within 5e-3. The
+    tail beyond the sampled test's 2e-3 is about 450 of 17.6 M lit rays, all inside the shadow
+    (critical distance -1.4 to -1.8 M: captured rays that cross the emitting layer on their way
+    in); the FP32 path's step inside the layer (0.2 or 0.3 sigma) barely changes it (2.9e-3 and
+    3.2e-3 at the worst ray). The FP64 tracer's own agreement with the oracle is pinned at 1/32 of
+    the sides below."""
+    from sirius_amd.accelerator import default_opts
+    cams, metrics = random_spacetime_batch()
+    o64, o32 = default_opts(abi.PRECISION_FP64), default_opts(abi.PRECISION_FP32_ESCALATE)
+    for o in (o64, o32):
+        o.emission = abi.EMISSION_VOLUMETRIC
+        o.n_bins = 8
+    print("[case] random batch 64 cameras fp64 and fp32, every ray", flush=True)
+    acc.trace_resident(cams, metrics, o64, stats=False)
+    ref = acc.fetch_results()
+    acc.trace_resident(cams, metrics, o32, stats=False)
+    got = acc.fetch_results()
+    keep = np.abs(acc.critical_distance(cams, metrics)) > 0.5
+    assert np.array_equal(got.cls[keep], ref.cls[keep])
+    sel = keep & (ref.intensity.max(0) > 0)
+    Ir, Ig = ref.intensity[:, sel], got.intensity[:, sel]
+    rel = np.abs(Ig - Ir).max(0) / np.maximum(Ir.max(0), 1e-2 * np.median(Ir.max(0)))
+    print(f"[batch every ray] {sel.sum()} lit rays, intensity error max {rel.max():.2e}, "
+          f"p99 {np.percentile(rel, 99):.2e}, p99.99 {np.percentile(rel, 99.99):.2e}, "
+          f"above 2e-3 {(rel > 2e-3).sum()}", flush=True)
+    assert np.percentile(rel, 99) < 1e-4 and np.percentile(rel, 99.99) < 2e-3 and rel.max() < 5e-3, \
+        (np.percentile(rel, 99), np.percentile(rel, 99.99), rel.max())
+
+
 def test_random_spacetime_batch_fp64_matches_oracle(acc):
     """The same batch at 1/32 of the image sides, every ray, plain FP64 against the oracle."""
     cams, metrics = random_spacetime_batch(scale=32)
