@@ -16,6 +16,7 @@
 // closed form eta - H l l (plus a Sherman-Morrison rank-1 term for Lambda), so no 4x4
 // inverse is ever formed on the device.
 #pragma once
+#include <cmath>
 #include <type_traits>
 
 #include <hip/hip_runtime.h>
@@ -57,6 +58,7 @@ struct TraceParams {
     double rtol, atol, h_init, h_min, h_max, far_slope;
     double r_escape;
     double x_min, x_max;
+    double bin_x0, bin_q;  // spectral bin centres: the first, and the ratio of neighbours (host-computed)
     double escalate_margin;
     double sym_h;
     double vol_j0, vol_h_over_r, vol_kappa, vol_r_min, vol_r_max, vol_tau_max;
@@ -936,10 +938,17 @@ SIRIUS_HD __forceinline__ T emitter_ut(const CamConst& c, const Spacetime<T>& s,
 }
 
 // Spectral bin centres x_k = x_min (x_max / x_min)^((k + 1/2) / n_bins): the first centre and the
-// ratio of neighbours (each centre is the previous one times the ratio).
+// ratio of neighbours (each centre is the previous one times the ratio). The host computes both
+// once per batch (bin_centres); kernels read them from the parameters instead of evaluating an
+// FP64 exp and log per ray or per volumetric step.
+inline void bin_centres(double x_min, double x_max, int n_bins, double& x0, double& q) {
+    if (n_bins <= 0) { q = 1.0; x0 = x_min; return; }
+    q = std::exp(std::log(x_max / x_min) / n_bins);
+    x0 = x_min * std::sqrt(q);
+}
 SIRIUS_HD __forceinline__ void bin_geometry(const TraceParams& tp, double& x0, double& q) {
-    q = exp(log(tp.x_max / tp.x_min) / tp.n_bins);
-    x0 = tp.x_min * sqrt(q);
+    q = tp.bin_q;
+    x0 = tp.bin_x0;
 }
 
 // Thin disk: g = (p.u_obs)/(p.u_emit), I_k = x_k^3 / (e^{x_k/(g tau)} - 1). The redshift and the

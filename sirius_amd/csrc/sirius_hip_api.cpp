@@ -454,6 +454,7 @@ int prepare_batch(Handle& H, const SiriusHipCamera* cams, int n_cam, const Siriu
     tp.first_count = nullptr;
     tp.max_steps = o.max_steps;
     tp.n_bins = o.n_bins;
+    bin_centres(o.x_min, o.x_max, o.n_bins, tp.bin_x0, tp.bin_q);
     tp.emission = B.family == SIRIUS_METRIC_KERR_SCHILD ? o.emission : SIRIUS_EMISSION_NONE;
     tp.sym_order = o.symplectic_order;
     tp.n_cam = n_cam;

@@ -673,8 +673,13 @@ __device__ __forceinline__ void persistent_body(const CamConst* __restrict__ cam
 // held to two waves: 268 -> 256 registers with 6 spilled, and the 1024x1024 warp frame goes from
 // 2.79 to 2.54 ms.
 constexpr int fp64_min_waves(int fam, bool multi) { return fam == 2 && !multi ? 2 : 1; }
+// The persistent tracers start on a 4 KiB boundary of the code object. Their loops are about
+// 19 KiB of instructions, and where the linker happened to place them changed their speed: the
+// FP32 thin-disk tracer, byte-identical in two builds, ran 1.06-1.16 ms per 1080p launch at one
+// address and 1.25-1.59 ms at another (an unrelated edit elsewhere in the file moved it), and
+// page-aligned it ran at the fast figure in both builds.
 template <class T, int FAM, int EMIT, bool MULTI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fp64_min_waves(FAM, MULTI))))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fp64_min_waves(FAM, MULTI)), aligned(4096)))
 void k_trace_persistent(SIRIUS_PERSISTENT_PARAMS) {
     persistent_body<T, FAM, EMIT, MULTI>(SIRIUS_PERSISTENT_ARGS);
 }
@@ -689,7 +694,7 @@ constexpr int fp32_min_waves(int emit, bool multi) {
     return emit == SIRIUS_EMISSION_THIN_DISK ? 3 : (emit == SIRIUS_EMISSION_VOLUMETRIC && !multi ? 2 : 1);
 }
 template <int FAM, int EMIT, bool MULTI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fp32_min_waves(EMIT, MULTI))))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fp32_min_waves(EMIT, MULTI)), aligned(4096)))
 void k_trace_persistent32(SIRIUS_PERSISTENT_PARAMS) {
     persistent_body<float, FAM, EMIT, MULTI>(SIRIUS_PERSISTENT_ARGS);
 }
